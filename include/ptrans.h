@@ -1,0 +1,219 @@
+/*
+ * ptrans.h -- C-ABI of libptrans, the gfx950 phonon Monte Carlo trace.
+ *
+ * Drop-in boundary.  In P-TRANS the trace is the Fortran module procedure
+ *   subroutine calcTransmit(trans, cbranch, ckpoint, angle)   (src/RTMCphon.f90:1608-2414)
+ * driven by
+ *   subroutine TransmissionFunction                            (src/RTMCphon.f90:1544-1604)
+ * whose OpenMP loop (RT:1573-1582) fills transmit(nbranch,nkpoint,nangle) and whose
+ * MPI_Allreduce (RT:1584-1590) sums it over ranks.  libptrans replaces the body of
+ * TransmissionFunction with one call, pt_trace_grid(), that runs every (k, b, angle)
+ * cell x Np particles on one GPU, one phonon path per lane.  The Fortran host keeps
+ * loadMCP (RT:252-1251), the .anphon reader (RT:661-733), effectiveMFP (RT:2418-2451)
+ * and fileoutput (RT:2601-2881); it hands over the geometry it built (vgrain, RT:122)
+ * flattened into the structures below, and the bulk phonon table (anphon, RT:174).
+ *
+ * Conventions
+ *  - Every function returns PT_OK (0) or a negative PT_E_* code; pt_last_error()
+ *    returns a message for the calling thread.  Nothing is printed.
+ *  - Host arrays passed in are copied at create time; handles own device memory.
+ *    Outputs are caller-allocated host arrays.
+ *  - Calls on one context are not re-entrant; each blocking call synchronises the
+ *    context's stream before it returns.
+ *  - Arrays indexed (branch, kpoint) are Fortran column-major: element (ib, ik),
+ *    1-based in the reference, is at [(ib-1) + nbranch*(ik-1)].
+ *  - Random numbers: Philox4x32-10, one stream per phonon path, keyed by the context
+ *    seed; results do not depend on the GPU, the launch shape or the order of paths.
+ */
+#ifndef PTRANS_H
+#define PTRANS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+/* status codes */
+#define PT_OK             0
+#define PT_E_INVALID     -1   /* bad argument */
+#define PT_E_HIP         -2   /* HIP runtime error */
+#define PT_E_NOMEM       -3   /* device or host allocation failed */
+#define PT_E_GEOMETRY    -4   /* inconsistent geometry descriptor */
+#define PT_E_UNSUPPORTED -5   /* feature not available in this build */
+#define PT_E_NODEVICE    -6   /* no usable GPU */
+
+/* surface shapes, src/RTMCphon.f90:72-74 (the cylinder axis is z, RT:3051-3085) */
+#define PT_SHAPE_POLYGON  1
+#define PT_SHAPE_SPHERE   2
+#define PT_SHAPE_CYLINDER 3
+
+/* transmissivity laws, src/RTMCphon.f90:77-79 and 1402-1429 */
+#define PT_TRANS_VALUE   1   /* t = trans_param                                    */
+#define PT_TRANS_GAMMA   2   /* t = 1/(gamma*omega/omega_max + 1), gamma=trans_param */
+#define PT_TRANS_TBC     3   /* as GAMMA; gamma was solved from a TBC by the host    */
+
+/* grain kinds, src/RTMCphon.f90:131-132 */
+#define PT_GRAIN_VORONOI 1
+#define PT_GRAIN_BRICKS  2
+
+/* arithmetic of the trace kernel */
+#define PT_PREC_FP64 0           /* every step in FP64 (the reference's arithmetic)      */
+
+/*
+ * One surface record: the fields of Surface_type (src/RTMCphon.f90:94-111) that the
+ * trace reads.  Polygon vertices live in pt_geom_desc.vertices; the closing vertex
+ * (vertex(:,nvertex+1) = vertex(:,1), RT:1093) is implicit.  normal, tangent and the
+ * polygon centre are the values the host's setupSurfaceGeometry produced (RT:1433-1491);
+ * pt_surface_setup_polygon() computes them the same way for hosts that need it.
+ */
+typedef struct pt_surface {
+    int32_t kind;            /* PT_SHAPE_*                                      */
+    int32_t nvertex;         /* polygon vertex count (>= 3); 0 otherwise        */
+    int64_t vertex_offset;   /* first vertex, in units of 3 doubles             */
+    int32_t internal;        /* surface%internal (only internal ones are hit)   */
+    int32_t incident;        /* surface%incident                                */
+    int32_t grain_inside;    /* surface%grain_inside                            */
+    int32_t grain_outside;   /* surface%grain_outside                           */
+    int32_t trans_kind;      /* PT_TRANS_*                                      */
+    int32_t reserved;
+    double  trans_param;     /* VALUE: t;  GAMMA/TBC: gamma                     */
+    double  specularity;     /* surface%specularity                             */
+    double  center[3];       /* surface%center                                  */
+    double  radius;          /* surface%radius                                  */
+    double  normal[3];       /* surface%normal (not renormalised, RT:1459)      */
+    double  tangent[9];      /* surface%tangent(3,3), column-major              */
+} pt_surface;
+
+/*
+ * The geometry: box, grains, surfaces.  Grain g (0-based, g = 0 is the box grain
+ * igrain_box, RT:123) owns surfaces [grain_offset[g], grain_offset[g+1]) in the order
+ * of vgrain(g)%vsurface; the collision scan visits grain 0 then the current grain in
+ * that order and keeps the first strictly nearest hit (RT:2012-2028).
+ */
+typedef struct pt_geom_desc {
+    double  box[3];              /* param%boxx, boxy, boxz (nm)                        */
+    double  scope[2];            /* param%scopey, scopez                               */
+    int32_t kind_grain;          /* PT_GRAIN_*                                         */
+    int32_t ngrain;              /* ngrain (grains 1..ngrain; bricks add fill grain)   */
+    int32_t igrain_fill;         /* ngrain+1 for bricks, -1 for voronoi                */
+    int32_t nmaterial;           /* param%nmaterial                                    */
+    int32_t ngrain_total;        /* entries of grain_offset minus one                  */
+    int32_t nsurface;            /* entries of surfaces                                */
+    int64_t nvertex;             /* entries of vertices / 3                            */
+    const int32_t    *grain_offset;   /* [ngrain_total+1]                              */
+    const int32_t    *grain_incident; /* [ngrain_total]  grain%incident                */
+    const double     *generatrix;     /* [ngrain_total*3] grain%generatrix             */
+    const pt_surface *surfaces;       /* [nsurface]                                    */
+    const double     *vertices;       /* [nvertex*3]                                   */
+} pt_geom_desc;
+
+/* Bulk phonon table (Anphon_type, src/RTMCphon.f90:161-174), column-major (nb, nk). */
+typedef struct pt_modes_desc {
+    int32_t nbranch;
+    int32_t nkpoint;
+    const double *omega;     /* rad/s, after the cm^-1 conversion of RT:727          */
+    const double *vel;       /* m/s                                                 */
+    const double *mfp;       /* m or nm, as the host uses it                        */
+    const double *weight;    /* multiplicity                                        */
+    double max_omega;        /* anphon%maxOmega (RT:728)                            */
+} pt_modes_desc;
+
+/* Counters filled by the trace calls. */
+typedef struct pt_stats {
+    int64_t paths;           /* paths traced                                        */
+    int64_t events;          /* event-loop iterations (RT:1693-1824), all paths      */
+    int64_t draws;           /* random numbers consumed                             */
+    int64_t lost;            /* paths that left the box (RT:1696-1701)               */
+    int64_t start_mismatch;  /* start-grain check failures (RT:1673-1682)            */
+    int64_t skipped;         /* particles not traced because of such a failure       */
+    double  kernel_ms;       /* device time of the trace kernels                     */
+} pt_stats;
+
+/* Per-path record of pt_replay(). */
+typedef struct pt_path_record {
+    int32_t end_side;        /* EndSide: 1 = x- (reflected), 2 = x+, 0 = left the box */
+    int32_t flags;           /* PT_PATH_* bits                                       */
+    int64_t events;          /* event-loop iterations (collision-1, RT:1659, 1705)   */
+    int64_t draws;           /* random numbers consumed by the path                  */
+    int32_t which_grain;     /* whichV at the end                                    */
+    int32_t start_grain;     /* whichV after the start-grain search                  */
+    double  pos[3];          /* final position (nm)                                  */
+    double  dir[3];          /* final direction                                      */
+} pt_path_record;
+
+#define PT_PATH_LOST           1   /* left the box: counted as transmitted (RT:1833) */
+#define PT_PATH_START_MISMATCH 2   /* start-grain check failed (RT:1673-1682)         */
+#define PT_PATH_EVENT_LIMIT    4   /* stopped at the event safety limit               */
+
+typedef struct pt_ctx   pt_ctx;
+typedef struct pt_geom  pt_geom;
+typedef struct pt_modes pt_modes;
+
+/* ---- library ---------------------------------------------------------------- */
+
+/* ABI version (PT_ABI_VERSION) and a human-readable build string. */
+int32_t     pt_version(void);
+const char *pt_build_info(void);
+
+/* Copies the calling thread's last error message into buf (always NUL-terminated).
+ * Returns the message length. */
+size_t pt_last_error(char *buf, size_t len);
+
+/* ---- context: one GPU, one HIP stream, one RNG seed ---------------------------- */
+/* Replaces MPI_Init/init_random_seed (main.f90:7-16, RT:3314-3330). */
+int pt_ctx_create(int device, uint64_t seed, pt_ctx **out);
+int pt_ctx_destroy(pt_ctx *ctx);
+int pt_ctx_set_seed(pt_ctx *ctx, uint64_t seed);
+
+/* ---- geometry ------------------------------------------------------------------ */
+/* Uploads vgrain (RT:122) as flattened by the host.  Validates indices. */
+int pt_geom_create(pt_ctx *ctx, const pt_geom_desc *desc, pt_geom **out);
+int pt_geom_destroy(pt_geom *geom);
+
+/*
+ * setupSurfaceGeometry for a polygon (src/RTMCphon.f90:1433-1491): centroid, the
+ * averaged (not renormalised) fan normal and the tangent frame, from nvertex >= 3
+ * vertices (open loop, xyz triples).  tangent is column-major 3x3.  Host-only helper.
+ */
+int pt_surface_setup_polygon(int32_t nvertex, const double *vertices,
+                             double center[3], double normal[3], double tangent[9]);
+
+/* ---- bulk phonon table ----------------------------------------------------------- */
+int pt_modes_create(pt_ctx *ctx, const pt_modes_desc *desc, pt_modes **out);
+int pt_modes_destroy(pt_modes *modes);
+
+/* ---- the trace ---------------------------------------------------------------------- */
+/*
+ * TransmissionFunction (src/RTMCphon.f90:1544-1604) for one sampling mean free path.
+ * For every cell (ik, ib, ia) with ia < nangle, traces np particles exactly as
+ * calcTransmit does and writes the transmitted COUNT to
+ *   counts[(ib-1) + nb*((ik-1) + nk*(ia-1))]
+ * (int64; the reference stores count/np as real(8), RT:1834-1838; counts for
+ * ia = nangle stay 0 as in RT:1576).  mfp_nm = 1e10 means ballistic (Matthiessen
+ * mode, RT:1686-1690).  vangle: nangle polar angles; NULL computes them as RT:1554-1556.
+ * Path p of cell c (c = ((ik-1)*nb + (ib-1))*(nangle-1) + (ia-1), the order of the
+ * reference's collapsed loop) uses Philox stream c*np + p.
+ * stats may be NULL.
+ */
+int pt_trace_grid(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
+                  double mfp_nm, int32_t nangle, const double *vangle, int64_t np,
+                  int32_t precision, int64_t *counts, pt_stats *stats);
+
+/*
+ * Replay of single paths for parity checks: traces paths first_path .. first_path+n-1
+ * of the pt_trace_grid numbering and records each one.  A path is traced even when an
+ * earlier particle of its cell failed the start-grain check (pt_trace_grid skips it).
+ */
+int pt_replay(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
+              double mfp_nm, int32_t nangle, const double *vangle, int64_t np,
+              int64_t first_path, int64_t n, pt_path_record *records);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTRANS_H */

@@ -1,0 +1,753 @@
+/*
+ * ptrans_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded restatement of P-TRANS's phonon Monte Carlo trace,
+ * used by the tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as
+ * the CHECKER of libptrans.  It is never linked into, loaded by or called from the
+ * product path.  Each function cites the reference code it follows
+ * (paths relative to the P-TRANS repository; RT = src/RTMCphon.f90).
+ *
+ * Arithmetic follows the reference operation by operation in FP64 (build with
+ * -ffp-contract=off): dot products and 3x3 products summed left to right, the REAL(4)
+ * literals 1.0e-7 (RT:1779) and 1e-10 (RT:3001) converted to double, strict
+ * comparisons exactly as written.
+ *
+ * Random numbers: draw d of path P is pt_philox_draw(seed, P, d) (Philox4x32-10), the
+ * stream the replay harness also feeds to the unmodified reference.
+ * Transcendentals: math = ORACLE_MATH_LIBM uses the C library's cos/sin/log (what the
+ * reference links by default); math = ORACLE_MATH_OWN uses pt_math.h, the functions
+ * the GPU kernel uses (and that the replay harness can link into the reference).
+ *
+ * Pins: see the tests -- golden per-path records and per-cell counts produced by the
+ * reference itself through the replay harness (oracle/replay), golden outputs of the
+ * reference's own geometry routines, and closed forms / invariants.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/ptrans.h"
+#include "../p-trans_amd/csrc/pt_math.h"
+#include "../p-trans_amd/csrc/pt_philox.h"
+
+#define ORACLE_MATH_LIBM 0
+#define ORACLE_MATH_OWN  1
+
+/* RT:62-64 */
+static const double O_PI = 3.14159265358979323846;
+#define O_TWOPI (O_PI * 2.0)
+#define O_PI_2 (O_PI * 0.5)
+static const double O_SMALL_VAL = 1.0e-5;            /* RT:69 */
+#define O_NUDGE ((double)1.0e-7f)                    /* REAL(4) literal, RT:1779 */
+#define O_SPHERE_EPS ((double)1e-10f)                /* REAL(4) literal, RT:3001 */
+#define O_EVENT_LIMIT 100000000LL                    /* safety stop, not in the reference */
+
+typedef struct {
+    const pt_geom_desc *g;
+    const pt_modes_desc *m;
+    double mfp;
+    int math;
+    uint64_t seed;
+    uint64_t path;
+    int64_t draw;
+    int cbranch, ckpoint;           /* 1-based, as calcTransmit's arguments */
+    /* optional logs: positions as trace.out writes them (RT:1661-1665, 1818-1822) and
+     * the reference source line of every random number */
+    double *tlog; int64_t tcap, tn;
+    int16_t *slog; int64_t scap, sn;
+} ostate;
+
+/* site = the line of the RANDOM_NUMBER call in src/RTMCphon.f90 that this draw mirrors */
+static double o_rand(ostate *s, int site)
+{
+    double u = pt_philox_draw(s->seed, s->path, (uint64_t)s->draw);
+    s->draw++;
+    if (s->slog && s->sn < s->scap) s->slog[s->sn] = (int16_t)site;
+    if (s->slog) s->sn++;
+    return u;
+}
+
+static void o_log_pos(ostate *s, const double *p)
+{
+    if (s->tlog && s->tn < s->tcap) {
+        s->tlog[3 * s->tn] = p[0]; s->tlog[3 * s->tn + 1] = p[1]; s->tlog[3 * s->tn + 2] = p[2];
+    }
+    if (s->tlog) s->tn++;
+}
+
+static void o_sincos(const ostate *s, double x, double *sn, double *cs)
+{
+    if (s->math == ORACLE_MATH_OWN) {
+        pt_sincos(x, sn, cs);
+    } else {
+        *sn = sin(x);
+        *cs = cos(x);
+    }
+}
+
+static double o_log(const ostate *s, double x)
+{
+    return s->math == ORACLE_MATH_OWN ? pt_log(x) : log(x);
+}
+
+static double dot3(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* cross_product, RT:185-193 */
+static void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* ---------------------------------------------------------------- geometry setup */
+
+/* setupSurfaceGeometry, polygon branch, RT:1433-1487 */
+int oracle_setup_polygon(int nv, const double *v, double *center, double *normal, double *tangent)
+{
+    int iv, k;
+    double a[3], b[3], c[3], len, inv;
+#define V(i) (v + 3 * (((i) - 1) % nv))   /* 1-based with the closing vertex v(nv+1)=v(1) */
+    if (nv < 3) return -1;
+    center[0] = center[1] = center[2] = 0.0;
+    for (iv = 1; iv <= nv; ++iv)
+        for (k = 0; k < 3; ++k) center[k] = center[k] + V(iv)[k];
+    inv = 1.0 / nv;
+    for (k = 0; k < 3; ++k) center[k] = inv * center[k];
+
+    normal[0] = normal[1] = normal[2] = 0.0;
+    for (iv = 1; iv <= nv - 1; ++iv) {
+        for (k = 0; k < 3; ++k) { a[k] = V(iv + 1)[k] - V(iv)[k]; b[k] = V(iv + 2)[k] - V(iv)[k]; }
+        cross3(a, b, c);
+        len = sqrt(dot3(c, c));
+        for (k = 0; k < 3; ++k) normal[k] = normal[k] + c[k] / len;
+    }
+    for (k = 0; k < 3; ++k) { a[k] = V(nv + 1)[k] - V(nv)[k]; b[k] = V(2)[k] - V(nv)[k]; }
+    cross3(a, b, c);
+    len = sqrt(dot3(c, c));
+    for (k = 0; k < 3; ++k) normal[k] = normal[k] + c[k] / len;
+    for (k = 0; k < 3; ++k) normal[k] = inv * normal[k];
+#undef V
+    /* tangent(:,3) = normal; columns 2 and 1, RT:1466-1486 */
+    tangent[6] = normal[0]; tangent[7] = normal[1]; tangent[8] = normal[2];
+    if (fabs(normal[2]) == 1.0 ||
+        (normal[0] * normal[0] == 0.0 && normal[1] * normal[1] == 0.0)) {
+        tangent[3] = 0.0; tangent[4] = 1.0; tangent[5] = 0.0;
+        tangent[0] = 1.0; tangent[1] = 0.0; tangent[2] = 0.0;
+    } else {
+        double r = sqrt(normal[0] * normal[0] + normal[1] * normal[1]);
+        tangent[3] = -normal[1] / r;
+        tangent[4] = normal[0] / r;
+        tangent[5] = 0.0;
+        cross3(tangent + 3, tangent + 6, tangent);
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- intersections */
+
+/* calcSurfaceDistancePolygon, RT:2907-2959 */
+int oracle_dist_polygon(const double *rs, const double *nd, const double *v, int nv,
+                        const double *ns, double *lhit)
+{
+    double cosang, d[3], rhit[3], ra[3], rb[3], ci[3], c1[3];
+    int i, k;
+    *lhit = 0.0;
+    cosang = dot3(nd, ns);
+    if (cosang <= 0.0) return 0;
+    for (k = 0; k < 3; ++k) d[k] = v[k] - rs[k];
+    *lhit = dot3(d, ns) / cosang;
+    if (*lhit < 0.0) return 0;
+    for (k = 0; k < 3; ++k) rhit[k] = rs[k] + nd[k] * *lhit;
+    for (i = 0; i < nv; ++i) {
+        const double *vi = v + 3 * i;
+        const double *vn = v + 3 * ((i + 1) % nv);
+        for (k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = rhit[k] - vi[k]; }
+        cross3(ra, rb, ci);
+        if (i == 0) {
+            c1[0] = ci[0]; c1[1] = ci[1]; c1[2] = ci[2];
+        } else if (dot3(c1, ci) < 0.0) {
+            return 0;
+        }
+    }
+    return 1;
+}
+
+/* isInsidePolygon, RT:2961-2983 */
+int oracle_inside_polygon(const double *v, int nv, const double *p)
+{
+    double ra[3], rb[3], ci[3], c1[3];
+    int i, k;
+    for (i = 0; i < nv; ++i) {
+        const double *vi = v + 3 * i;
+        const double *vn = v + 3 * ((i + 1) % nv);
+        for (k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = p[k] - vi[k]; }
+        cross3(ra, rb, ci);
+        if (i == 0) {
+            c1[0] = ci[0]; c1[1] = ci[1]; c1[2] = ci[2];
+        } else if (dot3(c1, ci) < 0.0) {
+            return 0;
+        }
+    }
+    return 1;
+}
+
+/* calcSurfaceDistanceSphere, RT:2985-3038 (the residual print of RT:3026-3036 has no effect) */
+int oracle_dist_sphere(const double *rs, const double *nd, const double *c, double r, double *lhit)
+{
+    double rsc[3], rd, rr, d, l1, l2, sq;
+    int k;
+    for (k = 0; k < 3; ++k) rsc[k] = c[k] - rs[k];
+    rd = dot3(rsc, nd);
+    rr = dot3(rsc, rsc);
+    if (rr > r * r - O_SPHERE_EPS) {
+        if (rd <= 0.0) return 0;
+    }
+    d = rd * rd + r * r - rr;
+    if (d < 0.0) return 0;
+    sq = sqrt(d);
+    l1 = rd - sq;
+    l2 = rd + sq;
+    if (l1 > 0.0) *lhit = l1;
+    else if (l2 > 0.0) *lhit = l2;
+    else return 0;
+    return 1;
+}
+
+/* calcSurfaceDistanceCylinder, RT:3040-3088 (axis z, clipped to 0 <= z <= boxz) */
+int oracle_dist_cylinder(const double *rs, const double *nd, const double *c, double r,
+                         double boxz, double *lhit)
+{
+    double rsc0, rsc1, t[3], nrm, sn, rd, rr, d, l1, l2, sq, zh;
+    rsc0 = c[0] - rs[0];
+    rsc1 = c[1] - rs[1];
+    t[0] = nd[0]; t[1] = nd[1]; t[2] = 0.0;
+    nrm = dot3(t, t);
+    sn = sqrt(nrm);
+    t[0] = t[0] / sn; t[1] = t[1] / sn;
+    rd = rsc0 * t[0] + rsc1 * t[1];
+    rr = rsc0 * rsc0 + rsc1 * rsc1;
+    d = rd * rd + r * r - rr;
+    if (d < 0.0) return 0;
+    sq = sqrt(d);
+    l1 = rd - sq;
+    l2 = rd + sq;
+    if (l1 > 0.0) *lhit = l1;
+    else if (l2 > 0.0) *lhit = l2;
+    else return 0;
+    *lhit = *lhit / sqrt(nd[0] * nd[0] + nd[1] * nd[1]);
+    zh = rs[2] + nd[2] * *lhit;
+    if (zh < 0.0 || boxz < zh) return 0;
+    return 1;
+}
+
+static int o_surface_distance(const pt_geom_desc *g, const pt_surface *sf, const double *p,
+                              const double *d, double *l)
+{
+    switch (sf->kind) {
+    case PT_SHAPE_POLYGON:
+        return oracle_dist_polygon(p, d, g->vertices + 3 * sf->vertex_offset, sf->nvertex,
+                                   sf->normal, l);
+    case PT_SHAPE_SPHERE:
+        return oracle_dist_sphere(p, d, sf->center, sf->radius, l);
+    case PT_SHAPE_CYLINDER:
+        return oracle_dist_cylinder(p, d, sf->center, sf->radius, g->box[2], l);
+    }
+    return 0;
+}
+
+/* calcSurfaceNormal{Polygon,Sphere,Cylinder}, RT:3119-3179 */
+void oracle_surface_normal(int kind, const double *pos, const double *center, double radius,
+                           const double *pnormal, double *n)
+{
+    double rcs[3], dd, inv;
+    int k;
+    if (kind == PT_SHAPE_POLYGON) {
+        n[0] = pnormal[0]; n[1] = pnormal[1]; n[2] = pnormal[2];
+        return;
+    }
+    for (k = 0; k < 3; ++k) rcs[k] = pos[k] - center[k];
+    if (kind == PT_SHAPE_CYLINDER) rcs[2] = 0.0;
+    dd = dot3(rcs, rcs);
+    inv = 1.0 / sqrt(dd);
+    for (k = 0; k < 3; ++k) n[k] = rcs[k] * inv;
+    if (dd > radius * radius)
+        for (k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
+}
+
+/* calcSurfaceRotation{Polygon,Sphere,Cylinder}, RT:3181-3301; rot column-major 3x3 */
+void oracle_surface_rotation(int kind, const double *pos, const double *center,
+                             const double *ptangent, double *rot)
+{
+    double a[3], dd, inv, r;
+    int k, axis_test;
+    if (kind == PT_SHAPE_POLYGON) {
+        memcpy(rot, ptangent, 9 * sizeof(double));
+        return;
+    }
+    for (k = 0; k < 3; ++k) a[k] = pos[k] - center[k];
+    if (kind == PT_SHAPE_CYLINDER) a[2] = 0.0;
+    dd = dot3(a, a);
+    inv = 1.0 / sqrt(dd);
+    for (k = 0; k < 3; ++k) a[k] = a[k] * inv;
+    rot[6] = a[0]; rot[7] = a[1]; rot[8] = a[2];
+    /* the sphere tests |n_z| == 1 (RT:3226), the cylinder |n_x| == 1 (RT:3283) */
+    axis_test = (kind == PT_SHAPE_SPHERE) ? (fabs(a[2]) == 1.0) : (fabs(a[0]) == 1.0);
+    if (axis_test || (a[0] * a[0] == 0.0 && a[1] * a[1] == 0.0)) {
+        rot[3] = 0.0; rot[4] = 1.0; rot[5] = 0.0;
+        rot[0] = 1.0; rot[1] = 0.0; rot[2] = 0.0;
+    } else {
+        r = sqrt(a[0] * a[0] + a[1] * a[1]);
+        rot[3] = -a[1] / r;
+        rot[4] = a[0] / r;
+        rot[5] = 0.0;
+        cross3(rot + 3, rot + 6, rot);
+    }
+}
+
+/* ---------------------------------------------------------------- samplers (RT:1845-2221) */
+
+/* determinTransmit, RT:2384-2413.  rand < 0 means "not present". */
+static int o_determin_transmit(ostate *s, const pt_surface *sf, double rand_in, int present)
+{
+    double threshold, value, om;
+    switch (sf->trans_kind) {
+    case PT_TRANS_VALUE:
+        threshold = sf->trans_param;
+        break;
+    case PT_TRANS_GAMMA:
+    case PT_TRANS_TBC:
+        if (s->g->nmaterial == 1) {
+            /* setupSurfaceTransmissivity, RT:1414-1421 */
+            double w = s->m->omega[(s->cbranch - 1) + (int64_t)s->m->nbranch * (s->ckpoint - 1)];
+            threshold = 1.0 / (sf->trans_param * w / s->m->max_omega + 1.0);
+        } else {
+            double t11 = 1.0 / (sf->trans_param + 1.0);   /* RT:1423 */
+            om = o_rand(s, 2399);                          /* RT:2399 */
+            threshold = t11 + om * (1 - t11);
+        }
+        break;
+    default:
+        threshold = 0.0;
+    }
+    value = present ? rand_in : o_rand(s, 2410);           /* RT:2410 */
+    return value <= threshold;
+}
+
+/* findInsideWhichGrain / isInsideGrain / isInsideSurface, RT:1874-1985 */
+static int o_inside_surface(const pt_surface *sf, const double *p)
+{
+    double d[3];
+    int k;
+    for (k = 0; k < 3; ++k) d[k] = p[k] - sf->center[k];
+    if (sf->kind == PT_SHAPE_POLYGON) return dot3(d, sf->normal) < 0;
+    if (sf->kind == PT_SHAPE_SPHERE) return dot3(d, d) < sf->radius * sf->radius;
+    if (sf->kind == PT_SHAPE_CYLINDER) return d[0] * d[0] + d[1] * d[1] < sf->radius * sf->radius;
+    return 0;
+}
+
+static int o_inside_grain(const pt_geom_desc *g, int ig, const double *p)
+{
+    int is;
+    for (is = g->grain_offset[ig]; is < g->grain_offset[ig + 1]; ++is) {
+        const pt_surface *sf = g->surfaces + is;
+        if (!sf->internal) continue;
+        if (sf->incident) continue;
+        if (!o_inside_surface(sf, p)) return 0;
+    }
+    return 1;
+}
+
+static int o_find_inside_which_grain(const pt_geom_desc *g, const double *p)
+{
+    int ig, count = 0, which = -1;
+    for (ig = 1; ig <= g->ngrain; ++ig) {
+        if (o_inside_grain(g, ig, p)) { which = ig; count++; }
+    }
+    if (count == 1) return which;
+    if (count == 0 && g->kind_grain == PT_GRAIN_BRICKS) return g->ngrain + 1;
+    if (count == 0) return -1;            /* voronoi: RT:1937-1957 */
+    return which;                         /* count >= 2: the last one, RT:1958-1979 */
+}
+
+/* findNearestIncidentGrain, RT:2100-2118 (first strict minimum; undefined if none) */
+static int o_nearest_incident_grain(const pt_geom_desc *g, const double *p)
+{
+    double d2min = 1.0e10, d2, d[3];
+    int ig, k, best = 0;
+    for (ig = 1; ig <= g->ngrain; ++ig) {
+        if (!g->grain_incident[ig]) continue;
+        for (k = 0; k < 3; ++k) d[k] = p[k] - g->generatrix[3 * ig + k];
+        d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (d2min > d2) { d2min = d2; best = ig; }
+    }
+    return best;
+}
+
+/* findIncidentCollisionGrain, RT:2131-2163.  The EXIT at RT:2153 leaves only the
+ * surface loop, so the LAST grain with an incident facet containing p wins. */
+static int o_incident_collision_grain(const pt_geom_desc *g, const double *p)
+{
+    int ig, is, found = -1;
+    for (ig = 1; ig <= g->ngrain; ++ig) {
+        if (!g->grain_incident[ig]) continue;
+        for (is = g->grain_offset[ig]; is < g->grain_offset[ig + 1]; ++is) {
+            const pt_surface *sf = g->surfaces + is;
+            if (!sf->incident) continue;
+            if (oracle_inside_polygon(g->vertices + 3 * sf->vertex_offset, sf->nvertex, p)) {
+                found = ig;
+                break;
+            }
+        }
+    }
+    return found >= 0 ? found : g->igrain_fill;
+}
+
+/* ---------------------------------------------------------------- one particle */
+
+/*
+ * One iteration of calcTransmit's particle loop (RT:1632-1835) for path `path`
+ * in cell (cbranch, ckpoint, angle index ia with polar angle theta).
+ * Returns 1 if the start-grain check failed (the reference then EXITs the particle
+ * loop, RT:1681), else 0.
+ */
+static int o_trace_particle(ostate *s, double cos_t, double sin_t, pt_path_record *rec)
+{
+    const pt_geom_desc *g = s->g;
+    double pos[3], dir[3], distance = 0.0, freepath, phi, sp, cp;
+    int end_side, igrain, isurface, which, tmpwhich, k;
+    int64_t events = 0;
+    int flags = 0;
+
+    s->draw = 0;
+    /* createRandomPosition, RT:1845-1856 */
+    {
+        double y = o_rand(s, 1850), z = o_rand(s, 1851);
+        pos[0] = 0.0;
+        pos[1] = y * g->box[1] * g->scope[0];
+        pos[2] = z * g->box[2] * g->scope[1];
+    }
+    /* createRandomDirectionPhi, RT:1860-1871 */
+    phi = o_rand(s, 1866);
+    phi = phi * O_TWOPI;
+    o_sincos(s, phi, &sp, &cp);
+    dir[0] = cos_t; dir[1] = sin_t * cp; dir[2] = sin_t * sp;
+
+    o_log_pos(s, pos);                                   /* RT:1661-1665 */
+    /* StartSide = 1 since Start_Position(1) == 0 (RT:1635-1641); RT:1644-1652 */
+    isurface = 1;
+    end_side = (dir[0] <= 0.0) ? 1 : 0;
+    igrain = 0;
+    which = 0;
+    if (g->ngrain > 0 && g->kind_grain == PT_GRAIN_VORONOI)
+        which = o_nearest_incident_grain(g, pos);
+    else if (g->ngrain > 0 && g->kind_grain == PT_GRAIN_BRICKS)
+        which = o_incident_collision_grain(g, pos);
+    rec->start_grain = which;
+
+    /* verbose=1 consistency check (main.f90:14, RT:1673-1682) */
+    tmpwhich = o_find_inside_which_grain(g, pos);
+    if (tmpwhich != which) {
+        rec->end_side = end_side;
+        rec->flags = PT_PATH_START_MISMATCH;
+        rec->events = 0;
+        rec->draws = s->draw;
+        rec->which_grain = which;
+        memcpy(rec->pos, pos, sizeof pos);
+        memcpy(rec->dir, dir, sizeof dir);
+        return 1;
+    }
+
+    /* initial FreePath, RT:1686-1690 and createRandomFreePath RT:2167-2173 */
+    if (s->mfp == 1.0e10) freepath = 1.0e10;
+    else freepath = -(s->mfp * o_log(s, o_rand(s, 2171)));
+
+    while (end_side == 0) {
+        int ig_pass, jreflect;
+        double dmin;
+        int isurf_min, igrain_min, hitflag;
+        /* isInsideBox, RT:1696-1703 and 2177-2187 */
+        if (!(pos[0] >= 0.0 - O_SMALL_VAL && pos[0] <= g->box[0] + O_SMALL_VAL &&
+              pos[1] >= 0.0 - O_SMALL_VAL && pos[1] <= g->box[1] + O_SMALL_VAL &&
+              pos[2] >= 0.0 - O_SMALL_VAL && pos[2] <= g->box[2] + O_SMALL_VAL)) {
+            flags |= PT_PATH_LOST;
+            break;
+        }
+        if (events >= O_EVENT_LIMIT) { flags |= PT_PATH_EVENT_LIMIT; break; }
+        events++;
+
+        /* findCollisionSurface, RT:1989-2096: grains 0 then whichV, first strict minimum */
+        dmin = 1.0e10;
+        isurf_min = isurface;
+        igrain_min = igrain;
+        hitflag = 0;
+        for (ig_pass = 0; ig_pass < 2; ++ig_pass) {
+            int ig = ig_pass == 0 ? 0 : which;
+            int is;
+            for (is = g->grain_offset[ig]; is < g->grain_offset[ig + 1]; ++is) {
+                const pt_surface *sf = g->surfaces + is;
+                double l;
+                if (!sf->internal) continue;
+                if (o_surface_distance(g, sf, pos, dir, &l) && dmin > l) {
+                    hitflag = 1;
+                    dmin = l;
+                    isurf_min = is - g->grain_offset[ig] + 1;
+                    igrain_min = ig;
+                }
+            }
+        }
+        (void)hitflag;
+        distance = dmin;
+        igrain = igrain_min;
+        isurface = isurf_min;
+
+        jreflect = 1;
+        if (freepath < distance) {
+            /* internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204 */
+            double ct, st, ph, sph, cph;
+            distance = freepath;
+            for (k = 0; k < 3; ++k) pos[k] = pos[k] + distance * dir[k];
+            ct = o_rand(s, 2197);
+            ph = o_rand(s, 2198);
+            ct = ct * 2.0 - 1.0;
+            ph = ph * O_TWOPI;
+            st = sqrt(1.0 - ct * ct);
+            o_sincos(s, ph, &sph, &cph);
+            dir[0] = ct; dir[1] = st * cph; dir[2] = st * sph;
+            isurface = 0;
+            igrain = 0;
+        } else {
+            for (k = 0; k < 3; ++k) pos[k] = pos[k] + distance * dir[k];
+            if (igrain == 0 && isurface == 1) {
+                end_side = 1;
+            } else if (igrain == 0 && isurface == 2) {
+                end_side = 2;
+            } else {
+                const pt_surface *sf = g->surfaces + g->grain_offset[igrain] + (isurface - 1);
+                double spec = o_rand(s, 2230);                    /* determinSpecular RT:2230 */
+                if (spec <= sf->specularity) {
+                    /* reflectSpecular, RT:2275-2328 */
+                    if (o_determin_transmit(s, sf, 0.0, 0)) {
+                        jreflect = 0;
+                    } else {
+                        double n[3], dd;
+                        oracle_surface_normal(sf->kind, pos, sf->center, sf->radius, sf->normal, n);
+                        if (dot3(n, dir) < 0)
+                            for (k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
+                        dd = dot3(n, dir);
+                        for (k = 0; k < 3; ++k) dir[k] = dir[k] - 2.0 * n[k] * dd;
+                        jreflect = 1;
+                    }
+                } else {
+                    /* reflectDiffusive, RT:2332-2380 */
+                    double n[3], rot[9], loc[3], st, ct, ph, sph, cph, u, md[3];
+                    oracle_surface_normal(sf->kind, pos, sf->center, sf->radius, sf->normal, n);
+                    if (dot3(n, dir) < 0.0)
+                        for (k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
+                    oracle_surface_rotation(sf->kind, pos, sf->center, sf->tangent, rot);
+                    /* createRandomDirectionSemiSphere, RT:2208-2221 */
+                    st = o_rand(s, 2213);
+                    st = sqrt(st);
+                    ct = sqrt(1.0 - st * st);
+                    ph = o_rand(s, 2218);
+                    ph = ph * O_TWOPI;
+                    o_sincos(s, ph, &sph, &cph);
+                    loc[0] = st * cph; loc[1] = st * sph; loc[2] = ct;
+                    /* matmul(tmpRotation, tmpDirection), j = 1..3 in order */
+                    for (k = 0; k < 3; ++k)
+                        md[k] = rot[k] * loc[0] + rot[3 + k] * loc[1] + rot[6 + k] * loc[2];
+                    if (dot3(dir, n) > 0.0)
+                        for (k = 0; k < 3; ++k) dir[k] = -md[k];
+                    else
+                        for (k = 0; k < 3; ++k) dir[k] = md[k];
+                    if (dot3(dir, n) > 0.0)
+                        for (k = 0; k < 3; ++k) dir[k] = -1.0 * dir[k];
+                    u = o_rand(s, 2361);                            /* RT:2361 */
+                    if (o_determin_transmit(s, sf, u, 1)) {
+                        for (k = 0; k < 3; ++k) dir[k] = -dir[k];
+                        jreflect = 0;
+                    } else {
+                        jreflect = 1;
+                    }
+                }
+            }
+        }
+        /* nudge, RT:1775-1780 */
+        for (k = 0; k < 3; ++k) pos[k] = pos[k] + O_NUDGE * dir[k];
+
+        /* FreePath update, RT:1783-1790 */
+        if (s->mfp == 1.0e10) {
+            freepath = 1.0e10;
+        } else {
+            freepath = freepath - distance;
+            if (freepath < 1.0e-10) freepath = -(s->mfp * o_log(s, o_rand(s, 2171)));
+        }
+        /* grain update on transmission, RT:1793-1796 */
+        if (jreflect == 0) {
+            const pt_surface *sf = g->surfaces + g->grain_offset[igrain] + (isurface - 1);
+            which = sf->grain_outside;
+            if (which <= 0) {
+                /* whichV = 0 makes the reference's DO ig=0,whichV,whichV a zero-step loop
+                 * (undefined); stop the path as lost. */
+                flags |= PT_PATH_LOST;
+                end_side = 0;
+                o_log_pos(s, pos);
+                break;
+            }
+        }
+        o_log_pos(s, pos);                               /* RT:1818-1822 */
+    }
+    rec->end_side = end_side;
+    rec->flags = flags;
+    rec->events = events;
+    rec->draws = s->draw;
+    rec->which_grain = which;
+    memcpy(rec->pos, pos, sizeof pos);
+    memcpy(rec->dir, dir, sizeof dir);
+    return 0;
+}
+
+/* param%vangle, RT:1554-1556 */
+static void o_angles(int nangle, const double *vangle_in, double *va)
+{
+    int ia;
+    for (ia = 1; ia <= nangle; ++ia)
+        va[ia - 1] = vangle_in ? vangle_in[ia - 1]
+                               : 0.0 + (double)(ia - 1) / (double)(nangle - 1) * O_PI_2;
+}
+
+static void o_cell_of_path(int64_t path, int64_t np, int nb, int nangle,
+                           int *ib, int *ik, int *ia)
+{
+    int64_t cell = path / np;
+    int64_t na = nangle - 1;
+    *ia = (int)(cell % na) + 1;
+    cell /= na;
+    *ib = (int)(cell % nb) + 1;
+    *ik = (int)(cell / nb) + 1;
+}
+
+/*
+ * TransmissionFunction + calcTransmit (RT:1544-1604, 1608-1841) for one sampling MFP,
+ * single-threaded in the order of the collapsed loop.  counts[(ib-1)+nb*((ik-1)+nk*(ia-1))]
+ * receives the transmitted count.  stats[0..5] = paths, events, draws, lost,
+ * start-mismatch, skipped.  Cells [cell_begin, cell_end) only (cell_end < 0: all).
+ */
+int oracle_trace_grid(const pt_geom_desc *g, const pt_modes_desc *m, double mfp, int nangle,
+                      const double *vangle, int64_t np, uint64_t seed, int math,
+                      int64_t cell_begin, int64_t cell_end, int64_t *counts, int64_t *stats)
+{
+    ostate s;
+    double *va;
+    int64_t ncell = (int64_t)m->nkpoint * m->nbranch * (nangle - 1), c, p;
+    if (nangle < 2 || np < 0) return -1;
+    va = (double *)malloc(sizeof(double) * nangle);
+    o_angles(nangle, vangle, va);
+    memset(&s, 0, sizeof s);
+    s.g = g; s.m = m; s.mfp = mfp; s.math = math; s.seed = seed;
+    if (cell_end < 0 || cell_end > ncell) cell_end = ncell;
+    for (c = cell_begin; c < cell_end; ++c) {
+        int ib, ik, ia;
+        double ct, st;
+        int64_t trans = 0;
+        o_cell_of_path(c * (np > 0 ? np : 1), np > 0 ? np : 1, m->nbranch, nangle, &ib, &ik, &ia);
+        s.cbranch = ib; s.ckpoint = ik;
+        if (math == ORACLE_MATH_OWN) pt_sincos(va[ia - 1], &st, &ct);
+        else { ct = cos(va[ia - 1]); st = sin(va[ia - 1]); }
+        for (p = 0; p < np; ++p) {
+            pt_path_record r;
+            s.path = (uint64_t)(c * np + p);
+            if (o_trace_particle(&s, ct, st, &r)) {
+                if (stats) { stats[4]++; stats[5] += np - p - 1; stats[2] += r.draws; stats[0]++; }
+                break;   /* EXIT of the particle loop, RT:1681 */
+            }
+            if (stats) {
+                stats[0]++; stats[1] += r.events; stats[2] += r.draws;
+                if (r.flags & PT_PATH_LOST) stats[3]++;
+            }
+            if (r.end_side != 1) trans++;     /* StartSide /= EndSide, RT:1833-1835 */
+        }
+        counts[(ib - 1) + (int64_t)m->nbranch * ((ik - 1) + (int64_t)m->nkpoint * (ia - 1))] = trans;
+    }
+    free(va);
+    return 0;
+}
+
+/* Single-path replay in the numbering of oracle_trace_grid. */
+int oracle_replay(const pt_geom_desc *g, const pt_modes_desc *m, double mfp, int nangle,
+                  const double *vangle, int64_t np, uint64_t seed, int math,
+                  int64_t first_path, int64_t n, pt_path_record *out)
+{
+    ostate s;
+    double *va;
+    int64_t i;
+    if (nangle < 2 || np <= 0) return -1;
+    va = (double *)malloc(sizeof(double) * nangle);
+    o_angles(nangle, vangle, va);
+    memset(&s, 0, sizeof s);
+    s.g = g; s.m = m; s.mfp = mfp; s.math = math; s.seed = seed;
+    for (i = 0; i < n; ++i) {
+        int ib, ik, ia;
+        double ct, st;
+        int64_t path = first_path + i;
+        o_cell_of_path(path, np, m->nbranch, nangle, &ib, &ik, &ia);
+        s.cbranch = ib; s.ckpoint = ik; s.path = (uint64_t)path;
+        if (math == ORACLE_MATH_OWN) pt_sincos(va[ia - 1], &st, &ct);
+        else { ct = cos(va[ia - 1]); st = sin(va[ia - 1]); }
+        memset(out + i, 0, sizeof *out);
+        o_trace_particle(&s, ct, st, out + i);
+    }
+    free(va);
+    return 0;
+}
+
+/*
+ * oracle_replay with logs: positions (3 doubles each) into traj, the draw sites into
+ * sites; traj_off / site_off (n+1 entries) delimit each path.  Returns -2 if a buffer
+ * is too small.
+ */
+int oracle_replay_log(const pt_geom_desc *g, const pt_modes_desc *m, double mfp, int nangle,
+                      const double *vangle, int64_t np, uint64_t seed, int math,
+                      int64_t first_path, int64_t n, pt_path_record *out,
+                      double *traj, int64_t traj_cap, int64_t *traj_off,
+                      int16_t *sites, int64_t site_cap, int64_t *site_off)
+{
+    ostate s;
+    double *va;
+    int64_t i;
+    if (nangle < 2 || np <= 0) return -1;
+    va = (double *)malloc(sizeof(double) * nangle);
+    o_angles(nangle, vangle, va);
+    memset(&s, 0, sizeof s);
+    s.g = g; s.m = m; s.mfp = mfp; s.math = math; s.seed = seed;
+    s.tlog = traj; s.tcap = traj_cap; s.slog = sites; s.scap = site_cap;
+    traj_off[0] = 0;
+    site_off[0] = 0;
+    for (i = 0; i < n; ++i) {
+        int ib, ik, ia;
+        double ct, st;
+        int64_t path = first_path + i;
+        o_cell_of_path(path, np, m->nbranch, nangle, &ib, &ik, &ia);
+        s.cbranch = ib; s.ckpoint = ik; s.path = (uint64_t)path;
+        if (math == ORACLE_MATH_OWN) pt_sincos(va[ia - 1], &st, &ct);
+        else { ct = cos(va[ia - 1]); st = sin(va[ia - 1]); }
+        memset(out + i, 0, sizeof *out);
+        o_trace_particle(&s, ct, st, out + i);
+        traj_off[i + 1] = s.tn;
+        site_off[i + 1] = s.sn;
+    }
+    free(va);
+    return (s.tn > traj_cap || s.sn > site_cap) ? -2 : 0;
+}
+
+/* Exposed for unit tests of the samplers' draw -> value maps and of the math. */
+double oracle_philox_draw(uint64_t seed, uint64_t path, uint64_t draw)
+{
+    return pt_philox_draw(seed, path, draw);
+}
+double oracle_pt_log(double x) { return pt_log(x); }
+void oracle_pt_sincos(double x, double *s, double *c) { pt_sincos(x, s, c); }
