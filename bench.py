@@ -1,0 +1,157 @@
+"""Throughput of the phonon Monte Carlo trace on one MI355X.
+
+Workload (the thin-film configuration): a 600 x 600 x 200 nm film, transport along x,
+three z-cylindrical holes of radius 50 nm (seeded, non-overlapping; specularity 0.3,
+vacuum t = 0), diffuse film faces (p = 0), mirror in-plane side walls (p = 1, the
+reference has no periodic boundary); synthetic bulk table 6 branches x 2000 frequency
+bins; 11 polar angles (10 traced); Np particles per (k, b, angle) cell chosen so one
+step traces ~1e8 phonon paths; ballistic (Matthiessen) trace in FP64 exactly as
+calcTransmit (src/RTMCphon.f90:1608-2414).  One step = one pt_trace_grid call, i.e.
+TransmissionFunction (RT:1544-1604) for the whole table; inputs are resident in HBM
+before timing starts.
+
+Prints one JSON line (metric, value = paths/s, roofline of the trace kernel and the
+CPU baseline: the oracle's C restatement on the host's cores, on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "p-trans_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+from ptrans import gen, geometry  # noqa: E402
+from ptrans.anphon import synthetic_table  # noqa: E402
+from ptrans.constants import M_TWOPI  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector, published
+METRIC = "phonon paths/s and κ-gradient evals/s on 1 MI355X at fixed κ std-error"
+
+
+def workload(np_per_cell: int, nangle: int = 11, nb: int = 6, nk: int = 2000):
+    model = gen.film_holes(seed=2, lx=600.0, ly=600.0, h=200.0, radius=50.0, nholes=3,
+                           p_hole=0.3, p_film=0.0, p_side=1.0, nphonon=np_per_cell, nangle=nangle)
+    geo = geometry.build_geometry(model)
+    t = synthetic_table(nb, nk, seed=20)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12          # RT:727
+    return model, geo, omega
+
+
+def cpu_baseline(geo, omega, mfp, nangle, np_per_cell, seed, seconds, threads):
+    """The oracle (C restatement, single-threaded per call) on `threads` host threads
+    (ctypes releases the GIL), each tracing whole cells; runs cells until `seconds`."""
+    from oracle import oracle as O
+    gdesc, _ga = geo.desc()
+    mdesc, _ma = geometry.modes_desc(omega, omega, omega, omega)
+    ncell = omega.size * (nangle - 1)
+    lock = threading.Lock()
+    state = {"next": 0, "paths": 0}
+    t_end = time.perf_counter() + seconds
+
+    def worker():
+        while time.perf_counter() < t_end:
+            with lock:
+                c = state["next"]
+                state["next"] += 1
+            if c >= ncell:
+                return
+            _, st = O.trace_grid(gdesc, mdesc, mfp, nangle, np_per_cell, seed, O.MATH_OWN,
+                                 cell_begin=c, cell_end=c + 1)
+            with lock:
+                state["paths"] += int(st[0])
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    return state["paths"] / dt, state["paths"], state["next"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--np", type=int, default=834, help="particles per cell (834 -> 1.0e8 paths)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(paths shard by id range; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ptrans.trace import Context
+
+    nangle, mfp, seed = 11, 1.0e10, 12345
+    model, geo, omega = workload(args.np, nangle)
+    nb, nk = omega.shape
+    paths = nb * nk * (nangle - 1) * args.np
+    with Context(seed=seed, device=0) as ctx:
+        g = ctx.geometry(geo)
+        m = ctx.modes(omega=omega)
+        for _ in range(args.warmup):
+            ctx.trace_grid(g, m, mfp, nangle, args.np)
+        torch.cuda.synchronize()
+        kern_ms = []
+        stats = None
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            counts, stats = ctx.trace_grid(g, m, mfp, nangle, args.np)
+            kern_ms.append(stats["kernel_ms"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    value = paths * args.steps / dt
+    k_ms = float(np.mean(kern_ms))
+    events_per_path = stats["events"] / stats["paths"]
+    transmitted = counts.sum() / paths
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        rate, npaths, ncells = cpu_baseline(geo, omega, mfp, nangle, args.np, seed,
+                                            args.cpu_seconds, threads)
+        cpu = {"value": rate, "unit": "paths/s", "cores": threads, "kind": "port",
+               "sample": "%d whole cells (%d paths) of the same workload in %.0f s, oracle C "
+                         "restatement, one thread per cell" % (ncells, npaths, args.cpu_seconds)}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded film geometry and bulk table)",
+        "config": {"workload": "thin film 600x600x200 nm, 3 z-cylinder holes r=50 nm p=0.3, "
+                               "6x2000 modes x 10 angles x %d particles, ballistic FP64 grid "
+                               "trace" % args.np,
+                   "paths_per_step": paths, "events_per_path": events_per_path,
+                   "transmission": transmitted},
+        "kernel_ms": k_ms,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -168,6 +168,9 @@ size_t pt_last_error(char *buf, size_t len);
 int pt_ctx_create(int device, uint64_t seed, pt_ctx **out);
 int pt_ctx_destroy(pt_ctx *ctx);
 int pt_ctx_set_seed(pt_ctx *ctx, uint64_t seed);
+/* Safety stop: a path ends (flag PT_PATH_EVENT_LIMIT) after this many events.  The
+ * reference loops without bound (RT:1693); default 1e8. */
+int pt_ctx_set_event_limit(pt_ctx *ctx, int64_t limit);
 
 /* ---- geometry ------------------------------------------------------------------ */
 /* Uploads vgrain (RT:122) as flattened by the host.  Validates indices. */

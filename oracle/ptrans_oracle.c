@@ -526,6 +526,10 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, pt_path_recor
                 end_side = 1;
             } else if (igrain == 0 && isurface == 2) {
                 end_side = 2;
+            } else if (isurface == 0) {
+                /* no surface after a scattering: vgrain(0)%vsurface(0) (undefined) */
+                flags |= PT_PATH_LOST;
+                break;
             } else {
                 const pt_surface *sf = g->surfaces + g->grain_offset[igrain] + (isurface - 1);
                 double spec = o_rand(s, 2230);                    /* determinSpecular RT:2230 */

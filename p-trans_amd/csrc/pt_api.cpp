@@ -1,0 +1,535 @@
+// pt_api.cpp -- the C-ABI of libptrans (include/ptrans.h): contexts, geometry and
+// mode tables in HBM, and the trace entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ptrans.h"
+#include "pt_internal.h"
+#include "pt_math.h"
+
+using namespace pt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define PT_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(PT_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+template <class T>
+int upload(const std::vector<T> &v, T **dptr, hipStream_t s)
+{
+    *dptr = nullptr;
+    if (v.empty()) return PT_OK;
+    PT_HIP(hipMalloc((void **)dptr, v.size() * sizeof(T)));
+    PT_HIP(hipMemcpyAsync(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return PT_OK;
+}
+
+constexpr double kPi = 3.14159265358979323846;
+
+}  // namespace
+
+struct pt_ctx {
+    int device = 0;
+    uint64_t seed = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int ncu = 0;
+    int blocks_per_cu = 0;
+    int threads = 256;
+    int chunk = 256;
+    int64_t event_limit = 100000000;
+    // grow-only scratch of the grid trace
+    uint8_t *outcome = nullptr;
+    size_t outcome_bytes = 0;
+    unsigned long long *counters = nullptr;   // queue + 6 stats + skipped
+};
+
+struct pt_geom {
+    pt_ctx *ctx = nullptr;
+    DevGeom dg{};
+    std::vector<void *> owned;
+};
+
+struct pt_modes {
+    pt_ctx *ctx = nullptr;
+    DevModes dm{};
+    int32_t nb = 0, nk = 0;
+    double *d_omega = nullptr;
+};
+
+extern "C" {
+
+int32_t pt_version(void) { return PT_ABI_VERSION; }
+
+const char *pt_build_info(void)
+{
+    return "libptrans " "1" " gfx950 HIP " __DATE__ " fp64-exact";
+}
+
+size_t pt_last_error(char *buf, size_t len)
+{
+    if (buf && len) {
+        size_t n = std::min(len - 1, g_err.size());
+        memcpy(buf, g_err.data(), n);
+        buf[n] = 0;
+    }
+    return g_err.size();
+}
+
+int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
+{
+    if (!out) return fail(PT_E_INVALID, "pt_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PT_E_NODEVICE, "pt_ctx_create: no HIP device");
+    if (device < 0 || device >= ndev)
+        return fail(PT_E_INVALID, "pt_ctx_create: device %d out of range (%d devices)", device, ndev);
+    PT_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    PT_HIP(hipGetDeviceProperties(&prop, device));
+    pt_ctx *c = new pt_ctx();
+    c->device = device;
+    c->seed = seed;
+    c->ncu = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = trace_grid_occupancy(c->threads, &c->blocks_per_cu);
+    if (e != hipSuccess) {
+        pt_ctx_destroy(c);
+        return fail(PT_E_HIP, "pt_ctx_create: %s", hipGetErrorString(e));
+    }
+    if (c->blocks_per_cu < 1) c->blocks_per_cu = 1;
+    *out = c;
+    return PT_OK;
+}
+
+int pt_ctx_destroy(pt_ctx *c)
+{
+    if (!c) return PT_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->outcome) (void)hipFree(c->outcome);
+    if (c->counters) (void)hipFree(c->counters);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PT_OK;
+}
+
+int pt_ctx_set_seed(pt_ctx *c, uint64_t seed)
+{
+    if (!c) return fail(PT_E_INVALID, "pt_ctx_set_seed: NULL context");
+    c->seed = seed;
+    return PT_OK;
+}
+
+int pt_ctx_set_event_limit(pt_ctx *c, int64_t limit)
+{
+    if (!c || limit < 1) return fail(PT_E_INVALID, "pt_ctx_set_event_limit: bad argument");
+    c->event_limit = limit;
+    return PT_OK;
+}
+
+int pt_surface_setup_polygon(int32_t nv, const double *v, double center[3], double normal[3],
+                             double tangent[9])
+{
+    // setupSurfaceGeometry, polygon branch (src/RTMCphon.f90:1440-1487)
+    if (nv < 3 || !v || !center || !normal || !tangent)
+        return fail(PT_E_INVALID, "pt_surface_setup_polygon: need >= 3 vertices");
+    auto V = [&](int i) { return v + 3 * ((i - 1) % nv); };   // 1-based, closed loop
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int iv = 1; iv <= nv; ++iv)
+        for (int k = 0; k < 3; ++k) c[k] = c[k] + V(iv)[k];
+    const double inv = 1.0 / nv;
+    for (int k = 0; k < 3; ++k) center[k] = inv * c[k];
+    double n[3] = {0.0, 0.0, 0.0};
+    auto add_unit_cross = [&](const double *p0, const double *p1, const double *p2) {
+        double a[3], b[3], cr[3];
+        for (int k = 0; k < 3; ++k) { a[k] = p1[k] - p0[k]; b[k] = p2[k] - p0[k]; }
+        cr[0] = a[1] * b[2] - a[2] * b[1];
+        cr[1] = a[2] * b[0] - a[0] * b[2];
+        cr[2] = a[0] * b[1] - a[1] * b[0];
+        const double len = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        for (int k = 0; k < 3; ++k) n[k] = n[k] + cr[k] / len;
+    };
+    for (int iv = 1; iv <= nv - 1; ++iv) add_unit_cross(V(iv), V(iv + 1), V(iv + 2));
+    add_unit_cross(V(nv), V(nv + 1), V(2));
+    for (int k = 0; k < 3; ++k) normal[k] = inv * n[k];
+    tangent[6] = normal[0]; tangent[7] = normal[1]; tangent[8] = normal[2];
+    if (std::fabs(normal[2]) == 1.0 ||
+        (normal[0] * normal[0] == 0.0 && normal[1] * normal[1] == 0.0)) {
+        tangent[3] = 0.0; tangent[4] = 1.0; tangent[5] = 0.0;
+        tangent[0] = 1.0; tangent[1] = 0.0; tangent[2] = 0.0;
+    } else {
+        const double r = std::sqrt(normal[0] * normal[0] + normal[1] * normal[1]);
+        tangent[3] = -normal[1] / r;
+        tangent[4] = normal[0] / r;
+        tangent[5] = 0.0;
+        const double *a = tangent + 3, *b = tangent + 6;
+        tangent[0] = a[1] * b[2] - a[2] * b[1];
+        tangent[1] = a[2] * b[0] - a[0] * b[2];
+        tangent[2] = a[0] * b[1] - a[1] * b[0];
+    }
+    return PT_OK;
+}
+
+int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
+{
+    if (!c || !d || !out) return fail(PT_E_INVALID, "pt_geom_create: NULL argument");
+    *out = nullptr;
+    const int ng = d->ngrain_total, ns = d->nsurface;
+    if (d->kind_grain != PT_GRAIN_VORONOI && d->kind_grain != PT_GRAIN_BRICKS)
+        return fail(PT_E_GEOMETRY, "kind_grain %d is neither voronoi nor bricks", d->kind_grain);
+    const int expect = d->kind_grain == PT_GRAIN_BRICKS ? d->ngrain + 2 : d->ngrain + 1;
+    if (d->ngrain < 0 || ng != expect)
+        return fail(PT_E_GEOMETRY, "ngrain_total %d, expected %d for ngrain %d", ng, expect, d->ngrain);
+    if (d->kind_grain == PT_GRAIN_BRICKS && d->igrain_fill != d->ngrain + 1)
+        return fail(PT_E_GEOMETRY, "igrain_fill must be ngrain+1 for bricks");
+    if (!d->grain_offset || !d->surfaces || (ns > 0 && d->nvertex > 0 && !d->vertices) ||
+        !d->grain_incident || !d->generatrix)
+        return fail(PT_E_INVALID, "pt_geom_create: NULL array in the descriptor");
+    if (d->grain_offset[0] != 0 || d->grain_offset[ng] != ns)
+        return fail(PT_E_GEOMETRY, "grain_offset must run from 0 to nsurface");
+    for (int g = 0; g < ng; ++g)
+        if (d->grain_offset[g + 1] < d->grain_offset[g])
+            return fail(PT_E_GEOMETRY, "grain_offset is not monotone at grain %d", g);
+    if (d->grain_offset[1] < 2)
+        return fail(PT_E_GEOMETRY, "the box grain needs its x- and x+ walls first");
+    if (!(d->box[0] > 0 && d->box[1] > 0 && d->box[2] > 0))
+        return fail(PT_E_GEOMETRY, "box dimensions must be positive");
+    if (d->nvertex > INT32_MAX / 3) return fail(PT_E_GEOMETRY, "too many vertices");
+
+    std::vector<DSurf> surf(ns);
+    std::vector<double> tang(9 * (size_t)ns);
+    for (int i = 0; i < ns; ++i) {
+        const pt_surface &s = d->surfaces[i];
+        DSurf &o = surf[i];
+        memset(&o, 0, sizeof o);
+        if (s.kind != PT_SHAPE_POLYGON && s.kind != PT_SHAPE_SPHERE && s.kind != PT_SHAPE_CYLINDER)
+            return fail(PT_E_GEOMETRY, "surface %d: unknown shape %d", i, s.kind);
+        if (s.kind == PT_SHAPE_POLYGON) {
+            if (s.nvertex < 3 || s.vertex_offset < 0 || s.vertex_offset + s.nvertex > d->nvertex)
+                return fail(PT_E_GEOMETRY, "surface %d: bad polygon vertex range", i);
+        } else if (!(s.radius > 0)) {
+            return fail(PT_E_GEOMETRY, "surface %d: radius must be positive", i);
+        }
+        if (s.internal) {
+            if (s.trans_kind < PT_TRANS_VALUE || s.trans_kind > PT_TRANS_TBC)
+                return fail(PT_E_GEOMETRY, "surface %d: unknown transmissivity law %d", i, s.trans_kind);
+            if (s.grain_outside < 0 || s.grain_outside >= ng)
+                return fail(PT_E_GEOMETRY, "surface %d: grain_outside %d out of range", i, s.grain_outside);
+        }
+        for (int k = 0; k < 3; ++k) { o.n[k] = s.normal[k]; o.c[k] = s.center[k]; }
+        o.r = s.radius;
+        o.p = s.specularity;
+        o.tparam = s.trans_param;
+        o.kind = s.kind;
+        o.nv = s.kind == PT_SHAPE_POLYGON ? s.nvertex : 0;
+        o.voff = (int32_t)s.vertex_offset;
+        o.gout = s.grain_outside;
+        o.tkind = s.trans_kind;
+        memcpy(&tang[9 * (size_t)i], s.tangent, 9 * sizeof(double));
+    }
+    // box walls x- and x+ must be the first two surfaces (EndSide, RT:1748-1753)
+    std::vector<int32_t> cand_off(ng + 1, 0), cand_idx, inc_off(ng + 1, 0), inc_idx;
+    for (int g = 0; g < ng; ++g) {
+        for (int i = d->grain_offset[g]; i < d->grain_offset[g + 1]; ++i) {
+            if (d->surfaces[i].internal) cand_idx.push_back(i);
+            if (d->surfaces[i].incident && d->surfaces[i].kind == PT_SHAPE_POLYGON) inc_idx.push_back(i);
+        }
+        cand_off[g + 1] = (int32_t)cand_idx.size();
+        inc_off[g + 1] = (int32_t)inc_idx.size();
+    }
+    std::vector<int32_t> ginc(d->grain_incident, d->grain_incident + ng);
+    std::vector<double> gen(d->generatrix, d->generatrix + 3 * (size_t)ng);
+    std::vector<double> vtx(d->vertices, d->vertices + 3 * (size_t)d->nvertex);
+
+    PT_HIP(hipSetDevice(c->device));
+    pt_geom *G = new pt_geom();
+    G->ctx = c;
+    int rc = PT_OK;
+    DSurf *d_surf = nullptr;
+    double *d_tang = nullptr, *d_vtx = nullptr, *d_gen = nullptr;
+    int32_t *d_co = nullptr, *d_ci = nullptr, *d_io = nullptr, *d_ii = nullptr, *d_ginc = nullptr;
+    if ((rc = upload(surf, &d_surf, c->stream)) || (rc = upload(tang, &d_tang, c->stream)) ||
+        (rc = upload(vtx, &d_vtx, c->stream)) || (rc = upload(gen, &d_gen, c->stream)) ||
+        (rc = upload(cand_off, &d_co, c->stream)) || (rc = upload(cand_idx, &d_ci, c->stream)) ||
+        (rc = upload(inc_off, &d_io, c->stream)) || (rc = upload(inc_idx, &d_ii, c->stream)) ||
+        (rc = upload(ginc, &d_ginc, c->stream))) {
+        for (void *p : {(void *)d_surf, (void *)d_tang, (void *)d_vtx, (void *)d_gen, (void *)d_co,
+                        (void *)d_ci, (void *)d_io, (void *)d_ii, (void *)d_ginc})
+            if (p) (void)hipFree(p);
+        delete G;
+        return rc;
+    }
+    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc};
+    DevGeom &g = G->dg;
+    g.surf = d_surf;
+    g.tang = d_tang;
+    g.vtx = d_vtx;
+    g.cand_off = d_co;
+    g.cand_idx = d_ci;
+    g.inc_off = d_io;
+    g.inc_idx = d_ii;
+    g.grain_incident = d_ginc;
+    g.gen = d_gen;
+    g.surf_base = nullptr;
+    for (int k = 0; k < 3; ++k) g.box[k] = d->box[k];
+    g.scope[0] = d->scope[0];
+    g.scope[1] = d->scope[1];
+    g.ngrain = d->ngrain;
+    g.kind_grain = d->kind_grain;
+    g.igrain_fill = d->igrain_fill;
+    g.nmaterial = d->nmaterial;
+    g.ngrain_total = ng;
+    g.nsurface = ns;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        pt_geom_destroy(G);
+        return fail(PT_E_HIP, "pt_geom_create: %s", hipGetErrorString(e));
+    }
+    *out = G;
+    return PT_OK;
+}
+
+int pt_geom_destroy(pt_geom *G)
+{
+    if (!G) return PT_OK;
+    (void)hipSetDevice(G->ctx->device);
+    (void)hipStreamSynchronize(G->ctx->stream);
+    for (void *p : G->owned)
+        if (p) (void)hipFree(p);
+    delete G;
+    return PT_OK;
+}
+
+int pt_modes_create(pt_ctx *c, const pt_modes_desc *d, pt_modes **out)
+{
+    if (!c || !d || !out) return fail(PT_E_INVALID, "pt_modes_create: NULL argument");
+    *out = nullptr;
+    if (d->nbranch < 1 || d->nkpoint < 1) return fail(PT_E_INVALID, "pt_modes_create: empty table");
+    if (!d->omega) return fail(PT_E_INVALID, "pt_modes_create: omega is NULL");
+    const size_t n = (size_t)d->nbranch * d->nkpoint;
+    std::vector<double> om(d->omega, d->omega + n);
+    PT_HIP(hipSetDevice(c->device));
+    pt_modes *M = new pt_modes();
+    M->ctx = c;
+    M->nb = d->nbranch;
+    M->nk = d->nkpoint;
+    int rc = upload(om, &M->d_omega, c->stream);
+    if (rc) { delete M; return rc; }
+    M->dm.omega = M->d_omega;
+    M->dm.nb = d->nbranch;
+    M->dm.nk = d->nkpoint;
+    M->dm.max_omega = d->max_omega;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        pt_modes_destroy(M);
+        return fail(PT_E_HIP, "pt_modes_create: %s", hipGetErrorString(e));
+    }
+    *out = M;
+    return PT_OK;
+}
+
+int pt_modes_destroy(pt_modes *M)
+{
+    if (!M) return PT_OK;
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipStreamSynchronize(M->ctx->stream);
+    if (M->d_omega) (void)hipFree(M->d_omega);
+    delete M;
+    return PT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// param%vangle (RT:1554-1556) and the cos/sin of each angle with the trace's own
+// sincos, so host and device see the same direction cosines.
+void angle_tables(int32_t nangle, const double *vangle, std::vector<double> &ct,
+                  std::vector<double> &st)
+{
+    ct.resize(nangle);
+    st.resize(nangle);
+    for (int ia = 1; ia <= nangle; ++ia) {
+        const double th = vangle ? vangle[ia - 1]
+                                 : 0.0 + (double)(ia - 1) / (double)(nangle - 1) * (kPi * 0.5);
+        pt_sincos(th, &st[ia - 1], &ct[ia - 1]);
+    }
+}
+
+int check_trace_args(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
+                     int64_t np)
+{
+    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
+    if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
+    if (nangle < 2) return fail(PT_E_INVALID, "nangle must be >= 2");
+    if (np < 0) return fail(PT_E_INVALID, "np must be >= 0");
+    if (!(mfp > 0)) return fail(PT_E_INVALID, "mean free path must be positive");
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
+                  const double *vangle, int64_t np, int32_t precision, int64_t *counts,
+                  pt_stats *stats)
+{
+    int rc = check_trace_args(c, G, M, mfp, nangle, np);
+    if (rc) return rc;
+    if (precision != PT_PREC_FP64) return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (!counts) return fail(PT_E_INVALID, "counts is NULL");
+    const int64_t nb = M->nb, nk = M->nk;
+    const int64_t ncell = nb * nk * (nangle - 1);
+    const int64_t total = ncell * np;
+    const size_t ncount = (size_t)(nb * nk * nangle);
+    memset(counts, 0, ncount * sizeof(int64_t));
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (total == 0) return PT_OK;
+
+    PT_HIP(hipSetDevice(c->device));
+    std::vector<double> ct, st;
+    angle_tables(nangle, vangle, ct, st);
+    if ((size_t)total > c->outcome_bytes) {
+        if (c->outcome) PT_HIP(hipFree(c->outcome));
+        c->outcome = nullptr;
+        c->outcome_bytes = 0;
+        PT_HIP(hipMalloc((void **)&c->outcome, (size_t)total));
+        c->outcome_bytes = (size_t)total;
+    }
+    double *d_ang = nullptr;
+    long long *d_counts = nullptr;
+    PT_HIP(hipMalloc((void **)&d_ang, 2 * nangle * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&d_counts, ncount * sizeof(long long));
+    if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "hipMalloc: %s", hipGetErrorString(e)); }
+    std::vector<double> ang(ct);
+    ang.insert(ang.end(), st.begin(), st.end());
+    GridParams P{};
+    P.g = G->dg;
+    P.m = M->dm;
+    P.cos_t = d_ang;
+    P.sin_t = d_ang + nangle;
+    P.seed = c->seed;
+    P.mfp = mfp;
+    P.np = np;
+    P.total = total;
+    P.nangle = nangle;
+    P.chunk = c->chunk;
+    P.event_limit = c->event_limit;
+    P.queue = c->counters;
+    P.stats = c->counters + 1;
+    P.outcome = c->outcome;
+    unsigned long long host_ctr[8] = {0};
+    float ms = 0.f;
+    const int blocks = c->ncu * c->blocks_per_cu;
+    do {
+        if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = launch_trace_grid(P, blocks, c->threads, c->stream))) break;
+        if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipMemcpyAsync(counts, d_counts, ncount * sizeof(long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipStreamSynchronize(c->stream))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    (void)hipFree(d_ang);
+    (void)hipFree(d_counts);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_grid: %s", hipGetErrorString(e));
+    if (stats) {
+        stats->paths = (int64_t)host_ctr[1];
+        stats->events = (int64_t)host_ctr[2];
+        stats->draws = (int64_t)host_ctr[3];
+        stats->lost = (int64_t)host_ctr[4];
+        stats->start_mismatch = (int64_t)host_ctr[5];
+        stats->skipped = (int64_t)host_ctr[7];
+        stats->kernel_ms = ms;
+    }
+    if ((int64_t)host_ctr[1] != total)
+        return fail(PT_E_HIP, "pt_trace_grid: %lld of %lld paths completed",
+                    (long long)host_ctr[1], (long long)total);
+    return PT_OK;
+}
+
+int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
+              const double *vangle, int64_t np, int64_t first_path, int64_t n,
+              pt_path_record *records)
+{
+    int rc = check_trace_args(c, G, M, mfp, nangle, np);
+    if (rc) return rc;
+    if (np < 1) return fail(PT_E_INVALID, "np must be >= 1");
+    if (n < 0 || first_path < 0) return fail(PT_E_INVALID, "bad path range");
+    if (n == 0) return PT_OK;
+    if (!records) return fail(PT_E_INVALID, "records is NULL");
+    const int64_t total = (int64_t)M->nb * M->nk * (nangle - 1) * np;
+    if (first_path + n > total) return fail(PT_E_INVALID, "path range beyond the last cell");
+    PT_HIP(hipSetDevice(c->device));
+    std::vector<double> ct, st;
+    angle_tables(nangle, vangle, ct, st);
+    std::vector<double> ang(ct);
+    ang.insert(ang.end(), st.begin(), st.end());
+    double *d_ang = nullptr;
+    pt_path_record *d_rec = nullptr;
+    PT_HIP(hipMalloc((void **)&d_ang, 2 * nangle * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&d_rec, n * sizeof(pt_path_record));
+    if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "hipMalloc: %s", hipGetErrorString(e)); }
+    ReplayParams P{};
+    P.g = G->dg;
+    P.m = M->dm;
+    P.cos_t = d_ang;
+    P.sin_t = d_ang + nangle;
+    P.seed = c->seed;
+    P.mfp = mfp;
+    P.np = np;
+    P.nangle = nangle;
+    P.first = first_path;
+    P.n = n;
+    P.event_limit = c->event_limit;
+    P.rec = d_rec;
+    do {
+        if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = launch_replay(P, c->stream))) break;
+        if ((e = hipMemcpyAsync(records, d_rec, n * sizeof(pt_path_record), hipMemcpyDeviceToHost, c->stream))) break;
+        e = hipStreamSynchronize(c->stream);
+    } while (0);
+    (void)hipFree(d_ang);
+    (void)hipFree(d_rec);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_replay: %s", hipGetErrorString(e));
+    return PT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// pt_internal.h -- device data layout shared by the kernels and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ptrans.h"
+
+namespace pt {
+
+// One surface as the trace kernels read it (the fields of Surface_type,
+// src/RTMCphon.f90:94-111, that calcTransmit touches).  112 bytes, 16-byte aligned.
+struct alignas(16) DSurf {
+    double n[3];        // polygon normal (not renormalised, RT:1459)
+    double c[3];        // centre (sphere/cylinder) or centroid (polygon)
+    double r;           // radius
+    double p;           // specularity
+    double tparam;      // transmissivity law parameter (t or gamma)
+    int32_t kind;       // PT_SHAPE_*
+    int32_t nv;         // polygon vertex count
+    int32_t voff;       // first vertex in the vertex pool
+    int32_t gout;       // grain_outside
+    int32_t tkind;      // PT_TRANS_*
+    int32_t pad[3];
+};
+static_assert(sizeof(DSurf) == 112, "DSurf layout");
+
+// Geometry in HBM.  Candidate lists hold, per grain, the global indices of its
+// internal surfaces in reference order (the scan of RT:2012-2028 skips the others).
+struct DevGeom {
+    const DSurf *surf;
+    const double *tang;        // 9 doubles per surface, column-major (tangent(:,j))
+    const double *vtx;         // xyz triples
+    const int32_t *cand_off;   // [ngrain_total+1]
+    const int32_t *cand_idx;
+    const int32_t *inc_off;    // incident surfaces per grain (findIncidentCollisionGrain)
+    const int32_t *inc_idx;
+    const int32_t *grain_incident;
+    const double *gen;         // generatrix xyz per grain
+    const int32_t *surf_base;  // global index of each grain's first surface
+    double box[3];
+    double scope[2];
+    int32_t ngrain;
+    int32_t kind_grain;
+    int32_t igrain_fill;
+    int32_t nmaterial;
+    int32_t ngrain_total;
+    int32_t nsurface;
+};
+
+struct DevModes {
+    const double *omega;       // (nb, nk) column-major
+    int32_t nb, nk;
+    double max_omega;
+};
+
+// Per-path outcome bytes of the grid trace.
+enum : uint8_t {
+    OUT_TRANSMITTED = 1,       // EndSide /= StartSide (RT:1833)
+    OUT_MISMATCH = 2,          // start-grain check failed (RT:1673-1682)
+};
+
+struct GridParams {
+    DevGeom g;
+    DevModes m;
+    const double *cos_t;       // per angle index
+    const double *sin_t;
+    uint64_t seed;
+    double mfp;
+    int64_t np;                // particles per cell
+    int64_t total;             // paths = ncell * np
+    int32_t nangle;
+    int32_t chunk;             // paths a wave takes from the queue at once
+    int64_t event_limit;
+    unsigned long long *queue; // next path id
+    unsigned long long *stats; // paths, events, draws, lost, mismatch, limit
+    uint8_t *outcome;          // [total]
+};
+
+struct ReplayParams {
+    DevGeom g;
+    DevModes m;
+    const double *cos_t;
+    const double *sin_t;
+    uint64_t seed;
+    double mfp;
+    int64_t np;
+    int32_t nangle;
+    int64_t first;
+    int64_t n;
+    int64_t event_limit;
+    pt_path_record *rec;
+};
+
+// kernels' launch wrappers (pt_trace.hip)
+hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, hipStream_t s);
+hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
+                              int64_t nb, int64_t nk, int32_t nangle,
+                              long long *counts, unsigned long long *skipped, hipStream_t s);
+hipError_t launch_replay(const ReplayParams &p, hipStream_t s);
+hipError_t trace_grid_occupancy(int threads, int *blocks_per_cu);
+
+}  // namespace pt

@@ -1,0 +1,674 @@
+// pt_trace.hip -- gfx950 kernels of the phonon Monte Carlo trace (FP64 path).
+//
+// One phonon path per lane.  The event loop restates calcTransmit
+// (src/RTMCphon.f90:1608-2414) and the geometry kernels (RT:2883-3301) operation by
+// operation in FP64; this file is compiled with -ffp-contract=off so that every lane
+// computes the same bits as a sequential CPU run on the same Philox stream.
+//
+// trace_grid_kernel is persistent: each wave takes chunks of path ids from one global
+// counter and refills lanes whose path has ended (ballot + prefix count), so lanes
+// with long (diffusive) paths do not hold idle lanes of the same wave hostage.
+#include "pt_internal.h"
+#include "pt_math.h"
+#include "pt_philox.h"
+
+namespace pt {
+
+namespace {
+
+constexpr double kTwoPi = 3.14159265358979323846 * 2.0;    // M_TWOPI, RT:63
+constexpr double kSmallVal = 1.0e-5;                         // SMALL_VAL, RT:69
+constexpr double kBallistic = 1.0e10;                        // RT:1686
+#define PT_NUDGE ((double)1.0e-7f)                           // REAL(4) literal, RT:1779
+#define PT_SPHERE_EPS ((double)1e-10f)                       // REAL(4) literal, RT:3001
+
+// Draw d of the lane's path: pt_philox_draw(seed, path, d), one Philox block per
+// two draws.
+struct Rng {
+    uint64_t seed;
+    uint64_t path;
+    uint32_t draw;
+    double spare;
+    __device__ double next()
+    {
+        double u;
+        if (draw & 1u) {
+            u = spare;
+        } else {
+            double b[2];
+            pt_philox_block(seed, path, (uint64_t)(draw >> 1), b);
+            u = b[0];
+            spare = b[1];
+        }
+        ++draw;
+        return u;
+    }
+};
+
+__device__ inline double dot3(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ inline void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// calcSurfaceDistancePolygon, RT:2907-2959: one-sided plane hit, then the convex
+// point-in-polygon test with on-edge counted inside.
+__device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
+                             const double *nd, double *lhit)
+{
+    const double cosang = nd[0] * s.n[0] + nd[1] * s.n[1] + nd[2] * s.n[2];
+    if (cosang <= 0.0) return false;
+    const double *v = g.vtx + 3 * (int64_t)s.voff;
+    double d[3] = {v[0] - rs[0], v[1] - rs[1], v[2] - rs[2]};
+    const double l = (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]) / cosang;
+    if (l < 0.0) return false;
+    double h[3];
+    for (int k = 0; k < 3; ++k) h[k] = rs[k] + nd[k] * l;
+    double c1[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < s.nv; ++i) {
+        const double *vi = v + 3 * i;
+        const double *vn = v + 3 * (i + 1 == s.nv ? 0 : i + 1);
+        double ra[3], rb[3], ci[3];
+        for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = h[k] - vi[k]; }
+        cross3(ra, rb, ci);
+        if (i == 0) {
+            c1[0] = ci[0]; c1[1] = ci[1]; c1[2] = ci[2];
+        } else if (dot3(c1, ci) < 0.0) {
+            return false;
+        }
+    }
+    *lhit = l;
+    return true;
+}
+
+// isInsidePolygon, RT:2961-2983
+__device__ bool inside_polygon(const double *v, int nv, const double *p)
+{
+    double c1[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < nv; ++i) {
+        const double *vi = v + 3 * i;
+        const double *vn = v + 3 * (i + 1 == nv ? 0 : i + 1);
+        double ra[3], rb[3], ci[3];
+        for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = p[k] - vi[k]; }
+        cross3(ra, rb, ci);
+        if (i == 0) {
+            c1[0] = ci[0]; c1[1] = ci[1]; c1[2] = ci[2];
+        } else if (dot3(c1, ci) < 0.0) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// calcSurfaceDistanceSphere, RT:2985-3038
+__device__ bool dist_sphere(const DSurf &s, const double *rs, const double *nd, double *lhit)
+{
+    double rsc[3] = {s.c[0] - rs[0], s.c[1] - rs[1], s.c[2] - rs[2]};
+    const double rd = dot3(rsc, nd);
+    const double rr = dot3(rsc, rsc);
+    if (rr > s.r * s.r - PT_SPHERE_EPS) {
+        if (rd <= 0.0) return false;
+    }
+    const double d = rd * rd + s.r * s.r - rr;
+    if (d < 0.0) return false;
+    const double sq = sqrt(d);
+    const double l1 = rd - sq, l2 = rd + sq;
+    if (l1 > 0.0) *lhit = l1;
+    else if (l2 > 0.0) *lhit = l2;
+    else return false;
+    return true;
+}
+
+// calcSurfaceDistanceCylinder, RT:3040-3088 (z axis, clipped to 0 <= z <= boxz)
+__device__ bool dist_cylinder(const DSurf &s, const double *rs, const double *nd, double boxz,
+                              double *lhit)
+{
+    const double rsc0 = s.c[0] - rs[0], rsc1 = s.c[1] - rs[1];
+    double t0 = nd[0], t1 = nd[1];
+    const double nrm = t0 * t0 + t1 * t1 + 0.0 * 0.0;
+    const double sn = sqrt(nrm);
+    t0 = t0 / sn;
+    t1 = t1 / sn;
+    const double rd = rsc0 * t0 + rsc1 * t1;
+    const double rr = rsc0 * rsc0 + rsc1 * rsc1;
+    const double d = rd * rd + s.r * s.r - rr;
+    if (d < 0.0) return false;
+    const double sq = sqrt(d);
+    const double l1 = rd - sq, l2 = rd + sq;
+    double l;
+    if (l1 > 0.0) l = l1;
+    else if (l2 > 0.0) l = l2;
+    else return false;
+    l = l / sqrt(nd[0] * nd[0] + nd[1] * nd[1]);
+    const double zh = rs[2] + nd[2] * l;
+    if (zh < 0.0 || boxz < zh) return false;
+    *lhit = l;
+    return true;
+}
+
+__device__ inline bool surface_distance(const DevGeom &g, const DSurf &s, const double *p,
+                                        const double *d, double *l)
+{
+    switch (s.kind) {
+    case PT_SHAPE_POLYGON: return dist_polygon(g, s, p, d, l);
+    case PT_SHAPE_SPHERE: return dist_sphere(s, p, d, l);
+    case PT_SHAPE_CYLINDER: return dist_cylinder(s, p, d, g.box[2], l);
+    }
+    return false;
+}
+
+// calcSurfaceNormal*, RT:3119-3179
+__device__ void surface_normal(const DSurf &s, const double *pos, double *n)
+{
+    if (s.kind == PT_SHAPE_POLYGON) {
+        n[0] = s.n[0]; n[1] = s.n[1]; n[2] = s.n[2];
+        return;
+    }
+    double rcs[3] = {pos[0] - s.c[0], pos[1] - s.c[1], pos[2] - s.c[2]};
+    if (s.kind == PT_SHAPE_CYLINDER) rcs[2] = 0.0;
+    const double dd = dot3(rcs, rcs);
+    const double inv = 1.0 / sqrt(dd);
+    for (int k = 0; k < 3; ++k) n[k] = rcs[k] * inv;
+    if (dd > s.r * s.r)
+        for (int k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
+}
+
+// calcSurfaceRotation*, RT:3181-3301 (column-major 3x3)
+__device__ void surface_rotation(const DevGeom &g, int idx, const DSurf &s, const double *pos,
+                                 double *rot)
+{
+    if (s.kind == PT_SHAPE_POLYGON) {
+        const double *t = g.tang + 9 * (int64_t)idx;
+        for (int k = 0; k < 9; ++k) rot[k] = t[k];
+        return;
+    }
+    double a[3] = {pos[0] - s.c[0], pos[1] - s.c[1], pos[2] - s.c[2]};
+    if (s.kind == PT_SHAPE_CYLINDER) a[2] = 0.0;
+    const double dd = dot3(a, a);
+    const double inv = 1.0 / sqrt(dd);
+    for (int k = 0; k < 3; ++k) a[k] = a[k] * inv;
+    rot[6] = a[0]; rot[7] = a[1]; rot[8] = a[2];
+    const bool axis = (s.kind == PT_SHAPE_SPHERE) ? (fabs(a[2]) == 1.0) : (fabs(a[0]) == 1.0);
+    if (axis || (a[0] * a[0] == 0.0 && a[1] * a[1] == 0.0)) {
+        rot[3] = 0.0; rot[4] = 1.0; rot[5] = 0.0;
+        rot[0] = 1.0; rot[1] = 0.0; rot[2] = 0.0;
+    } else {
+        const double r = sqrt(a[0] * a[0] + a[1] * a[1]);
+        rot[3] = -a[1] / r;
+        rot[4] = a[0] / r;
+        rot[5] = 0.0;
+        cross3(rot + 3, rot + 6, rot);
+    }
+}
+
+// isInsideSurface / isInsideGrain / findInsideWhichGrain, RT:1874-1985
+__device__ bool inside_surface(const DSurf &s, const double *p)
+{
+    double d[3] = {p[0] - s.c[0], p[1] - s.c[1], p[2] - s.c[2]};
+    if (s.kind == PT_SHAPE_POLYGON) return dot3(d, s.n) < 0;
+    if (s.kind == PT_SHAPE_SPHERE) return dot3(d, d) < s.r * s.r;
+    if (s.kind == PT_SHAPE_CYLINDER) return d[0] * d[0] + d[1] * d[1] < s.r * s.r;
+    return false;
+}
+
+__device__ int find_inside_which_grain(const DevGeom &g, const double *p)
+{
+    int count = 0, which = -1;
+    for (int ig = 1; ig <= g.ngrain; ++ig) {
+        bool in = true;
+        // the candidate list of a grain is its internal surfaces; incident ones are
+        // never internal (RT:1108-1111), so this is exactly isInsideGrain's set
+        for (int j = g.cand_off[ig]; j < g.cand_off[ig + 1]; ++j) {
+            if (!inside_surface(g.surf[g.cand_idx[j]], p)) { in = false; break; }
+        }
+        if (in) { which = ig; ++count; }
+    }
+    if (count == 1) return which;
+    if (count == 0 && g.kind_grain == PT_GRAIN_BRICKS) return g.ngrain + 1;
+    if (count == 0) return -1;
+    return which;
+}
+
+// findNearestIncidentGrain, RT:2100-2118
+__device__ int nearest_incident_grain(const DevGeom &g, const double *p)
+{
+    double d2min = 1.0e10;
+    int best = 0;
+    for (int ig = 1; ig <= g.ngrain; ++ig) {
+        if (!g.grain_incident[ig]) continue;
+        const double *q = g.gen + 3 * ig;
+        const double d0 = p[0] - q[0], d1 = p[1] - q[1], d2 = p[2] - q[2];
+        const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+        if (d2min > dd) { d2min = dd; best = ig; }
+    }
+    return best;
+}
+
+// findIncidentCollisionGrain, RT:2131-2163 (the last grain with a containing facet wins)
+__device__ int incident_collision_grain(const DevGeom &g, const double *p)
+{
+    int found = -1;
+    for (int ig = 1; ig <= g.ngrain; ++ig) {
+        if (!g.grain_incident[ig]) continue;
+        for (int j = g.inc_off[ig]; j < g.inc_off[ig + 1]; ++j) {
+            const DSurf &s = g.surf[g.inc_idx[j]];
+            if (inside_polygon(g.vtx + 3 * (int64_t)s.voff, s.nv, p)) { found = ig; break; }
+        }
+    }
+    return found >= 0 ? found : g.igrain_fill;
+}
+
+// State of one phonon path.
+struct Path {
+    double pos[3];
+    double dir[3];
+    double fp;              // FreePath
+    double omega;           // mode frequency of the cell (Gamma/TBC laws)
+    int which;              // whichV
+    int cur;                // last surface hit (global index), -1 after a scattering
+    int end_side;
+    int flags;
+    int start_grain;
+    int64_t events;
+};
+
+// determinTransmit, RT:2384-2413
+__device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, double max_omega,
+                                  Rng &rng, double rand_in, bool present)
+{
+    double threshold;
+    switch (s.tkind) {
+    case PT_TRANS_VALUE:
+        threshold = s.tparam;
+        break;
+    case PT_TRANS_GAMMA:
+    case PT_TRANS_TBC:
+        if (nmaterial == 1) {
+            threshold = 1.0 / (s.tparam * omega / max_omega + 1.0);       // RT:1417-1418
+        } else {
+            const double t11 = 1.0 / (s.tparam + 1.0);                     // RT:1423
+            const double om = rng.next();                                  // RT:2399
+            threshold = t11 + om * (1 - t11);
+        }
+        break;
+    default:
+        threshold = 0.0;
+    }
+    const double value = present ? rand_in : rng.next();                   // RT:2410
+    return value <= threshold;
+}
+
+// One iteration of the particle loop up to the event loop (RT:1632-1690).
+// Returns false if the start-grain check fails (the reference EXITs the particle loop).
+__device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double sin_t, Rng &rng,
+                           Path &ph)
+{
+    const double y = rng.next(), z = rng.next();                            // RT:1850-1851
+    ph.pos[0] = 0.0;
+    ph.pos[1] = y * g.box[1] * g.scope[0];
+    ph.pos[2] = z * g.box[2] * g.scope[1];
+    double phi = rng.next();                                                // RT:1866
+    phi = phi * kTwoPi;
+    double sp, cp;
+    pt_sincos(phi, &sp, &cp);
+    ph.dir[0] = cos_t;
+    ph.dir[1] = sin_t * cp;
+    ph.dir[2] = sin_t * sp;
+    ph.end_side = (ph.dir[0] <= 0.0) ? 1 : 0;                               // RT:1644-1652
+    ph.cur = 0;                                                             // igrain=0, isurface=1
+    ph.flags = 0;
+    ph.events = 0;
+    int which = 0;
+    if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos);
+    else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos);
+    ph.which = which;
+    ph.start_grain = which;
+    if (find_inside_which_grain(g, ph.pos) != which) {                      // RT:1673-1682
+        ph.flags = PT_PATH_START_MISMATCH;
+        return false;
+    }
+    if (mfp == kBallistic) ph.fp = kBallistic;                              // RT:1686-1690
+    else ph.fp = -(mfp * pt_log(rng.next()));                               // RT:2171-2172
+    return true;
+}
+
+// One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
+__device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_t event_limit,
+                          Rng &rng, Path &ph)
+{
+    // isInsideBox, RT:2177-2187
+    if (!(ph.pos[0] >= 0.0 - kSmallVal && ph.pos[0] <= g.box[0] + kSmallVal &&
+          ph.pos[1] >= 0.0 - kSmallVal && ph.pos[1] <= g.box[1] + kSmallVal &&
+          ph.pos[2] >= 0.0 - kSmallVal && ph.pos[2] <= g.box[2] + kSmallVal)) {
+        ph.flags |= PT_PATH_LOST;
+        return true;
+    }
+    if (ph.events >= event_limit) {
+        ph.flags |= PT_PATH_EVENT_LIMIT;
+        return true;
+    }
+    ph.events++;
+
+    // findCollisionSurface, RT:1989-2096: box grain, then the current grain; the first
+    // strictly nearest hit wins; with no hit the previous surface is kept.
+    double dmin = 1.0e10;
+    int best = ph.cur;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ig = pass == 0 ? 0 : ph.which;
+        if (pass == 1 && ig == 0) break;
+        const int j1 = g.cand_off[ig + 1];
+        for (int j = g.cand_off[ig]; j < j1; ++j) {
+            const int idx = g.cand_idx[j];
+            double l;
+            if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l) && dmin > l) {
+                dmin = l;
+                best = idx;
+            }
+        }
+    }
+    double distance = dmin;
+    ph.cur = best;
+
+    int jreflect = 1;
+    if (ph.fp < distance) {
+        // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
+        distance = ph.fp;
+        for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
+        double ct = rng.next();                                             // RT:2197
+        double ph_ = rng.next();                                            // RT:2198
+        ct = ct * 2.0 - 1.0;
+        ph_ = ph_ * kTwoPi;
+        const double st = sqrt(1.0 - ct * ct);
+        double sph, cph;
+        pt_sincos(ph_, &sph, &cph);
+        ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
+        ph.cur = -1;
+    } else {
+        for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
+        if (ph.cur == 0) {
+            ph.end_side = 1;
+        } else if (ph.cur == 1) {
+            ph.end_side = 2;
+        } else if (ph.cur < 0) {
+            // no surface to reflect on: vgrain(0)%vsurface(0) in the reference (undefined)
+            ph.flags |= PT_PATH_LOST;
+            return true;
+        } else {
+            const DSurf s = g.surf[ph.cur];
+            const double spec = rng.next();                                 // RT:2230
+            if (spec <= s.p) {
+                // reflectSpecular, RT:2275-2328
+                if (determin_transmit(s, g.nmaterial, ph.omega, max_omega, rng, 0.0, false)) {
+                    jreflect = 0;
+                } else {
+                    double n[3];
+                    surface_normal(s, ph.pos, n);
+                    if (dot3(n, ph.dir) < 0)
+                        for (int k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
+                    const double dd = dot3(n, ph.dir);
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = ph.dir[k] - 2.0 * n[k] * dd;
+                }
+            } else {
+                // reflectDiffusive, RT:2332-2380
+                double n[3], rot[9];
+                surface_normal(s, ph.pos, n);
+                if (dot3(n, ph.dir) < 0.0)
+                    for (int k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
+                surface_rotation(g, ph.cur, s, ph.pos, rot);
+                double st = rng.next();                                     // RT:2213
+                st = sqrt(st);
+                const double ct = sqrt(1.0 - st * st);
+                double phd = rng.next();                                    // RT:2218
+                phd = phd * kTwoPi;
+                double sph, cph;
+                pt_sincos(phd, &sph, &cph);
+                const double loc[3] = {st * cph, st * sph, ct};
+                double md[3];
+                for (int k = 0; k < 3; ++k)
+                    md[k] = rot[k] * loc[0] + rot[3 + k] * loc[1] + rot[6 + k] * loc[2];
+                if (dot3(ph.dir, n) > 0.0)
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = -md[k];
+                else
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = md[k];
+                if (dot3(ph.dir, n) > 0.0)
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = -1.0 * ph.dir[k];
+                const double u = rng.next();                                // RT:2361
+                if (determin_transmit(s, g.nmaterial, ph.omega, max_omega, rng, u, true)) {
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = -ph.dir[k];
+                    jreflect = 0;
+                }
+            }
+        }
+    }
+    // nudge, RT:1775-1780
+    for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + PT_NUDGE * ph.dir[k];
+    // FreePath, RT:1783-1790
+    if (mfp == kBallistic) {
+        ph.fp = kBallistic;
+    } else {
+        ph.fp = ph.fp - distance;
+        if (ph.fp < 1.0e-10) ph.fp = -(mfp * pt_log(rng.next()));           // RT:2171
+    }
+    // whichV on transmission, RT:1793-1796
+    if (jreflect == 0) {
+        ph.which = g.surf[ph.cur].gout;
+        if (ph.which <= 0) {            // DO ig=0,whichV,whichV with whichV=0 is undefined
+            ph.flags |= PT_PATH_LOST;
+            ph.end_side = 0;
+            return true;
+        }
+    }
+    return ph.end_side != 0;
+}
+
+__device__ inline void cell_of_path(int64_t pid, int64_t np, int nb, int nangle, int *ib, int *ik,
+                                    int *ia)
+{
+    int64_t cell = pid / np;
+    const int64_t na = nangle - 1;
+    *ia = (int)(cell % na) + 1;
+    cell /= na;
+    *ib = (int)(cell % nb) + 1;
+    *ik = (int)(cell / nb) + 1;
+}
+
+__device__ inline double cell_omega(const DevModes &m, int ib, int ik)
+{
+    return m.omega ? m.omega[(ib - 1) + (int64_t)m.nb * (ik - 1)] : 0.0;
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
+{
+    const int lane = threadIdx.x & 63;
+    Rng rng;
+    rng.seed = P.seed;
+    Path ph;
+    bool active = false;
+    int64_t pid = 0;
+    unsigned long long s_paths = 0, s_events = 0, s_draws = 0, s_lost = 0, s_mis = 0, s_lim = 0;
+    // wave-private slice of the global path queue
+    int64_t chunk_next = 0, chunk_end = 0;
+    bool exhausted = false;
+
+    while (true) {
+        const unsigned long long need = __ballot(!active);
+        if (need != 0ull && !exhausted) {
+            if (chunk_next >= chunk_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(P.queue, (unsigned long long)P.chunk);
+                base = __shfl(base, 0, 64);
+                chunk_next = (int64_t)base;
+                chunk_end = chunk_next + P.chunk < P.total ? chunk_next + P.chunk : P.total;
+                if (chunk_next >= P.total) exhausted = true;
+            }
+            if (!exhausted) {
+                const int64_t avail = chunk_end - chunk_next;
+                const int rank = __popcll(need & ((1ull << lane) - 1ull));
+                const int nneed = __popcll(need);
+                if (!active && rank < avail) {
+                    pid = chunk_next + rank;
+                    int ib, ik, ia;
+                    cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
+                    ph.omega = cell_omega(P.m, ib, ik);
+                    rng.path = (uint64_t)pid;
+                    rng.draw = 0;
+                    if (start_path(P.g, P.mfp, P.cos_t[ia - 1], P.sin_t[ia - 1], rng, ph)) {
+                        active = true;
+                    } else {
+                        P.outcome[pid] = OUT_MISMATCH;
+                        s_paths++;
+                        s_mis++;
+                        s_draws += rng.draw;
+                    }
+                }
+                chunk_next += (avail < nneed ? avail : nneed);
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            if (ph.end_side != 0 ||
+                step_path(P.g, P.mfp, P.m.max_omega, P.event_limit, rng, ph)) {
+                P.outcome[pid] = (ph.end_side != 1) ? OUT_TRANSMITTED : 0;
+                s_paths++;
+                s_events += (unsigned long long)ph.events;
+                s_draws += rng.draw;
+                if (ph.flags & PT_PATH_LOST) s_lost++;
+                if (ph.flags & PT_PATH_EVENT_LIMIT) s_lim++;
+                active = false;
+            }
+        }
+    }
+    s_paths = wave_sum(s_paths);
+    s_events = wave_sum(s_events);
+    s_draws = wave_sum(s_draws);
+    s_lost = wave_sum(s_lost);
+    s_mis = wave_sum(s_mis);
+    s_lim = wave_sum(s_lim);
+    if (lane == 0) {
+        atomicAdd(P.stats + 0, s_paths);
+        atomicAdd(P.stats + 1, s_events);
+        atomicAdd(P.stats + 2, s_draws);
+        atomicAdd(P.stats + 3, s_lost);
+        atomicAdd(P.stats + 4, s_mis);
+        atomicAdd(P.stats + 5, s_lim);
+    }
+}
+
+// Per-cell counts from the outcome bytes: transmitted paths with a particle index
+// below the first start-grain failure of the cell (the reference EXITs its particle
+// loop there, RT:1681).  One block per cell.
+__global__ void __launch_bounds__(256) reduce_grid_kernel(const uint8_t *__restrict__ outcome,
+                                                          int64_t np, int64_t nb, int64_t nk,
+                                                          int32_t nangle, long long *counts,
+                                                          unsigned long long *skipped)
+{
+    __shared__ long long s_first[4];
+    __shared__ long long s_cnt[4];
+    const int64_t cell = blockIdx.x;
+    const uint8_t *o = outcome + cell * np;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    long long first = np;
+    for (int64_t p = threadIdx.x; p < np; p += blockDim.x)
+        if ((o[p] & OUT_MISMATCH) && p < first) first = p;
+    for (int off = 32; off > 0; off >>= 1) {
+        const long long t = __shfl_xor(first, off, 64);
+        first = t < first ? t : first;
+    }
+    if (lane == 0) s_first[wid] = first;
+    __syncthreads();
+    first = s_first[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) first = s_first[w] < first ? s_first[w] : first;
+    long long cnt = 0;
+    for (int64_t p = threadIdx.x; p < first; p += blockDim.x) cnt += (o[p] & OUT_TRANSMITTED) ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    __syncthreads();
+    if (lane == 0) s_cnt[wid] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += s_cnt[w];
+        // cell = ((ik-1)*nb + (ib-1))*(nangle-1) + (ia-1)  ->  transmit(ib,ik,ia)
+        const int64_t na = nangle - 1;
+        const int64_t ia = cell % na, rest = cell / na;
+        const int64_t ib = rest % nb, ik = rest / nb;
+        counts[ib + nb * (ik + nk * ia)] = tot;
+        if (first < np) atomicAdd(skipped, (unsigned long long)(np - first - 1));
+    }
+}
+
+__global__ void __launch_bounds__(64) replay_kernel(ReplayParams P)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int64_t pid = P.first + i;
+    int ib, ik, ia;
+    cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
+    Rng rng;
+    rng.seed = P.seed;
+    rng.path = (uint64_t)pid;
+    rng.draw = 0;
+    Path ph;
+    ph.omega = cell_omega(P.m, ib, ik);
+    if (start_path(P.g, P.mfp, P.cos_t[ia - 1], P.sin_t[ia - 1], rng, ph)) {
+        while (ph.end_side == 0) {
+            if (step_path(P.g, P.mfp, P.m.max_omega, P.event_limit, rng, ph)) break;
+        }
+    }
+    pt_path_record r;
+    r.end_side = ph.end_side;
+    r.flags = ph.flags;
+    r.events = ph.events;
+    r.draws = rng.draw;
+    r.which_grain = ph.which;
+    r.start_grain = ph.start_grain;
+    for (int k = 0; k < 3; ++k) { r.pos[k] = ph.pos[k]; r.dir[k] = ph.dir[k]; }
+    P.rec[i] = r;
+}
+
+hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, hipStream_t s)
+{
+    hipLaunchKernelGGL(trace_grid_kernel, dim3(blocks), dim3(threads), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t trace_grid_occupancy(int threads, int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_grid_kernel, threads, 0);
+}
+
+hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,
+                              int64_t nk, int32_t nangle, long long *counts,
+                              unsigned long long *skipped, hipStream_t s)
+{
+    if (ncell == 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_grid_kernel, dim3((unsigned)ncell), dim3(256), 0, s, outcome, np, nb,
+                       nk, nangle, counts, skipped);
+    return hipGetLastError();
+}
+
+hipError_t launch_replay(const ReplayParams &p, hipStream_t s)
+{
+    if (p.n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((p.n + 63) / 64);
+    hipLaunchKernelGGL(replay_kernel, dim3(blocks), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace pt

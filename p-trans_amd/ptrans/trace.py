@@ -1,0 +1,133 @@
+"""The trace through libptrans: the host-side mirror of TransmissionFunction /
+calcTransmit (src/RTMCphon.f90:1544-1604, 1608-2414).
+
+    with Context(seed) as ctx:
+        geom = ctx.geometry(build_geometry(read_mcp("x.mcp")))
+        modes = ctx.modes(read_anphon("Materials/Si.anphon"))
+        transmit = ctx.transmission_function(geom, modes, mfp_nm=1e10, nangle=11, np_=1000)
+
+transmission_function returns transmit(nbranch, nkpoint, nangle) as the reference's
+module array (count/np per cell, last angle 0), and the raw int64 counts.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .constants import BALLISTIC_MFP, PREC_FP64
+from .geometry import Geometry, modes_desc
+
+
+class Geom:
+    def __init__(self, ctx, geometry: Geometry):
+        self.ctx = ctx
+        self.geometry = geometry
+        self._desc, self._arrays = geometry.desc()
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().pt_geom_create(ctx.handle, ctypes.addressof(self._desc), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _lib.load().pt_geom_destroy(self.handle)
+            self.handle = None
+
+
+class Modes:
+    def __init__(self, ctx, omega, vel=None, mfp=None, weight=None, max_omega=None):
+        self.ctx = ctx
+        omega = np.asarray(omega, dtype=np.float64)
+        vel = omega * 0 if vel is None else vel
+        mfp = omega * 0 if mfp is None else mfp
+        weight = np.ones_like(omega) if weight is None else weight
+        self.nbranch, self.nkpoint = omega.shape
+        self._desc, self._arrays = modes_desc(omega, vel, mfp, weight, max_omega)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().pt_modes_create(ctx.handle, ctypes.addressof(self._desc), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _lib.load().pt_modes_destroy(self.handle)
+            self.handle = None
+
+
+class Context:
+    """One GPU, one stream, one Philox seed (pt_ctx)."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.pt_ctx_create(device, seed, ctypes.byref(h)))
+        self.handle = h
+        self._owned = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        for o in self._owned:
+            o.close()
+        self._owned = []
+        if self.handle:
+            _lib.load().pt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def set_seed(self, seed: int):
+        _lib.check(_lib.load().pt_ctx_set_seed(self.handle, seed))
+
+    def set_event_limit(self, limit: int):
+        _lib.check(_lib.load().pt_ctx_set_event_limit(self.handle, limit))
+
+    def geometry(self, geometry: Geometry) -> Geom:
+        g = Geom(self, geometry)
+        self._owned.append(g)
+        return g
+
+    def modes(self, table=None, omega=None, max_omega=None) -> Modes:
+        if table is not None:
+            m = Modes(self, table.omega, table.vel, table.mfp, table.weight, max_omega)
+        else:
+            m = Modes(self, omega, max_omega=max_omega)
+        self._owned.append(m)
+        return m
+
+    def trace_grid(self, geom: Geom, modes: Modes, mfp_nm: float, nangle: int, np_: int,
+                   vangle=None, precision: int = PREC_FP64):
+        """pt_trace_grid: counts (nb, nk, nangle) int64 and the stats dict."""
+        nb, nk = modes.nbranch, modes.nkpoint
+        counts = np.zeros(nb * nk * nangle, dtype=np.int64)
+        stats = _lib.PtStats()
+        va = None
+        if vangle is not None:
+            va = np.ascontiguousarray(vangle, dtype=np.float64)
+        _lib.check(_lib.load().pt_trace_grid(
+            self.handle, geom.handle, modes.handle, float(mfp_nm), int(nangle),
+            va.ctypes.data if va is not None else None, int(np_), int(precision),
+            counts.ctypes.data, ctypes.byref(stats)))
+        return counts.reshape((nb, nk, nangle), order="F"), stats.as_dict()
+
+    def transmission_function(self, geom: Geom, modes: Modes, mfp_nm: float = BALLISTIC_MFP,
+                              nangle: int = 91, np_: int = 1000, vangle=None):
+        """TransmissionFunction (RT:1544-1604): transmit(ib, ik, ia) = count/np."""
+        counts, stats = self.trace_grid(geom, modes, mfp_nm, nangle, np_, vangle)
+        transmit = counts.astype(np.float64) / float(np_) if np_ > 0 else counts * 0.0
+        return transmit, counts, stats
+
+    def replay(self, geom: Geom, modes: Modes, mfp_nm: float, nangle: int, np_: int,
+               first: int, n: int, vangle=None):
+        """pt_replay: per-path records of paths first .. first+n-1."""
+        out = np.zeros(n, dtype=_lib.RECORD_DTYPE)
+        va = None
+        if vangle is not None:
+            va = np.ascontiguousarray(vangle, dtype=np.float64)
+        _lib.check(_lib.load().pt_replay(
+            self.handle, geom.handle, modes.handle, float(mfp_nm), int(nangle),
+            va.ctypes.data if va is not None else None, int(np_), int(first), int(n),
+            out.ctypes.data))
+        return out

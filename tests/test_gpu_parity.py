@@ -1,0 +1,108 @@
+"""libptrans on the GPU against the oracle and the reference's golden vectors.
+
+Per path (pt_replay): exit side, event count, draw count, final position and direction
+are bit-identical to the oracle with the shared transcendentals (the GPU uses the same
+pt_math.h), and exit/draws/events/REAL(4) position equal what the reference wrote.
+Per cell (pt_trace_grid): counts equal the reference's TransmissionFunction counts.
+At larger sizes: closed forms and invariants.
+"""
+import numpy as np
+import pytest
+
+from _golden import CASES, Case
+from oracle import oracle as O
+from ptrans import gen, geometry
+from ptrans.anphon import synthetic_table
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_case(ctx, c, seed):
+    ctx.set_seed(seed)
+    g = ctx.geometry(c.geom)
+    t = c.table
+    m = ctx.modes(omega=t.omega, max_omega=t.max_omega)
+    return g, m
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_replay_bitexact_vs_oracle_and_reference(gpu_ctx, name):
+    c = Case(name)
+    mt = c.meta
+    print("case", name, flush=True)
+    g, m = _gpu_case(gpu_ctx, c, mt["paths_seed"])
+    rec = gpu_ctx.replay(g, m, mt["mfp"], mt["nangle"], mt["nphonon"], 0, mt["npaths"])
+    orc = O.replay(c.gdesc, c.mdesc, mt["mfp"], mt["nangle"], mt["nphonon"], mt["paths_seed"],
+                   0, mt["npaths"], math=O.MATH_OWN)
+    for f in ("end_side", "flags", "events", "draws", "which_grain", "start_grain"):
+        np.testing.assert_array_equal(rec[f], orc[f], err_msg=f)
+    np.testing.assert_array_equal(rec["pos"].view(np.uint64), orc["pos"].view(np.uint64))
+    np.testing.assert_array_equal(rec["dir"].view(np.uint64), orc["dir"].view(np.uint64))
+    ref = c.ref("own")
+    np.testing.assert_array_equal((rec["end_side"] != 1).astype(np.int8), ref["trans"])
+    np.testing.assert_array_equal(rec["draws"], ref["draws"])
+    np.testing.assert_array_equal(rec["events"], np.diff(ref["traj_off"]) - 1)
+    last = ref["traj"][ref["traj_off"][1:] - 1]
+    np.testing.assert_array_equal(rec["pos"].astype(np.float32), last)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_grid_counts_vs_reference(gpu_ctx, name):
+    c = Case(name)
+    mt = c.meta
+    print("case", name, flush=True)
+    g, m = _gpu_case(gpu_ctx, c, mt["grid_seed"])
+    counts, stats = gpu_ctx.trace_grid(g, m, mt["mfp"], mt["nangle"], mt["nphonon"])
+    np.testing.assert_array_equal(counts, c.ref("own")["grid"])
+    assert stats["paths"] == mt["nbranch"] * mt["nkpoint"] * (mt["nangle"] - 1) * mt["nphonon"]
+    assert stats["lost"] == 0 and stats["start_mismatch"] == 0
+
+
+def test_grid_larger_vs_oracle_and_stats(gpu_ctx):
+    """A few hundred thousand paths of the film-with-holes input with internal
+    scattering: counts and the total event and draw counts equal the oracle's."""
+    model = gen.film_holes(seed=2, nphonon=2000, nangle=7, mfp=(300.0,))
+    geo = geometry.build_geometry(model)
+    t = synthetic_table(3, 10, seed=5)
+    om = t.f_thz / 0.03 * 0.03 * (3.14159265358979323846 * 2.0) * 1.0e12
+    gdesc, _ga = geo.desc()
+    mdesc, _ma = geometry.modes_desc(om, om, om, om)
+    gpu_ctx.set_seed(99)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    counts, stats = gpu_ctx.trace_grid(g, m, 300.0, 7, 2000)
+    oc, ostats = O.trace_grid(gdesc, mdesc, 300.0, 7, 2000, 99, math=O.MATH_OWN)
+    np.testing.assert_array_equal(counts, oc)
+    assert stats["paths"] == ostats[0] and stats["events"] == ostats[1]
+    assert stats["draws"] == ostats[2]
+
+
+def test_specular_wire_transmits_everything(gpu_ctx):
+    """Closed form: a ballistic wire with mirror walls (p = 1) transmits every phonon
+    that enters, at every angle, so every traced cell counts exactly np."""
+    model = gen.nanowire(p_wall=1.0, nphonon=5000, nangle=11)
+    geo = geometry.build_geometry(model)
+    om = np.full((2, 4), 1.0e13)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    counts, stats = gpu_ctx.trace_grid(g, m, 1.0e10, 11, 5000)
+    assert np.all(counts[:, :, :-1] == 5000)
+    assert np.all(counts[:, :, -1] == 0)
+
+
+def test_results_independent_of_run(gpu_ctx):
+    """Counter-based streams: two runs of the same input give identical counts, and a
+    different seed gives different counts."""
+    model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.4, nphonon=500, nangle=5,
+                           mfp=(500.0,))
+    geo = geometry.build_geometry(model)
+    om = np.linspace(1e12, 9e13, 8).reshape(2, 4)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    gpu_ctx.set_seed(5)
+    a, _ = gpu_ctx.trace_grid(g, m, 500.0, 5, 500)
+    b, _ = gpu_ctx.trace_grid(g, m, 500.0, 5, 500)
+    gpu_ctx.set_seed(6)
+    c2, _ = gpu_ctx.trace_grid(g, m, 500.0, 5, 500)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c2)
