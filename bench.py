@@ -75,7 +75,25 @@ def cpu_baseline(geo, omega, mfp, nangle, np_per_cell, seed, seconds, threads):
     for t in ths:
         t.join()
     dt = time.perf_counter() - t0
-    return state["paths"] / dt, state["paths"], state["next"]
+    return state["paths"] / dt, state["paths"], min(state["next"], ncell), dt
+
+
+def roofline(cstats, kernel_ms):
+    """FP64-VALU roofline of trace_grid_kernel: algorithmic FP64 operations of one
+    launch (instrumented twin kernel, same paths) over the timed kernel's average
+    duration (HIP events on the library's stream).  HBM traffic per launch comes from
+    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present."""
+    ops = cstats["fp64_ops"]
+    achieved = ops / (kernel_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "trace_grid_film_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    return {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "fp64_ops_per_launch": ops, "fp64_ops_per_path": ops / max(1, cstats["paths"]),
+            "philox_blocks_per_path": cstats["philox_blocks"] / max(1, cstats["paths"])}
 
 
 def main():
@@ -87,7 +105,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(args.calibrate_copy * 2 ** 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel()}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(paths shard by id range; see DESIGN.md)", file=sys.stderr)
@@ -114,6 +144,10 @@ def main():
             kern_ms.append(stats["kernel_ms"])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        # one more launch of the instrumented twin kernel (untimed) for the FP64 op count
+        ctx.set_op_counting(True)
+        _, cstats = ctx.trace_grid(g, m, mfp, nangle, args.np)
+        ctx.set_op_counting(False)
     value = paths * args.steps / dt
     k_ms = float(np.mean(kern_ms))
     events_per_path = stats["events"] / stats["paths"]
@@ -122,11 +156,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, npaths, ncells = cpu_baseline(geo, omega, mfp, nangle, args.np, seed,
-                                            args.cpu_seconds, threads)
+        rate, npaths, ncells, cdt = cpu_baseline(geo, omega, mfp, nangle, args.np, seed,
+                                                 args.cpu_seconds, threads)
         cpu = {"value": rate, "unit": "paths/s", "cores": threads, "kind": "port",
-               "sample": "%d whole cells (%d paths) of the same workload in %.0f s, oracle C "
-                         "restatement, one thread per cell" % (ncells, npaths, args.cpu_seconds)}
+               "sample": "%d whole cells (%d paths) of the same workload in %.1f s: the oracle's "
+                         "C restatement, %d host threads, one cell per call"
+                         % (ncells, npaths, cdt, threads)}
 
     out = {
         "metric": METRIC,
@@ -147,7 +182,7 @@ def main():
                    "paths_per_step": paths, "events_per_path": events_per_path,
                    "transmission": transmitted},
         "kernel_ms": k_ms,
-        "roofline": None,
+        "roofline": roofline(cstats, k_ms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -130,7 +130,10 @@ typedef struct pt_stats {
     int64_t lost;            /* paths that left the box (RT:1696-1701)               */
     int64_t start_mismatch;  /* start-grain check failures (RT:1673-1682)            */
     int64_t skipped;         /* particles not traced because of such a failure       */
-    double  kernel_ms;       /* device time of the trace kernels                     */
+    double  kernel_ms;       /* device time of the trace kernel (HIP events)         */
+    double  reduce_ms;       /* device time of the per-cell tally reduction          */
+    int64_t fp64_ops;        /* algorithmic FP64 operations (op counting on only)    */
+    int64_t philox_blocks;   /* Philox4x32-10 blocks evaluated (op counting on only) */
 } pt_stats;
 
 /* Per-path record of pt_replay(). */
@@ -171,6 +174,9 @@ int pt_ctx_set_seed(pt_ctx *ctx, uint64_t seed);
 /* Safety stop: a path ends (flag PT_PATH_EVENT_LIMIT) after this many events.  The
  * reference loops without bound (RT:1693); default 1e8. */
 int pt_ctx_set_event_limit(pt_ctx *ctx, int64_t limit);
+/* on != 0: later trace calls run the instrumented kernel that also fills
+ * pt_stats.fp64_ops and philox_blocks (for roofline accounting; slower). */
+int pt_ctx_set_op_counting(pt_ctx *ctx, int32_t on);
 
 /* ---- geometry ------------------------------------------------------------------ */
 /* Uploads vgrain (RT:122) as flattened by the host.  Validates indices. */

@@ -13,6 +13,9 @@
 !       numbering path = cell*Np + p, cell = ((ik-1)*nb + (ib-1))*(nangle-1) + (ia-1);
 !       prints "P path trans draws"; trajectories go to trace.out (RT:1661-1665,
 !       1818-1831), random-number call sites to draws.log.
+!   replay post  <file.mcp> <transmit.txt>
+!       effectiveMFP (RT:2418) on transmit(nb,nk,nangle) blocks read from the file (one
+!       per sampling MFP of the .mcp), then fileoutput (RT:2601); prints SEMFP.
 !   replay geom  <cases.txt>
 !       runs setupSurfaceGeometry, isInsidePolygon, calcSurfaceDistance*,
 !       calcSurfaceNormal* and calcSurfaceRotation* on the cases in the file.
@@ -55,6 +58,8 @@ program replay_driver
     call run_paths()
   case ('geom')
     call run_geom()
+  case ('post')
+    call run_post()
   case default
     write(*,*) 'usage: replay grid|paths|geom ...'
     stop 2
@@ -135,6 +140,28 @@ contains
     call replay_log_close()
     close(fd_trace)
   end subroutine run_paths
+
+  subroutine run_post()
+    character(len=256) :: tfile, mcp
+    integer :: u, ia, is
+    real(8), allocatable :: semfp(:,:,:)
+    call load(status)
+    call getarg(2, mcp)
+    call getarg(3, tfile)
+    do ia = 1, param%nangle          ! as TransmissionFunction, RT:1554-1556
+      param%vangle(ia) = 0.0d0 + dble(ia-1)/dble(param%nangle-1)* M_PI_2
+    end do
+    allocate(semfp(anphon%nkpoint, anphon%nbranch, param%nMeanFreePath))
+    open(newunit=u, file=trim(tfile), action='read')
+    do is = 1, param%nMeanFreePath
+      read(u,*) transmit
+      call effectiveMFP(semfp(:,:,is))
+    end do
+    close(u)
+    write(*,'(a)') 'SEMFP'
+    write(*,*) semfp
+    call fileoutput(semfp, trim(mcp))
+  end subroutine run_post
 
   subroutine run_geom()
     character(len=256) :: casefile

@@ -57,7 +57,8 @@ struct pt_ctx {
     int device = 0;
     uint64_t seed = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    bool count_ops = false;
     int ncu = 0;
     int blocks_per_cu = 0;
     int threads = 256;
@@ -120,6 +121,7 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = trace_grid_occupancy(c->threads, &c->blocks_per_cu);
     if (e != hipSuccess) {
@@ -140,6 +142,7 @@ int pt_ctx_destroy(pt_ctx *c)
     if (c->counters) (void)hipFree(c->counters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PT_OK;
@@ -156,6 +159,13 @@ int pt_ctx_set_event_limit(pt_ctx *c, int64_t limit)
 {
     if (!c || limit < 1) return fail(PT_E_INVALID, "pt_ctx_set_event_limit: bad argument");
     c->event_limit = limit;
+    return PT_OK;
+}
+
+int pt_ctx_set_op_counting(pt_ctx *c, int32_t on)
+{
+    if (!c) return fail(PT_E_INVALID, "pt_ctx_set_op_counting: NULL context");
+    c->count_ops = on != 0;
     return PT_OK;
 }
 
@@ -451,21 +461,23 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     P.queue = c->counters;
     P.stats = c->counters + 1;
     P.outcome = c->outcome;
-    unsigned long long host_ctr[8] = {0};
-    float ms = 0.f;
+    unsigned long long host_ctr[16] = {0};
+    float ms = 0.f, ms_red = 0.f;
     const int blocks = c->ncu * c->blocks_per_cu;
     do {
         if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
-        if ((e = hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace_grid(P, blocks, c->threads, c->stream))) break;
-        if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
+        if ((e = launch_trace_grid(P, blocks, c->threads, c->count_ops, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
+        if ((e = hipEventRecord(c->ev2, c->stream))) break;
         if ((e = hipMemcpyAsync(counts, d_counts, ncount * sizeof(long long), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipMemcpyAsync(host_ctr, c->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipStreamSynchronize(c->stream))) break;
-        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        if ((e = hipEventElapsedTime(&ms, c->ev0, c->ev1))) break;
+        e = hipEventElapsedTime(&ms_red, c->ev1, c->ev2);
     } while (0);
     (void)hipFree(d_ang);
     (void)hipFree(d_counts);
@@ -478,6 +490,9 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         stats->start_mismatch = (int64_t)host_ctr[5];
         stats->skipped = (int64_t)host_ctr[7];
         stats->kernel_ms = ms;
+        stats->reduce_ms = ms_red;
+        stats->fp64_ops = (int64_t)host_ctr[9];
+        stats->philox_blocks = (int64_t)host_ctr[10];
     }
     if ((int64_t)host_ctr[1] != total)
         return fail(PT_E_HIP, "pt_trace_grid: %lld of %lld paths completed",

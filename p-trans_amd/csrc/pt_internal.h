@@ -73,7 +73,8 @@ struct GridParams {
     int32_t chunk;             // paths a wave takes from the queue at once
     int64_t event_limit;
     unsigned long long *queue; // next path id
-    unsigned long long *stats; // paths, events, draws, lost, mismatch, limit
+    unsigned long long *stats; // paths, events, draws, lost, mismatch, limit, -, -,
+                               // fp64 ops, philox blocks (counting kernel only)
     uint8_t *outcome;          // [total]
 };
 
@@ -93,7 +94,8 @@ struct ReplayParams {
 };
 
 // kernels' launch wrappers (pt_trace.hip)
-hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, hipStream_t s);
+hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, bool count,
+                             hipStream_t s);
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
                               int64_t nb, int64_t nk, int32_t nangle,
                               long long *counts, unsigned long long *skipped, hipStream_t s);

@@ -22,6 +22,19 @@ constexpr double kBallistic = 1.0e10;                        // RT:1686
 #define PT_NUDGE ((double)1.0e-7f)                           // REAL(4) literal, RT:1779
 #define PT_SPHERE_EPS ((double)1e-10f)                       // REAL(4) literal, RT:3001
 
+// Algorithmic FP64 operation count of the trace (roofline accounting).  Every add,
+// sub, mul, div and sqrt of the restated arithmetic counts one; comparisons, sign
+// flips by selection and integer work do not.  The per-branch constants below are
+// counted off the code they sit next to.  The timed kernel never stores its OpCount,
+// so the compiler removes the counting; trace_grid_kernel<true> writes it out.
+struct OpCount {
+    unsigned long long flops;
+    unsigned long long philox;     // Philox4x32-10 blocks (integer VALU work)
+};
+
+constexpr unsigned kFlopsLog = 33;     // pt_log: 4 + Horner 20 + 5 + 4
+constexpr unsigned kFlopsSincos = 46;  // pt_sincos: reduction 9, sin 18, cos 19
+
 // Draw d of the lane's path: pt_philox_draw(seed, path, d), one Philox block per
 // two draws.
 struct Rng {
@@ -29,6 +42,7 @@ struct Rng {
     uint64_t path;
     uint32_t draw;
     double spare;
+    OpCount *oc;
     __device__ double next()
     {
         double u;
@@ -36,6 +50,7 @@ struct Rng {
             u = spare;
         } else {
             double b[2];
+            oc->philox++;
             pt_philox_block(seed, path, (uint64_t)(draw >> 1), b);
             u = b[0];
             spare = b[1];
@@ -60,16 +75,19 @@ __device__ inline void cross3(const double *a, const double *b, double *c)
 // calcSurfaceDistancePolygon, RT:2907-2959: one-sided plane hit, then the convex
 // point-in-polygon test with on-edge counted inside.
 __device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
-                             const double *nd, double *lhit)
+                             const double *nd, double *lhit, OpCount &oc)
 {
     const double cosang = nd[0] * s.n[0] + nd[1] * s.n[1] + nd[2] * s.n[2];
+    oc.flops += 5;
     if (cosang <= 0.0) return false;
+    oc.flops += 9;
     const double *v = g.vtx + 3 * (int64_t)s.voff;
     double d[3] = {v[0] - rs[0], v[1] - rs[1], v[2] - rs[2]};
     const double l = (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]) / cosang;
     if (l < 0.0) return false;
     double h[3];
     for (int k = 0; k < 3; ++k) h[k] = rs[k] + nd[k] * l;
+    oc.flops += 6;
     double c1[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < s.nv; ++i) {
         const double *vi = v + 3 * i;
@@ -77,6 +95,7 @@ __device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
         double ra[3], rb[3], ci[3];
         for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = h[k] - vi[k]; }
         cross3(ra, rb, ci);
+        oc.flops += (i == 0) ? 15 : 20;
         if (i == 0) {
             c1[0] = ci[0]; c1[1] = ci[1]; c1[2] = ci[2];
         } else if (dot3(c1, ci) < 0.0) {
@@ -88,7 +107,7 @@ __device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
 }
 
 // isInsidePolygon, RT:2961-2983
-__device__ bool inside_polygon(const double *v, int nv, const double *p)
+__device__ bool inside_polygon(const double *v, int nv, const double *p, OpCount &oc)
 {
     double c1[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < nv; ++i) {
@@ -97,6 +116,7 @@ __device__ bool inside_polygon(const double *v, int nv, const double *p)
         double ra[3], rb[3], ci[3];
         for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = p[k] - vi[k]; }
         cross3(ra, rb, ci);
+        oc.flops += (i == 0) ? 15 : 20;
         if (i == 0) {
             c1[0] = ci[0]; c1[1] = ci[1]; c1[2] = ci[2];
         } else if (dot3(c1, ci) < 0.0) {
@@ -107,16 +127,20 @@ __device__ bool inside_polygon(const double *v, int nv, const double *p)
 }
 
 // calcSurfaceDistanceSphere, RT:2985-3038
-__device__ bool dist_sphere(const DSurf &s, const double *rs, const double *nd, double *lhit)
+__device__ bool dist_sphere(const DSurf &s, const double *rs, const double *nd, double *lhit,
+                            OpCount &oc)
 {
     double rsc[3] = {s.c[0] - rs[0], s.c[1] - rs[1], s.c[2] - rs[2]};
     const double rd = dot3(rsc, nd);
     const double rr = dot3(rsc, rsc);
+    oc.flops += 15;
     if (rr > s.r * s.r - PT_SPHERE_EPS) {
         if (rd <= 0.0) return false;
     }
     const double d = rd * rd + s.r * s.r - rr;
+    oc.flops += 4;
     if (d < 0.0) return false;
+    oc.flops += 3;
     const double sq = sqrt(d);
     const double l1 = rd - sq, l2 = rd + sq;
     if (l1 > 0.0) *lhit = l1;
@@ -127,7 +151,7 @@ __device__ bool dist_sphere(const DSurf &s, const double *rs, const double *nd, 
 
 // calcSurfaceDistanceCylinder, RT:3040-3088 (z axis, clipped to 0 <= z <= boxz)
 __device__ bool dist_cylinder(const DSurf &s, const double *rs, const double *nd, double boxz,
-                              double *lhit)
+                              double *lhit, OpCount &oc)
 {
     const double rsc0 = s.c[0] - rs[0], rsc1 = s.c[1] - rs[1];
     double t0 = nd[0], t1 = nd[1];
@@ -138,7 +162,9 @@ __device__ bool dist_cylinder(const DSurf &s, const double *rs, const double *nd
     const double rd = rsc0 * t0 + rsc1 * t1;
     const double rr = rsc0 * rsc0 + rsc1 * rsc1;
     const double d = rd * rd + s.r * s.r - rr;
+    oc.flops += 20;
     if (d < 0.0) return false;
+    oc.flops += 3;
     const double sq = sqrt(d);
     const double l1 = rd - sq, l2 = rd + sq;
     double l;
@@ -147,24 +173,25 @@ __device__ bool dist_cylinder(const DSurf &s, const double *rs, const double *nd
     else return false;
     l = l / sqrt(nd[0] * nd[0] + nd[1] * nd[1]);
     const double zh = rs[2] + nd[2] * l;
+    oc.flops += 7;
     if (zh < 0.0 || boxz < zh) return false;
     *lhit = l;
     return true;
 }
 
 __device__ inline bool surface_distance(const DevGeom &g, const DSurf &s, const double *p,
-                                        const double *d, double *l)
+                                        const double *d, double *l, OpCount &oc)
 {
     switch (s.kind) {
-    case PT_SHAPE_POLYGON: return dist_polygon(g, s, p, d, l);
-    case PT_SHAPE_SPHERE: return dist_sphere(s, p, d, l);
-    case PT_SHAPE_CYLINDER: return dist_cylinder(s, p, d, g.box[2], l);
+    case PT_SHAPE_POLYGON: return dist_polygon(g, s, p, d, l, oc);
+    case PT_SHAPE_SPHERE: return dist_sphere(s, p, d, l, oc);
+    case PT_SHAPE_CYLINDER: return dist_cylinder(s, p, d, g.box[2], l, oc);
     }
     return false;
 }
 
 // calcSurfaceNormal*, RT:3119-3179
-__device__ void surface_normal(const DSurf &s, const double *pos, double *n)
+__device__ void surface_normal(const DSurf &s, const double *pos, double *n, OpCount &oc)
 {
     if (s.kind == PT_SHAPE_POLYGON) {
         n[0] = s.n[0]; n[1] = s.n[1]; n[2] = s.n[2];
@@ -175,13 +202,14 @@ __device__ void surface_normal(const DSurf &s, const double *pos, double *n)
     const double dd = dot3(rcs, rcs);
     const double inv = 1.0 / sqrt(dd);
     for (int k = 0; k < 3; ++k) n[k] = rcs[k] * inv;
+    oc.flops += 14;
     if (dd > s.r * s.r)
         for (int k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
 }
 
 // calcSurfaceRotation*, RT:3181-3301 (column-major 3x3)
 __device__ void surface_rotation(const DevGeom &g, int idx, const DSurf &s, const double *pos,
-                                 double *rot)
+                                 double *rot, OpCount &oc)
 {
     if (s.kind == PT_SHAPE_POLYGON) {
         const double *t = g.tang + 9 * (int64_t)idx;
@@ -194,6 +222,7 @@ __device__ void surface_rotation(const DevGeom &g, int idx, const DSurf &s, cons
     const double inv = 1.0 / sqrt(dd);
     for (int k = 0; k < 3; ++k) a[k] = a[k] * inv;
     rot[6] = a[0]; rot[7] = a[1]; rot[8] = a[2];
+    oc.flops += 15;
     const bool axis = (s.kind == PT_SHAPE_SPHERE) ? (fabs(a[2]) == 1.0) : (fabs(a[0]) == 1.0);
     if (axis || (a[0] * a[0] == 0.0 && a[1] * a[1] == 0.0)) {
         rot[3] = 0.0; rot[4] = 1.0; rot[5] = 0.0;
@@ -204,20 +233,22 @@ __device__ void surface_rotation(const DevGeom &g, int idx, const DSurf &s, cons
         rot[4] = a[0] / r;
         rot[5] = 0.0;
         cross3(rot + 3, rot + 6, rot);
+        oc.flops += 15;
     }
 }
 
 // isInsideSurface / isInsideGrain / findInsideWhichGrain, RT:1874-1985
-__device__ bool inside_surface(const DSurf &s, const double *p)
+__device__ bool inside_surface(const DSurf &s, const double *p, OpCount &oc)
 {
     double d[3] = {p[0] - s.c[0], p[1] - s.c[1], p[2] - s.c[2]};
+    oc.flops += 8;
     if (s.kind == PT_SHAPE_POLYGON) return dot3(d, s.n) < 0;
     if (s.kind == PT_SHAPE_SPHERE) return dot3(d, d) < s.r * s.r;
     if (s.kind == PT_SHAPE_CYLINDER) return d[0] * d[0] + d[1] * d[1] < s.r * s.r;
     return false;
 }
 
-__device__ int find_inside_which_grain(const DevGeom &g, const double *p)
+__device__ int find_inside_which_grain(const DevGeom &g, const double *p, OpCount &oc)
 {
     int count = 0, which = -1;
     for (int ig = 1; ig <= g.ngrain; ++ig) {
@@ -225,7 +256,7 @@ __device__ int find_inside_which_grain(const DevGeom &g, const double *p)
         // the candidate list of a grain is its internal surfaces; incident ones are
         // never internal (RT:1108-1111), so this is exactly isInsideGrain's set
         for (int j = g.cand_off[ig]; j < g.cand_off[ig + 1]; ++j) {
-            if (!inside_surface(g.surf[g.cand_idx[j]], p)) { in = false; break; }
+            if (!inside_surface(g.surf[g.cand_idx[j]], p, oc)) { in = false; break; }
         }
         if (in) { which = ig; ++count; }
     }
@@ -236,7 +267,7 @@ __device__ int find_inside_which_grain(const DevGeom &g, const double *p)
 }
 
 // findNearestIncidentGrain, RT:2100-2118
-__device__ int nearest_incident_grain(const DevGeom &g, const double *p)
+__device__ int nearest_incident_grain(const DevGeom &g, const double *p, OpCount &oc)
 {
     double d2min = 1.0e10;
     int best = 0;
@@ -245,20 +276,21 @@ __device__ int nearest_incident_grain(const DevGeom &g, const double *p)
         const double *q = g.gen + 3 * ig;
         const double d0 = p[0] - q[0], d1 = p[1] - q[1], d2 = p[2] - q[2];
         const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+        oc.flops += 8;
         if (d2min > dd) { d2min = dd; best = ig; }
     }
     return best;
 }
 
 // findIncidentCollisionGrain, RT:2131-2163 (the last grain with a containing facet wins)
-__device__ int incident_collision_grain(const DevGeom &g, const double *p)
+__device__ int incident_collision_grain(const DevGeom &g, const double *p, OpCount &oc)
 {
     int found = -1;
     for (int ig = 1; ig <= g.ngrain; ++ig) {
         if (!g.grain_incident[ig]) continue;
         for (int j = g.inc_off[ig]; j < g.inc_off[ig + 1]; ++j) {
             const DSurf &s = g.surf[g.inc_idx[j]];
-            if (inside_polygon(g.vtx + 3 * (int64_t)s.voff, s.nv, p)) { found = ig; break; }
+            if (inside_polygon(g.vtx + 3 * (int64_t)s.voff, s.nv, p, oc)) { found = ig; break; }
         }
     }
     return found >= 0 ? found : g.igrain_fill;
@@ -291,10 +323,12 @@ __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, d
     case PT_TRANS_TBC:
         if (nmaterial == 1) {
             threshold = 1.0 / (s.tparam * omega / max_omega + 1.0);       // RT:1417-1418
+            rng.oc->flops += 4;
         } else {
             const double t11 = 1.0 / (s.tparam + 1.0);                     // RT:1423
             const double om = rng.next();                                  // RT:2399
             threshold = t11 + om * (1 - t11);
+            rng.oc->flops += 5;
         }
         break;
     default:
@@ -309,6 +343,8 @@ __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, d
 __device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double sin_t, Rng &rng,
                            Path &ph)
 {
+    OpCount &oc = *rng.oc;
+    oc.flops += 7 + kFlopsSincos;
     const double y = rng.next(), z = rng.next();                            // RT:1850-1851
     ph.pos[0] = 0.0;
     ph.pos[1] = y * g.box[1] * g.scope[0];
@@ -325,16 +361,16 @@ __device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double si
     ph.flags = 0;
     ph.events = 0;
     int which = 0;
-    if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos);
-    else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos);
+    if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
+    else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos, oc);
     ph.which = which;
     ph.start_grain = which;
-    if (find_inside_which_grain(g, ph.pos) != which) {                      // RT:1673-1682
+    if (find_inside_which_grain(g, ph.pos, oc) != which) {                      // RT:1673-1682
         ph.flags = PT_PATH_START_MISMATCH;
         return false;
     }
     if (mfp == kBallistic) ph.fp = kBallistic;                              // RT:1686-1690
-    else ph.fp = -(mfp * pt_log(rng.next()));                               // RT:2171-2172
+    else { ph.fp = -(mfp * pt_log(rng.next())); oc.flops += 1 + kFlopsLog; } // RT:2171-2172
     return true;
 }
 
@@ -342,6 +378,8 @@ __device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double si
 __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_t event_limit,
                           Rng &rng, Path &ph)
 {
+    OpCount &oc = *rng.oc;
+    oc.flops += 3;                       // box + SMALL_VAL
     // isInsideBox, RT:2177-2187
     if (!(ph.pos[0] >= 0.0 - kSmallVal && ph.pos[0] <= g.box[0] + kSmallVal &&
           ph.pos[1] >= 0.0 - kSmallVal && ph.pos[1] <= g.box[1] + kSmallVal &&
@@ -366,7 +404,7 @@ __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_
         for (int j = g.cand_off[ig]; j < j1; ++j) {
             const int idx = g.cand_idx[j];
             double l;
-            if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l) && dmin > l) {
+            if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) && dmin > l) {
                 dmin = l;
                 best = idx;
             }
@@ -380,6 +418,7 @@ __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_
         // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
         distance = ph.fp;
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
+        oc.flops += 6;
         double ct = rng.next();                                             // RT:2197
         double ph_ = rng.next();                                            // RT:2198
         ct = ct * 2.0 - 1.0;
@@ -389,8 +428,10 @@ __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_
         pt_sincos(ph_, &sph, &cph);
         ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
         ph.cur = -1;
+        oc.flops += 8 + kFlopsSincos;    // + 6 above
     } else {
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
+        oc.flops += 6;
         if (ph.cur == 0) {
             ph.end_side = 1;
         } else if (ph.cur == 1) {
@@ -408,19 +449,21 @@ __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_
                     jreflect = 0;
                 } else {
                     double n[3];
-                    surface_normal(s, ph.pos, n);
+                    surface_normal(s, ph.pos, n, oc);
                     if (dot3(n, ph.dir) < 0)
                         for (int k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
                     const double dd = dot3(n, ph.dir);
                     for (int k = 0; k < 3; ++k) ph.dir[k] = ph.dir[k] - 2.0 * n[k] * dd;
+                    oc.flops += 19;
                 }
             } else {
                 // reflectDiffusive, RT:2332-2380
                 double n[3], rot[9];
-                surface_normal(s, ph.pos, n);
+                surface_normal(s, ph.pos, n, oc);
                 if (dot3(n, ph.dir) < 0.0)
                     for (int k = 0; k < 3; ++k) n[k] = -1.0 * n[k];
-                surface_rotation(g, ph.cur, s, ph.pos, rot);
+                surface_rotation(g, ph.cur, s, ph.pos, rot, oc);
+                oc.flops += 5 + 4 + 1 + kFlopsSincos + 2 + 15 + 10;
                 double st = rng.next();                                     // RT:2213
                 st = sqrt(st);
                 const double ct = sqrt(1.0 - st * st);
@@ -448,12 +491,17 @@ __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_
     }
     // nudge, RT:1775-1780
     for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + PT_NUDGE * ph.dir[k];
+    oc.flops += 6;
     // FreePath, RT:1783-1790
     if (mfp == kBallistic) {
         ph.fp = kBallistic;
     } else {
         ph.fp = ph.fp - distance;
-        if (ph.fp < 1.0e-10) ph.fp = -(mfp * pt_log(rng.next()));           // RT:2171
+        oc.flops += 1;
+        if (ph.fp < 1.0e-10) {
+            ph.fp = -(mfp * pt_log(rng.next()));                             // RT:2171
+            oc.flops += 1 + kFlopsLog;
+        }
     }
     // whichV on transmission, RT:1793-1796
     if (jreflect == 0) {
@@ -491,10 +539,13 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
 
 }  // namespace
 
+template <bool kCount>
 __global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
 {
     const int lane = threadIdx.x & 63;
+    OpCount oc = {0ull, 0ull};
     Rng rng;
+    rng.oc = &oc;
     rng.seed = P.seed;
     Path ph;
     bool active = false;
@@ -569,6 +620,13 @@ __global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
         atomicAdd(P.stats + 4, s_mis);
         atomicAdd(P.stats + 5, s_lim);
     }
+    if (kCount) {
+        const unsigned long long f = wave_sum(oc.flops), b = wave_sum(oc.philox);
+        if (lane == 0) {
+            atomicAdd(P.stats + 8, f);
+            atomicAdd(P.stats + 9, b);
+        }
+    }
 }
 
 // Per-cell counts from the outcome bytes: transmitted paths with a particle index
@@ -620,7 +678,9 @@ __global__ void __launch_bounds__(64) replay_kernel(ReplayParams P)
     const int64_t pid = P.first + i;
     int ib, ik, ia;
     cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
+    OpCount oc = {0ull, 0ull};
     Rng rng;
+    rng.oc = &oc;
     rng.seed = P.seed;
     rng.path = (uint64_t)pid;
     rng.draw = 0;
@@ -642,15 +702,20 @@ __global__ void __launch_bounds__(64) replay_kernel(ReplayParams P)
     P.rec[i] = r;
 }
 
-hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, hipStream_t s)
+hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, bool count,
+                             hipStream_t s)
 {
-    hipLaunchKernelGGL(trace_grid_kernel, dim3(blocks), dim3(threads), 0, s, p);
+    if (count)
+        hipLaunchKernelGGL(trace_grid_kernel<true>, dim3(blocks), dim3(threads), 0, s, p);
+    else
+        hipLaunchKernelGGL(trace_grid_kernel<false>, dim3(blocks), dim3(threads), 0, s, p);
     return hipGetLastError();
 }
 
 hipError_t trace_grid_occupancy(int threads, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_grid_kernel, threads, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_grid_kernel<false>,
+                                                        threads, 0);
 }
 
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libptrans.so")
 EXPORTS = (
     "pt_version", "pt_build_info", "pt_last_error",
     "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_set_seed", "pt_ctx_set_event_limit",
+    "pt_ctx_set_op_counting",
     "pt_geom_create", "pt_geom_destroy", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
     "pt_trace_grid", "pt_replay",
@@ -26,7 +27,9 @@ EXPORTS = (
 class PtStats(ctypes.Structure):
     _fields_ = [("paths", ctypes.c_int64), ("events", ctypes.c_int64), ("draws", ctypes.c_int64),
                 ("lost", ctypes.c_int64), ("start_mismatch", ctypes.c_int64),
-                ("skipped", ctypes.c_int64), ("kernel_ms", ctypes.c_double)]
+                ("skipped", ctypes.c_int64), ("kernel_ms", ctypes.c_double),
+                ("reduce_ms", ctypes.c_double), ("fp64_ops", ctypes.c_int64),
+                ("philox_blocks", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -64,6 +67,7 @@ def load():
     L.pt_ctx_destroy.argtypes = [vp]
     L.pt_ctx_set_seed.argtypes = [vp, u64]
     L.pt_ctx_set_event_limit.argtypes = [vp, i64]
+    L.pt_ctx_set_op_counting.argtypes = [vp, i32]
     L.pt_geom_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
     L.pt_geom_destroy.argtypes = [vp]
     L.pt_surface_setup_polygon.argtypes = [i32, vp, vp, vp, vp]

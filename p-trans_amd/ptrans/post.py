@@ -1,0 +1,233 @@
+"""From transmission to effective mean free paths and thermal conductivity: the host
+steps after the trace, effectiveMFP (src/RTMCphon.f90:2418-2451) and fileoutput
+(RT:2601-2881), with the reference's arithmetic order and output formats, plus the
+standard error of kappa that the reference does not report.
+
+Units as in the reference: SEMFP in nm, bulk and effective MFP inside fileoutput in m,
+kappa in W m^-1 K^-1 when the table's volume is in m^3.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import List, Optional
+
+import numpy as np
+
+from .constants import HBAR, KB, M_PI_2
+
+
+def vangle(nangle: int) -> np.ndarray:
+    """param%vangle, RT:1554-1556."""
+    return np.array([0.0 + float(ia - 1) / float(nangle - 1) * M_PI_2 for ia in range(1, nangle + 1)])
+
+
+def effective_mfp(transmit: np.ndarray, angles: np.ndarray, boxx: float) -> np.ndarray:
+    """effectiveMFP (RT:2418-2451): composite Simpson of sin(t)cos(t)T(t) over the
+    angle grid, times (3/2)L.  transmit is (nb, nk, nangle); returns EMFP (nk, nb) in the
+    reference's layout (SEMFP(:,:,s)).  The shared WT array of the reference's OpenMP
+    loop (RT:2423, 2427) is a data race; this is its single-threaded meaning."""
+    nb, nk, na = transmit.shape
+    ds = (angles[1] - angles[0]) / 3.0 * 3.0 * 0.5 * boxx
+    sc = np.array([math.sin(a) * math.cos(a) for a in angles])
+    out = np.zeros((nk, nb))
+    for ik in range(nk):
+        for ib in range(nb):
+            wt = sc * transmit[ib, ik, :]
+            acc = 0.0
+            for ia in range(1, na - 1, 2):            # ia = 2, 4, ..., nangle-1 (1-based)
+                acc = acc + (wt[ia - 1] + 4.0 * wt[ia] + wt[ia + 1])
+            out[ik, ib] = acc * ds
+    return out
+
+
+def heat_capacity(omega: float, temperature: float) -> float:
+    """cap(ii,jj), RT:2696-2701."""
+    x = HBAR * omega / (KB * temperature)
+    if x <= 1.0e-10:
+        return KB
+    return KB * ((x / 2) / math.sinh(x / 2)) ** 2
+
+
+@dataclasses.dataclass
+class KappaResult:
+    kappa: float                 # sum of Thermal_conductivity (RT:2754-2755)
+    mode_kappa: np.ndarray       # (nb, nk) after the reference's per-branch sort by omega
+    emfp_m: np.ndarray           # (nb, nk) effective MFP [m]
+    bulk_mfp_m: np.ndarray       # (nb, nk)
+    omega: np.ndarray            # (nb, nk) sorted
+    kappa_stderr: Optional[float] = None
+
+
+def kappa(semfp: np.ndarray, table, temperature: float, mfp_samples: List[float],
+          matthiessen: bool, counts: Optional[np.ndarray] = None, np_: int = 0,
+          angles: Optional[np.ndarray] = None, boxx: float = 0.0) -> KappaResult:
+    """fileoutput's conductivity (RT:2645-2755) for nmaterial = 1.
+
+    semfp: (nk, nb, nMFP) in nm.  table: an Anphon (omega rad/s, tau ps, vel, weight).
+    If counts (nb, nk, nangle[, nMFP]) and np_ are given, also the standard error of
+    kappa from the binomial variance T(1-T)/np of every traced cell, propagated
+    linearly through Simpson and the kappa sum (Matthiessen: to first order)."""
+    nb, nk = table.omega.shape
+    rtemp = table.rtemp
+    tau = table.tau.copy()
+    vel = table.vel.copy()
+    omega = table.omega.copy()
+    weight = table.weight.copy()
+    semfp = semfp.copy()
+    mfp = np.zeros((nb, nk))
+    for ii in range(nk):
+        for jj in range(nb):
+            tau[jj, ii] = tau[jj, ii] * 1.0e-12 * (rtemp / temperature)
+            mfp[jj, ii] = tau[jj, ii] * vel[jj, ii]
+    nqfbz = int(weight.sum(dtype=np.float64) / nb)
+    # per-branch exchange sort by omega (RT:2658-2691)
+    for ii in range(nb):
+        for jj in range(nk - 1):
+            for kk in range(jj + 1, nk):
+                if omega[ii, jj] > omega[ii, kk]:
+                    for a in (omega, weight, tau, vel, mfp):
+                        a[ii, jj], a[ii, kk] = a[ii, kk], a[ii, jj]
+                    semfp[jj, ii, :], semfp[kk, ii, :] = semfp[kk, ii, :].copy(), semfp[jj, ii, :].copy()
+    cap = np.array([[heat_capacity(omega[ii, jj], temperature) for jj in range(nk)] for ii in range(nb)])
+    emfp = np.zeros((nb, nk))
+    tc = np.zeros((nb, nk))
+    vm = list(mfp_samples)
+    for ii in range(nb):
+        for jj in range(nk):
+            if matthiessen:
+                if mfp[ii, jj] == 0.0:
+                    emfp[ii, jj] = semfp[jj, ii, 0] * 1.0e-9
+                else:
+                    b = semfp[jj, ii, 0] * 1.0e-9
+                    inv_b = 1.0 / b if b != 0.0 else math.inf
+                    emfp[ii, jj] = 1.0 / (1.0 / mfp[ii, jj] + inv_b)
+            else:
+                m = mfp[ii, jj]
+                if m < vm[0] * 1.0e-9:
+                    emfp[ii, jj] = semfp[jj, ii, 0] * 1.0e-9 * m / (vm[0] * 1.0e-9)
+                elif m > vm[-1] * 1.0e-9:
+                    emfp[ii, jj] = semfp[jj, ii, len(vm) - 1] * 1.0e-9
+                else:
+                    for s in range(len(vm) - 1):
+                        if vm[s] * 1.0e-9 <= m < vm[s + 1] * 1.0e-9:
+                            t = (m - vm[s] * 1.0e-9) / ((vm[s + 1] - vm[s]) * 1.0e-9)
+                            emfp[ii, jj] = (semfp[jj, ii, s] * 1.0e-9 * (1.0 - t)
+                                            + semfp[jj, ii, s + 1] * 1.0e-9 * t)
+                            break
+            tc[ii, jj] = cap[ii, jj] * weight[ii, jj] * emfp[ii, jj] * vel[ii, jj] / 3 / table.vol / nqfbz
+    total = 0.0
+    for jj in range(nk):                 # column-major order of sum(Thermal_conductivity)
+        for ii in range(nb):
+            total = total + tc[ii, jj]
+    res = KappaResult(kappa=total, mode_kappa=tc, emfp_m=emfp, bulk_mfp_m=mfp, omega=omega)
+    if counts is not None and np_ > 0 and angles is not None and matthiessen:
+        res.kappa_stderr = _kappa_stderr_matthiessen(counts, np_, angles, boxx, semfp, mfp,
+                                                     cap, weight, vel, table.vol, nqfbz,
+                                                     table.omega, omega)
+    return res
+
+
+def _kappa_stderr_matthiessen(counts, np_, angles, boxx, semfp, mfp, cap, weight, vel, vol,
+                              nqfbz, omega_unsorted, omega_sorted):
+    """sigma_kappa with independent binomial cells: Var T = T(1-T)/np; SEMFP is linear in
+    T (Simpson weights w_a), and d kappa_m / d SEMFP_m = c_m * (EMFP_m/B_m)^2 * 1e-9
+    with B = SEMFP*1e-9 (Matthiessen)."""
+    nb, nk, na = counts.shape
+    T = counts / float(np_)
+    var_t = T * (1.0 - T) / float(np_)
+    ds = (angles[1] - angles[0]) / 3.0 * 3.0 * 0.5 * boxx
+    sc = np.sin(angles) * np.cos(angles)
+    w = np.zeros(na)
+    for ia in range(1, na - 1, 2):
+        w[ia - 1] += 1.0
+        w[ia] += 4.0
+        w[ia + 1] += 1.0
+    w = w * sc * ds
+    var_semfp = (var_t * w ** 2).sum(axis=2)          # (nb, nk), unsorted order
+    total = 0.0
+    for ii in range(nb):
+        order = np.argsort(omega_unsorted[ii], kind="stable")
+        for jj in range(nk):
+            b = semfp[jj, ii, 0] * 1.0e-9
+            m = mfp[ii, jj]
+            if b <= 0.0 or m == 0.0:
+                continue
+            e = 1.0 / (1.0 / m + 1.0 / b)
+            c = cap[ii, jj] * weight[ii, jj] * vel[ii, jj] / 3 / vol / nqfbz
+            d = c * (e / b) ** 2 * 1.0e-9
+            total += d * d * var_semfp[ii, order[jj]]
+    return math.sqrt(total)
+
+
+# ------------------------------------------------------------------ output files
+
+def fortran_e(x: float, w: int, d: int) -> str:
+    """Fortran Ew.d: 0.ddddE+xx, right-justified in w columns."""
+    if x == 0.0:
+        s = "0." + "0" * d + "E+00"
+    else:
+        t = "%.*e" % (d - 1, abs(x))
+        mant, ex = t.split("e")
+        digits = mant.replace(".", "")
+        e = int(ex) + 1
+        s = ("-" if x < 0 else "") + "0." + digits + ("E%+03d" % e)
+    return s.rjust(w)
+
+
+def write_outputs(res: KappaResult, directory: str, nphonon: int, semfp: np.ndarray,
+                  temperature: float, nbranch: int, nkpoint: int) -> None:
+    """thermalproperty.out, effectiveMFP.out, phononconductivity.out (RT:2634-2871)."""
+    ave = float(semfp.sum()) / (nbranch * nkpoint * semfp.shape[2])
+    with open(os.path.join(directory, "thermalproperty.out"), "w") as f:
+        f.write("# Datatype = thermalproperty\n")
+        f.write("# Incident_Phonon                               =   %6d\n" % nphonon)
+        f.write("# Averaged Effective_MeanFreePath [nm]          = %20.10f\n" % ave)
+        f.write("# Thermal_conductivity@%sK [W m-1 K-1]  = %20.10f\n"
+                % (("%.6f" % temperature), res.kappa))
+    nb, nk = res.mode_kappa.shape
+    recs = [(ii + 1, jj + 1, res.bulk_mfp_m[ii, jj], res.emfp_m[ii, jj], res.mode_kappa[ii, jj])
+            for ii in range(nb) for jj in range(nk)]
+    n = len(recs)
+    for i in range(n):                                      # RT:2775-2784
+        for j in range(i + 1, n):
+            if recs[i][3] > recs[j][3]:
+                recs[i], recs[j] = recs[j], recs[i]
+    acc = []
+    for i, r in enumerate(recs):
+        acc.append(r[4] if i == 0 else r[4] + acc[-1])
+    with open(os.path.join(directory, "effectiveMFP.out"), "w") as f:
+        f.write("# Datatype = Effective Mean Free Path of phonon.\n")
+        f.write("# data are sorted by effective MFP in ascending order.\n")
+        f.write("#   total branches: %3d\n" % nb)
+        f.write("#   total Kpoints:  %3d\n" % nk)
+        f.write("#\n")
+        f.write("# branch, kpoint, bulk MFP[nm], effective MFP[nm], ThermalConductivity, "
+                "and its accumulated[W m-1 K-1].\n")
+        for r, a in zip(recs, acc):
+            f.write("%5d%5d%s%s%s%s\n" % (r[0], r[1], fortran_e(r[2] * 1.0e9, 15, 6),
+                                          fortran_e(r[3] * 1.0e9, 15, 6), fortran_e(r[4], 15, 6),
+                                          fortran_e(a, 15, 6)))
+    freqs = []
+    for ii in range(nb):
+        for jj in range(nk):
+            o = res.omega[ii, jj]
+            if o not in freqs:
+                freqs.append(o)
+    freqs.sort()
+    nf = len(freqs) - 1                                     # the last one is dropped (RT:2841)
+    spec = np.zeros((max(nf, 0), nb))
+    for i in range(nf):
+        for jj in range(nb):
+            for kk in range(nk):
+                if res.omega[jj, kk] == freqs[i]:
+                    spec[i, jj] += res.mode_kappa[jj, kk]
+    with open(os.path.join(directory, "phononconductivity.out"), "w") as f:
+        f.write("# Datatype = phononconductivity\n")
+        f.write("# Nb = %6d\n" % nf)
+        f.write("# Nbranch = %4d\n" % nb)
+        f.write("# omega, fluxphononout(1:Nbranch)\n")
+        for i in range(nf):
+            f.write(fortran_e(freqs[i], 20, 10) + "".join(fortran_e(spec[i, j], 20, 10)
+                                                          for j in range(nb)) + "\n")

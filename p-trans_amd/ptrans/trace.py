@@ -84,6 +84,9 @@ class Context:
     def set_event_limit(self, limit: int):
         _lib.check(_lib.load().pt_ctx_set_event_limit(self.handle, limit))
 
+    def set_op_counting(self, on: bool):
+        _lib.check(_lib.load().pt_ctx_set_op_counting(self.handle, int(bool(on))))
+
     def geometry(self, geometry: Geometry) -> Geom:
         g = Geom(self, geometry)
         self._owned.append(g)

@@ -194,9 +194,70 @@ def run_geom():
         fh.write(out.stdout)
 
 
+# post-processing cases: name -> (model, table (nb, nk, seed), np, seed)
+POST_CASES = {
+    "post_matthiessen": (lambda: gen.nanowire(p_wall=0.0, nphonon=60, nangle=11), (3, 12, 21), 60, 31),
+    "post_mfp_log": (lambda: gen.film_holes(seed=2, nphonon=40, nangle=7, mfp=(10.0, 1000.0, 3, "log")),
+                     (2, 15, 22), 40, 32),
+}
+
+
+def run_post(name, spec):
+    """effectiveMFP + fileoutput of the reference on transmission tables traced by the
+    oracle (which is pinned to the reference by the path cases above)."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    from ptrans import geometry
+    factory, (nb, nk, tseed), np_, seed = spec
+    model = factory()
+    outdir = os.path.join(HERE, name)
+    os.makedirs(outdir, exist_ok=True)
+    mcp.write_mcp(os.path.join(outdir, "input.mcp"), model)
+    table = anphon.synthetic_table(nb, nk, tseed)
+    anphon.write_anphon(os.path.join(outdir, "Si.anphon"), table)
+    tab = anphon.read_anphon(os.path.join(outdir, "Si.anphon"), "Si")
+    geo = geometry.build_geometry(model)
+    gdesc, _ga = geo.desc()
+    mdesc, _ma = geometry.modes_desc(tab.omega, tab.vel, tab.mfp, tab.weight)
+    blocks = []
+    for mfp in model.mean_free_paths():
+        counts, _ = O.trace_grid(gdesc, mdesc, mfp, model.nangle, np_, seed)
+        blocks.append(counts)
+    counts = np.stack(blocks, axis=-1)
+    np.save(os.path.join(outdir, "counts.npy"), counts)
+    with open(os.path.join(outdir, "transmit.txt"), "w") as f:
+        for k in range(counts.shape[-1]):
+            t = counts[..., k].astype(np.float64) / float(np_)
+            f.write(" ".join("%.17g" % x for x in t.reshape(-1, order="F")) + "\n")
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "Materials"))
+        shutil.copy(os.path.join(outdir, "input.mcp"), os.path.join(tmp, "input.mcp"))
+        shutil.copy(os.path.join(outdir, "Si.anphon"), os.path.join(tmp, "Materials", "Si.anphon"))
+        shutil.copy(os.path.join(outdir, "transmit.txt"), os.path.join(tmp, "transmit.txt"))
+        out = subprocess.run([os.path.join(REF_DIR, "replay_libm"), "post", "input.mcp",
+                              "transmit.txt"], cwd=tmp, env=dict(os.environ, OMP_NUM_THREADS="1"),
+                             capture_output=True, text=True, timeout=120, check=True)
+        lines = out.stdout.splitlines()
+        k = lines.index("SEMFP")
+        vals = []
+        for ln in lines[k + 1:]:
+            try:
+                vals.extend(float(x) for x in ln.split())
+            except ValueError:
+                break
+        semfp = np.array(vals[:nk * nb * counts.shape[-1]]).reshape((nk, nb, counts.shape[-1]), order="F")
+        np.save(os.path.join(outdir, "semfp_ref.npy"), semfp)
+        for fn in ("thermalproperty.out", "effectiveMFP.out", "phononconductivity.out"):
+            shutil.copy(os.path.join(tmp, fn), os.path.join(outdir, fn))
+    return semfp
+
+
 def main():
     if not os.path.exists(os.path.join(REF_DIR, "replay_own")):
         sys.exit("oracle/_ref is not built: make -C oracle ref (needs /root/reference)")
+    for name, spec in POST_CASES.items():
+        sem = run_post(name, spec)
+        print("%-22s SEMFP mean %.6g" % (name, sem.mean()))
     run_geom()
     for name, spec in CASES.items():
         for tag in ("own", "libm"):
