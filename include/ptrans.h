@@ -111,15 +111,19 @@ typedef struct pt_geom_desc {
     const double     *vertices;       /* [nvertex*3]                                   */
 } pt_geom_desc;
 
-/* Bulk phonon table (Anphon_type, src/RTMCphon.f90:161-174), column-major (nb, nk). */
+/* Bulk phonon table (Anphon_type, src/RTMCphon.f90:161-174), column-major (nb, nk).
+ * The grid trace reads only omega and max_omega.  The alias-mode trace (an extension)
+ * also needs vel, mfp and weight, and samples mode m with probability proportional to
+ * C_m*v_m*mfp_m*weight_m, C the mode heat capacity at `temperature` (RT:2696-2701). */
 typedef struct pt_modes_desc {
     int32_t nbranch;
     int32_t nkpoint;
     const double *omega;     /* rad/s, after the cm^-1 conversion of RT:727          */
-    const double *vel;       /* m/s                                                 */
-    const double *mfp;       /* m or nm, as the host uses it                        */
-    const double *weight;    /* multiplicity                                        */
+    const double *vel;       /* m/s (alias mode; may be NULL for the grid trace)    */
+    const double *mfp;       /* bulk mean free path in nm (alias mode; may be NULL) */
+    const double *weight;    /* multiplicity (alias mode; may be NULL)              */
     double max_omega;        /* anphon%maxOmega (RT:728)                            */
+    double temperature;      /* K, for the heat capacity of the alias weights       */
 } pt_modes_desc;
 
 /* Counters filled by the trace calls. */
@@ -192,8 +196,14 @@ int pt_surface_setup_polygon(int32_t nvertex, const double *vertices,
                              double center[3], double normal[3], double tangent[9]);
 
 /* ---- bulk phonon table ----------------------------------------------------------- */
+/* Uploads the table; with vel, mfp and weight present it also builds the alias table. */
 int pt_modes_create(pt_ctx *ctx, const pt_modes_desc *desc, pt_modes **out);
 int pt_modes_destroy(pt_modes *modes);
+
+/* Vose alias table of n >= 1 non-negative weights (host-only helper; pt_modes_create
+ * uses it): mode i is drawn as i0 = floor(u0*n), then i0 if u1 < prob[i0] else
+ * alias[i0].  No counterpart in the reference (it enumerates every mode). */
+int pt_alias_build(int64_t n, const double *weight, double *prob, int32_t *alias);
 
 /* ---- the trace ---------------------------------------------------------------------- */
 /*
@@ -220,6 +230,25 @@ int pt_trace_grid(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
 int pt_replay(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
               double mfp_nm, int32_t nangle, const double *vangle, int64_t np,
               int64_t first_path, int64_t n, pt_path_record *records);
+
+/*
+ * Alias-mode trace (an extension of the reference's estimator, whose (k, b, angle)
+ * enumeration is pt_trace_grid): npaths paths, path i with Philox stream i; each path
+ * draws its mode from the alias table, enters through x = 0 with Lambertian polar
+ * angle (cos(theta) = sqrt(u)) and random azimuth, scatters internally with mean free
+ * path mfp_m / branch_scale[b] (branch_scale NULL: 1), and is traced by the same event
+ * loop as calcTransmit (RT:1693-1824).  Adds per mode m = (ib-1) + nb*(ik-1) the paths
+ * drawn and transmitted to mode_paths and mode_trans (int64, caller-allocated nb*nk,
+ * accumulated).  kappa/kappa_bulk = (3L/4) * E[T/mfp_m] (host side).
+ */
+int pt_trace_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, int64_t npaths,
+                   int32_t precision, const double *branch_scale, int64_t *mode_paths,
+                   int64_t *mode_trans, pt_stats *stats);
+
+/* Per-path records of alias-mode paths first .. first+n-1 (parity checks). */
+int pt_replay_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
+                    const double *branch_scale, int64_t first, int64_t n,
+                    pt_path_record *records);
 
 #ifdef __cplusplus
 }

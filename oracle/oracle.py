@@ -43,6 +43,8 @@ def lib():
         L.oracle_replay.argtypes = [vp, vp, dbl, i32, vp, i64, u64, i32, i64, i64, vp]
         L.oracle_replay_log.argtypes = [vp, vp, dbl, i32, vp, i64, u64, i32, i64, i64, vp,
                                         vp, i64, vp, vp, i64, vp]
+        L.oracle_build_alias.argtypes = [i64, vp, vp, vp]
+        L.oracle_trace_alias.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp, vp, vp]
         L.oracle_philox_draw.restype = dbl
         L.oracle_philox_draw.argtypes = [u64, u64, u64]
         L.oracle_pt_log.restype = dbl
@@ -105,6 +107,33 @@ def replay_log(gdesc, mdesc, mfp, nangle, np_, seed, first, n, math=MATH_OWN, va
     if rc != 0:
         raise RuntimeError("oracle_replay_log failed (%d)" % rc)
     return out, traj[:toff[-1]], toff, sites[:soff[-1]], soff
+
+
+def build_alias(weights):
+    w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+    prob = np.zeros(w.size)
+    alias = np.zeros(w.size, dtype=np.int32)
+    if lib().oracle_build_alias(w.size, _p(w), _p(prob), _p(alias)) != 0:
+        raise RuntimeError("oracle_build_alias failed")
+    return prob, alias
+
+
+def trace_alias(gdesc, mdesc, mfp_nm, prob, alias, seed, first, n, scale=None, math=MATH_OWN,
+                records=False):
+    """oracle_trace_alias: per-mode (nb, nk) paths and transmitted counts (+ records)."""
+    nb, nk = mdesc.nbranch, mdesc.nkpoint
+    mfp = np.ascontiguousarray(np.asarray(mfp_nm, dtype=np.float64).reshape(-1, order="F"))
+    prob = np.ascontiguousarray(prob, dtype=np.float64)
+    alias = np.ascontiguousarray(alias, dtype=np.int32)
+    sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+    mp = np.zeros(nb * nk, dtype=np.int64)
+    mt = np.zeros(nb * nk, dtype=np.int64)
+    rec = np.zeros(n, dtype=RECORD_DTYPE) if records else None
+    lib().oracle_trace_alias(ctypes.addressof(gdesc), ctypes.addressof(mdesc), _p(mfp), _p(prob),
+                             _p(alias), _p(sc), int(seed), int(math), int(first), int(n),
+                             _p(mp), _p(mt), _p(rec))
+    out = (mp.reshape((nb, nk), order="F"), mt.reshape((nb, nk), order="F"))
+    return out + (rec,) if records else out
 
 
 def philox_draw(seed, path, draw):

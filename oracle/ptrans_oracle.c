@@ -415,7 +415,8 @@ static int o_incident_collision_grain(const pt_geom_desc *g, const double *p)
  * Returns 1 if the start-grain check failed (the reference then EXITs the particle
  * loop, RT:1681), else 0.
  */
-static int o_trace_particle(ostate *s, double cos_t, double sin_t, pt_path_record *rec)
+static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
+                            pt_path_record *rec)
 {
     const pt_geom_desc *g = s->g;
     double pos[3], dir[3], distance = 0.0, freepath, phi, sp, cp;
@@ -423,13 +424,18 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, pt_path_recor
     int64_t events = 0;
     int flags = 0;
 
-    s->draw = 0;
     /* createRandomPosition, RT:1845-1856 */
     {
         double y = o_rand(s, 1850), z = o_rand(s, 1851);
         pos[0] = 0.0;
         pos[1] = y * g->box[1] * g->scope[0];
         pos[2] = z * g->box[2] * g->scope[1];
+    }
+    if (lambert) {
+        /* alias-mode extension: Lambertian incidence, cos(theta) = sqrt(u) */
+        double u = o_rand(s, 9003);
+        cos_t = sqrt(u);
+        sin_t = sqrt(1.0 - u);
     }
     /* createRandomDirectionPhi, RT:1860-1871 */
     phi = o_rand(s, 1866);
@@ -665,7 +671,7 @@ int oracle_trace_grid(const pt_geom_desc *g, const pt_modes_desc *m, double mfp,
         for (p = 0; p < np; ++p) {
             pt_path_record r;
             s.path = (uint64_t)(c * np + p);
-            if (o_trace_particle(&s, ct, st, &r)) {
+            if ((s.draw = 0, o_trace_particle(&s, ct, st, 0, &r))) {
                 if (stats) { stats[4]++; stats[5] += np - p - 1; stats[2] += r.draws; stats[0]++; }
                 break;   /* EXIT of the particle loop, RT:1681 */
             }
@@ -703,7 +709,8 @@ int oracle_replay(const pt_geom_desc *g, const pt_modes_desc *m, double mfp, int
         if (math == ORACLE_MATH_OWN) pt_sincos(va[ia - 1], &st, &ct);
         else { ct = cos(va[ia - 1]); st = sin(va[ia - 1]); }
         memset(out + i, 0, sizeof *out);
-        o_trace_particle(&s, ct, st, out + i);
+        s.draw = 0;
+        o_trace_particle(&s, ct, st, 0, out + i);
     }
     free(va);
     return 0;
@@ -740,12 +747,97 @@ int oracle_replay_log(const pt_geom_desc *g, const pt_modes_desc *m, double mfp,
         if (math == ORACLE_MATH_OWN) pt_sincos(va[ia - 1], &st, &ct);
         else { ct = cos(va[ia - 1]); st = sin(va[ia - 1]); }
         memset(out + i, 0, sizeof *out);
-        o_trace_particle(&s, ct, st, out + i);
+        s.draw = 0;
+        o_trace_particle(&s, ct, st, 0, out + i);
         traj_off[i + 1] = s.tn;
         site_off[i + 1] = s.sn;
     }
     free(va);
     return (s.tn > traj_cap || s.sn > site_cap) ? -2 : 0;
+}
+
+/* ------------------------------------------------------------------ alias mode
+ * (an extension of the reference: mode sampling by a Vose alias table over
+ * (branch, kpoint) with weight C*v*Lambda*w, Lambertian incidence, per-mode free
+ * flights with mean free path Lambda_m / scale_b).  Not in P-TRANS; pinned by the
+ * implied-probability identity of the table and by the grid-mode equivalence tests. */
+
+/* Vose's alias method: prob[i] in [0,1], alias[i]; P(i) = (prob[i] + sum over j with
+ * alias[j] = i of (1 - prob[j])) / n.  Stacks are processed in index order. */
+int oracle_build_alias(int64_t n, const double *w, double *prob, int32_t *alias)
+{
+    double tot = 0.0, *q;
+    int64_t *sm, *lg, ns = 0, nl = 0, i;
+    if (n <= 0) return -1;
+    for (i = 0; i < n; ++i) {
+        if (!(w[i] >= 0.0)) return -1;
+        tot += w[i];
+    }
+    if (!(tot > 0.0)) return -1;
+    q = (double *)malloc(sizeof(double) * n);
+    sm = (int64_t *)malloc(sizeof(int64_t) * n);
+    lg = (int64_t *)malloc(sizeof(int64_t) * n);
+    for (i = 0; i < n; ++i) {
+        q[i] = w[i] * (double)n / tot;
+        alias[i] = (int32_t)i;
+        if (q[i] < 1.0) sm[ns++] = i; else lg[nl++] = i;
+    }
+    while (ns > 0 && nl > 0) {
+        int64_t a = sm[--ns], b = lg[--nl];
+        prob[a] = q[a];
+        alias[a] = (int32_t)b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if (q[b] < 1.0) sm[ns++] = b; else lg[nl++] = b;
+    }
+    while (nl > 0) prob[lg[--nl]] = 1.0;
+    while (ns > 0) prob[sm[--ns]] = 1.0;      /* rounding leftovers */
+    free(q); free(sm); free(lg);
+    return 0;
+}
+
+static int64_t o_sample_mode(ostate *s, int64_t n, const double *prob, const int32_t *alias)
+{
+    double u0 = o_rand(s, 9001), u1 = o_rand(s, 9002);
+    int64_t i = (int64_t)(u0 * (double)n);
+    if (i >= n) i = n - 1;
+    return u1 < prob[i] ? i : (int64_t)alias[i];
+}
+
+/*
+ * Alias-mode trace of paths [first, first+n): per mode m = (ib-1) + nb*(ik-1), the
+ * number of paths sampled (mode_paths) and transmitted (mode_trans), both added to.
+ * mfp_nm: per-mode mean free paths (nb*nk); scale: per-branch rate scale (nb) or NULL.
+ * records: per-path records or NULL.
+ */
+int oracle_trace_alias(const pt_geom_desc *g, const pt_modes_desc *m, const double *mfp_nm,
+                       const double *prob, const int32_t *alias, const double *scale,
+                       uint64_t seed, int math, int64_t first, int64_t n,
+                       int64_t *mode_paths, int64_t *mode_trans, pt_path_record *records)
+{
+    ostate s;
+    int64_t i, nm = (int64_t)m->nbranch * m->nkpoint;
+    memset(&s, 0, sizeof s);
+    s.g = g; s.m = m; s.math = math; s.seed = seed;
+    for (i = 0; i < n; ++i) {
+        pt_path_record r;
+        int64_t mode;
+        int ib;
+        s.path = (uint64_t)(first + i);
+        s.draw = 0;
+        mode = o_sample_mode(&s, nm, prob, alias);
+        ib = (int)(mode % m->nbranch);
+        s.cbranch = ib + 1;
+        s.ckpoint = (int)(mode / m->nbranch) + 1;
+        s.mfp = scale ? mfp_nm[mode] / scale[ib] : mfp_nm[mode];
+        memset(&r, 0, sizeof r);
+        o_trace_particle(&s, 0.0, 0.0, 1, &r);
+        if (!(r.flags & PT_PATH_START_MISMATCH)) {
+            if (mode_paths) mode_paths[mode]++;
+            if (mode_trans && r.end_side != 1) mode_trans[mode]++;
+        }
+        if (records) records[i] = r;
+    }
+    return 0;
 }
 
 /* Exposed for unit tests of the samplers' draw -> value maps and of the math. */

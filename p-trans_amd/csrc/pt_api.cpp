@@ -68,6 +68,8 @@ struct pt_ctx {
     uint8_t *outcome = nullptr;
     size_t outcome_bytes = 0;
     unsigned long long *counters = nullptr;   // queue + 6 stats + skipped
+    double *scale = nullptr;                   // per-branch scale of the current call
+    int scale_n = 0;
 };
 
 struct pt_geom {
@@ -80,7 +82,8 @@ struct pt_modes {
     pt_ctx *ctx = nullptr;
     DevModes dm{};
     int32_t nb = 0, nk = 0;
-    double *d_omega = nullptr;
+    bool has_alias = false;
+    std::vector<void *> owned;
 };
 
 extern "C" {
@@ -123,7 +126,7 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = trace_grid_occupancy(c->threads, &c->blocks_per_cu);
+    if (e == hipSuccess) e = trace_occupancy(c->threads, &c->blocks_per_cu);
     if (e != hipSuccess) {
         pt_ctx_destroy(c);
         return fail(PT_E_HIP, "pt_ctx_create: %s", hipGetErrorString(e));
@@ -140,6 +143,7 @@ int pt_ctx_destroy(pt_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->outcome) (void)hipFree(c->outcome);
     if (c->counters) (void)hipFree(c->counters);
+    if (c->scale) (void)hipFree(c->scale);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -342,6 +346,41 @@ int pt_geom_destroy(pt_geom *G)
     return PT_OK;
 }
 
+int pt_alias_build(int64_t n, const double *w, double *prob, int32_t *alias)
+{
+    // Vose's alias method; small and large work lists are processed from their ends.
+    if (n < 1 || n > INT32_MAX || !w || !prob || !alias)
+        return fail(PT_E_INVALID, "pt_alias_build: bad arguments");
+    double tot = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(w[i] >= 0.0) || !std::isfinite(w[i]))
+            return fail(PT_E_INVALID, "pt_alias_build: weight %lld is negative or not finite", (long long)i);
+        tot += w[i];
+    }
+    if (!(tot > 0.0)) return fail(PT_E_INVALID, "pt_alias_build: all weights are zero");
+    std::vector<double> q(n);
+    std::vector<int64_t> sm, lg;
+    sm.reserve(n);
+    lg.reserve(n);
+    for (int64_t i = 0; i < n; ++i) {
+        q[i] = w[i] * (double)n / tot;
+        alias[i] = (int32_t)i;
+        if (q[i] < 1.0) sm.push_back(i); else lg.push_back(i);
+    }
+    while (!sm.empty() && !lg.empty()) {
+        const int64_t a = sm.back(), b = lg.back();
+        sm.pop_back();
+        lg.pop_back();
+        prob[a] = q[a];
+        alias[a] = (int32_t)b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if (q[b] < 1.0) sm.push_back(b); else lg.push_back(b);
+    }
+    while (!lg.empty()) { prob[lg.back()] = 1.0; lg.pop_back(); }
+    while (!sm.empty()) { prob[sm.back()] = 1.0; sm.pop_back(); }
+    return PT_OK;
+}
+
 int pt_modes_create(pt_ctx *c, const pt_modes_desc *d, pt_modes **out)
 {
     if (!c || !d || !out) return fail(PT_E_INVALID, "pt_modes_create: NULL argument");
@@ -349,15 +388,44 @@ int pt_modes_create(pt_ctx *c, const pt_modes_desc *d, pt_modes **out)
     if (d->nbranch < 1 || d->nkpoint < 1) return fail(PT_E_INVALID, "pt_modes_create: empty table");
     if (!d->omega) return fail(PT_E_INVALID, "pt_modes_create: omega is NULL");
     const size_t n = (size_t)d->nbranch * d->nkpoint;
+    if (n > (size_t)INT32_MAX) return fail(PT_E_INVALID, "pt_modes_create: table too large");
     std::vector<double> om(d->omega, d->omega + n);
+    const bool alias = d->vel && d->mfp && d->weight;
+    std::vector<double> mfp, prob(n), wts(n);
+    std::vector<int32_t> al(n);
+    if (alias) {
+        if (!(d->temperature > 0)) return fail(PT_E_INVALID, "pt_modes_create: temperature must be positive");
+        mfp.assign(d->mfp, d->mfp + n);
+        // heat capacity, src/RTMCphon.f90:2696-2701
+        const double hbar = 1.054571e-34, kb = 1.3806503e-23;
+        for (size_t i = 0; i < n; ++i) {
+            const double x = hbar * om[i] / (kb * d->temperature);
+            const double cap = x <= 1.0e-10 ? kb : kb * std::pow((x / 2) / std::sinh(x / 2), 2.0);
+            wts[i] = cap * d->vel[i] * d->mfp[i] * d->weight[i];
+            if (!(mfp[i] > 0)) return fail(PT_E_INVALID, "pt_modes_create: mfp[%zu] must be positive", i);
+        }
+        int rc = pt_alias_build((int64_t)n, wts.data(), prob.data(), al.data());
+        if (rc) return rc;
+    }
     PT_HIP(hipSetDevice(c->device));
     pt_modes *M = new pt_modes();
     M->ctx = c;
     M->nb = d->nbranch;
     M->nk = d->nkpoint;
-    int rc = upload(om, &M->d_omega, c->stream);
-    if (rc) { delete M; return rc; }
-    M->dm.omega = M->d_omega;
+    M->has_alias = alias;
+    double *d_om = nullptr, *d_mfp = nullptr, *d_prob = nullptr;
+    int32_t *d_al = nullptr;
+    int rc = upload(om, &d_om, c->stream);
+    if (!rc && alias) rc = upload(mfp, &d_mfp, c->stream);
+    if (!rc && alias) rc = upload(prob, &d_prob, c->stream);
+    if (!rc && alias) rc = upload(al, &d_al, c->stream);
+    M->owned = {d_om, d_mfp, d_prob, d_al};
+    if (rc) { pt_modes_destroy(M); return rc; }
+    M->dm.omega = d_om;
+    M->dm.mfp = d_mfp;
+    M->dm.prob = d_prob;
+    M->dm.alias = d_al;
+    M->dm.scale = nullptr;
     M->dm.nb = d->nbranch;
     M->dm.nk = d->nkpoint;
     M->dm.max_omega = d->max_omega;
@@ -375,7 +443,8 @@ int pt_modes_destroy(pt_modes *M)
     if (!M) return PT_OK;
     (void)hipSetDevice(M->ctx->device);
     (void)hipStreamSynchronize(M->ctx->stream);
-    if (M->d_omega) (void)hipFree(M->d_omega);
+    for (void *p : M->owned)
+        if (p) (void)hipFree(p);
     delete M;
     return PT_OK;
 }
@@ -406,6 +475,73 @@ int check_trace_args(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp,
     if (nangle < 2) return fail(PT_E_INVALID, "nangle must be >= 2");
     if (np < 0) return fail(PT_E_INVALID, "np must be >= 0");
     if (!(mfp > 0)) return fail(PT_E_INVALID, "mean free path must be positive");
+    return PT_OK;
+}
+
+}  // namespace
+
+namespace {
+
+TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
+{
+    TraceParams P{};
+    P.g = G->dg;
+    P.m = M->dm;
+    P.seed = c->seed;
+    P.chunk = c->chunk;
+    P.event_limit = c->event_limit;
+    P.queue = c->counters;
+    P.stats = c->counters + 1;
+    return P;
+}
+
+int set_scale(pt_ctx *c, const pt_modes *M, const double *scale, TraceParams &P)
+{
+    P.m.scale = nullptr;
+    if (!scale) return PT_OK;
+    for (int b = 0; b < M->nb; ++b)
+        if (!(scale[b] > 0) || !std::isfinite(scale[b]))
+            return fail(PT_E_INVALID, "branch_scale[%d] must be positive and finite", b);
+    if (c->scale_n < M->nb) {
+        if (c->scale) PT_HIP(hipFree(c->scale));
+        c->scale = nullptr;
+        c->scale_n = 0;
+        PT_HIP(hipMalloc((void **)&c->scale, M->nb * sizeof(double)));
+        c->scale_n = M->nb;
+    }
+    PT_HIP(hipMemcpyAsync(c->scale, scale, M->nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    P.m.scale = c->scale;
+    return PT_OK;
+}
+
+void fill_stats(pt_stats *stats, const unsigned long long *h, float ms, float ms_red)
+{
+    if (!stats) return;
+    stats->paths = (int64_t)h[1];
+    stats->events = (int64_t)h[2];
+    stats->draws = (int64_t)h[3];
+    stats->lost = (int64_t)h[4];
+    stats->start_mismatch = (int64_t)h[5];
+    stats->skipped = (int64_t)h[7];
+    stats->kernel_ms = ms;
+    stats->reduce_ms = ms_red;
+    stats->fp64_ops = (int64_t)h[9];
+    stats->philox_blocks = (int64_t)h[10];
+}
+
+int replay_common(pt_ctx *c, TraceParams &P, int mode, int64_t first, int64_t n,
+                  pt_path_record *records)
+{
+    pt_path_record *d_rec = nullptr;
+    PT_HIP(hipMalloc((void **)&d_rec, n * sizeof(pt_path_record)));
+    hipError_t e;
+    do {
+        if ((e = launch_replay(P, mode, first, n, d_rec, c->stream))) break;
+        if ((e = hipMemcpyAsync(records, d_rec, n * sizeof(pt_path_record), hipMemcpyDeviceToHost, c->stream))) break;
+        e = hipStreamSynchronize(c->stream);
+    } while (0);
+    (void)hipFree(d_rec);
+    if (e != hipSuccess) return fail(PT_E_HIP, "replay: %s", hipGetErrorString(e));
     return PT_OK;
 }
 
@@ -446,20 +582,13 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "hipMalloc: %s", hipGetErrorString(e)); }
     std::vector<double> ang(ct);
     ang.insert(ang.end(), st.begin(), st.end());
-    GridParams P{};
-    P.g = G->dg;
-    P.m = M->dm;
+    TraceParams P = base_params(c, G, M);
     P.cos_t = d_ang;
     P.sin_t = d_ang + nangle;
-    P.seed = c->seed;
     P.mfp = mfp;
     P.np = np;
     P.total = total;
     P.nangle = nangle;
-    P.chunk = c->chunk;
-    P.event_limit = c->event_limit;
-    P.queue = c->counters;
-    P.stats = c->counters + 1;
     P.outcome = c->outcome;
     unsigned long long host_ctr[16] = {0};
     float ms = 0.f, ms_red = 0.f;
@@ -469,7 +598,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace_grid(P, blocks, c->threads, c->count_ops, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_GRID, blocks, c->threads, c->count_ops, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
         if ((e = hipEventRecord(c->ev2, c->stream))) break;
@@ -482,18 +611,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     (void)hipFree(d_ang);
     (void)hipFree(d_counts);
     if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_grid: %s", hipGetErrorString(e));
-    if (stats) {
-        stats->paths = (int64_t)host_ctr[1];
-        stats->events = (int64_t)host_ctr[2];
-        stats->draws = (int64_t)host_ctr[3];
-        stats->lost = (int64_t)host_ctr[4];
-        stats->start_mismatch = (int64_t)host_ctr[5];
-        stats->skipped = (int64_t)host_ctr[7];
-        stats->kernel_ms = ms;
-        stats->reduce_ms = ms_red;
-        stats->fp64_ops = (int64_t)host_ctr[9];
-        stats->philox_blocks = (int64_t)host_ctr[10];
-    }
+    fill_stats(stats, host_ctr, ms, ms_red);
     if ((int64_t)host_ctr[1] != total)
         return fail(PT_E_HIP, "pt_trace_grid: %lld of %lld paths completed",
                     (long long)host_ctr[1], (long long)total);
@@ -518,33 +636,84 @@ int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_
     std::vector<double> ang(ct);
     ang.insert(ang.end(), st.begin(), st.end());
     double *d_ang = nullptr;
-    pt_path_record *d_rec = nullptr;
     PT_HIP(hipMalloc((void **)&d_ang, 2 * nangle * sizeof(double)));
-    hipError_t e = hipMalloc((void **)&d_rec, n * sizeof(pt_path_record));
-    if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "hipMalloc: %s", hipGetErrorString(e)); }
-    ReplayParams P{};
-    P.g = G->dg;
-    P.m = M->dm;
+    hipError_t e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "pt_replay: %s", hipGetErrorString(e)); }
+    TraceParams P = base_params(c, G, M);
     P.cos_t = d_ang;
     P.sin_t = d_ang + nangle;
-    P.seed = c->seed;
     P.mfp = mfp;
     P.np = np;
     P.nangle = nangle;
-    P.first = first_path;
-    P.n = n;
-    P.event_limit = c->event_limit;
-    P.rec = d_rec;
-    do {
-        if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
-        if ((e = launch_replay(P, c->stream))) break;
-        if ((e = hipMemcpyAsync(records, d_rec, n * sizeof(pt_path_record), hipMemcpyDeviceToHost, c->stream))) break;
-        e = hipStreamSynchronize(c->stream);
-    } while (0);
+    rc = replay_common(c, P, TRACE_GRID, first_path, n, records);
     (void)hipFree(d_ang);
-    (void)hipFree(d_rec);
-    if (e != hipSuccess) return fail(PT_E_HIP, "pt_replay: %s", hipGetErrorString(e));
+    return rc;
+}
+
+int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npaths,
+                   int32_t precision, const double *branch_scale, int64_t *mode_paths,
+                   int64_t *mode_trans, pt_stats *stats)
+{
+    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
+    if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
+    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
+    if (precision != PT_PREC_FP64) return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
+    if (!mode_paths || !mode_trans) return fail(PT_E_INVALID, "tally arrays are NULL");
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (npaths == 0) return PT_OK;
+    const size_t nm = (size_t)M->nb * M->nk;
+    PT_HIP(hipSetDevice(c->device));
+    TraceParams P = base_params(c, G, M);
+    int rc = set_scale(c, M, branch_scale, P);
+    if (rc) return rc;
+    P.total = npaths;
+    unsigned long long *d_tal = nullptr;
+    PT_HIP(hipMalloc((void **)&d_tal, 2 * nm * sizeof(unsigned long long)));
+    P.mode_paths = d_tal;
+    P.mode_trans = d_tal + nm;
+    std::vector<unsigned long long> h_tal(2 * nm);
+    unsigned long long host_ctr[16] = {0};
+    float ms = 0.f;
+    const int blocks = c->ncu * c->blocks_per_cu;
+    hipError_t e;
+    do {
+        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipStreamSynchronize(c->stream))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    (void)hipFree(d_tal);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_alias: %s", hipGetErrorString(e));
+    for (size_t i = 0; i < nm; ++i) {
+        mode_paths[i] += (int64_t)h_tal[i];
+        mode_trans[i] += (int64_t)h_tal[nm + i];
+    }
+    fill_stats(stats, host_ctr, ms, 0.f);
+    if ((int64_t)host_ctr[1] != npaths)
+        return fail(PT_E_HIP, "pt_trace_alias: %lld of %lld paths completed",
+                    (long long)host_ctr[1], (long long)npaths);
     return PT_OK;
+}
+
+int pt_replay_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, const double *branch_scale,
+                    int64_t first, int64_t n, pt_path_record *records)
+{
+    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
+    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
+    if (n < 0 || first < 0) return fail(PT_E_INVALID, "bad path range");
+    if (n == 0) return PT_OK;
+    if (!records) return fail(PT_E_INVALID, "records is NULL");
+    PT_HIP(hipSetDevice(c->device));
+    TraceParams P = base_params(c, G, M);
+    int rc = set_scale(c, M, branch_scale, P);
+    if (rc) return rc;
+    return replay_common(c, P, TRACE_ALIAS, first, n, records);
 }
 
 }  // extern "C"

@@ -50,6 +50,10 @@ struct DevGeom {
 
 struct DevModes {
     const double *omega;       // (nb, nk) column-major
+    const double *mfp;         // (nb, nk) mean free paths [nm] (alias mode)
+    const double *prob;        // Vose table (alias mode)
+    const int32_t *alias;
+    const double *scale;       // per-branch scattering-rate scale or nullptr
     int32_t nb, nk;
     double max_omega;
 };
@@ -60,46 +64,36 @@ enum : uint8_t {
     OUT_MISMATCH = 2,          // start-grain check failed (RT:1673-1682)
 };
 
-struct GridParams {
+enum { TRACE_GRID = 0, TRACE_ALIAS = 1 };
+
+struct TraceParams {
     DevGeom g;
     DevModes m;
-    const double *cos_t;       // per angle index
+    const double *cos_t;       // grid: per angle index
     const double *sin_t;
     uint64_t seed;
-    double mfp;
-    int64_t np;                // particles per cell
-    int64_t total;             // paths = ncell * np
-    int32_t nangle;
+    double mfp;                // grid: the sampling mean free path (1e10 ballistic)
+    int64_t np;                // grid: particles per cell
+    int64_t total;             // paths of the launch
+    int32_t nangle;            // grid
     int32_t chunk;             // paths a wave takes from the queue at once
     int64_t event_limit;
     unsigned long long *queue; // next path id
     unsigned long long *stats; // paths, events, draws, lost, mismatch, limit, -, -,
                                // fp64 ops, philox blocks (counting kernel only)
-    uint8_t *outcome;          // [total]
-};
-
-struct ReplayParams {
-    DevGeom g;
-    DevModes m;
-    const double *cos_t;
-    const double *sin_t;
-    uint64_t seed;
-    double mfp;
-    int64_t np;
-    int32_t nangle;
-    int64_t first;
-    int64_t n;
-    int64_t event_limit;
-    pt_path_record *rec;
+    uint8_t *outcome;          // grid: [total]
+    unsigned long long *mode_paths;   // alias: [nb*nk]
+    unsigned long long *mode_trans;   // alias: [nb*nk]
 };
 
 // kernels' launch wrappers (pt_trace.hip)
-hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, bool count,
-                             hipStream_t s);
+hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
+                        hipStream_t s);
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
                               int64_t nb, int64_t nk, int32_t nangle,
                               long long *counts, unsigned long long *skipped, hipStream_t s);
-hipError_t launch_replay(const ReplayParams &p, hipStream_t s);
-hipError_t trace_grid_occupancy(int threads, int *blocks_per_cu);
+hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n,
+                         pt_path_record *rec, hipStream_t s);
+hipError_t trace_occupancy(int threads, int *blocks_per_cu);
 
 }  // namespace pt

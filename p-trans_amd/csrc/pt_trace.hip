@@ -301,7 +301,8 @@ struct Path {
     double pos[3];
     double dir[3];
     double fp;              // FreePath
-    double omega;           // mode frequency of the cell (Gamma/TBC laws)
+    double mfp;             // mean free path of the path's mode (1e10: ballistic)
+    double omega;           // mode frequency (Gamma/TBC laws)
     int which;              // whichV
     int cur;                // last surface hit (global index), -1 after a scattering
     int end_side;
@@ -340,7 +341,9 @@ __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, d
 
 // One iteration of the particle loop up to the event loop (RT:1632-1690).
 // Returns false if the start-grain check fails (the reference EXITs the particle loop).
-__device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double sin_t, Rng &rng,
+// lambert: the alias-mode extension draws the polar angle, cos(theta) = sqrt(u),
+// after the position instead of taking the cell's fixed angle.
+__device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool lambert, Rng &rng,
                            Path &ph)
 {
     OpCount &oc = *rng.oc;
@@ -349,6 +352,12 @@ __device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double si
     ph.pos[0] = 0.0;
     ph.pos[1] = y * g.box[1] * g.scope[0];
     ph.pos[2] = z * g.box[2] * g.scope[1];
+    if (lambert) {
+        const double u = rng.next();
+        cos_t = sqrt(u);
+        sin_t = sqrt(1.0 - u);
+        oc.flops += 3;
+    }
     double phi = rng.next();                                                // RT:1866
     phi = phi * kTwoPi;
     double sp, cp;
@@ -369,14 +378,14 @@ __device__ bool start_path(const DevGeom &g, double mfp, double cos_t, double si
         ph.flags = PT_PATH_START_MISMATCH;
         return false;
     }
-    if (mfp == kBallistic) ph.fp = kBallistic;                              // RT:1686-1690
-    else { ph.fp = -(mfp * pt_log(rng.next())); oc.flops += 1 + kFlopsLog; } // RT:2171-2172
+    if (ph.mfp == kBallistic) ph.fp = kBallistic;                           // RT:1686-1690
+    else { ph.fp = -(ph.mfp * pt_log(rng.next())); oc.flops += 1 + kFlopsLog; } // RT:2171-2172
     return true;
 }
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
-__device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_t event_limit,
-                          Rng &rng, Path &ph)
+__device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
+                          Path &ph)
 {
     OpCount &oc = *rng.oc;
     oc.flops += 3;                       // box + SMALL_VAL
@@ -493,13 +502,13 @@ __device__ bool step_path(const DevGeom &g, double mfp, double max_omega, int64_
     for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + PT_NUDGE * ph.dir[k];
     oc.flops += 6;
     // FreePath, RT:1783-1790
-    if (mfp == kBallistic) {
+    if (ph.mfp == kBallistic) {
         ph.fp = kBallistic;
     } else {
         ph.fp = ph.fp - distance;
         oc.flops += 1;
         if (ph.fp < 1.0e-10) {
-            ph.fp = -(mfp * pt_log(rng.next()));                             // RT:2171
+            ph.fp = -(ph.mfp * pt_log(rng.next()));                          // RT:2171
             oc.flops += 1 + kFlopsLog;
         }
     }
@@ -537,10 +546,48 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// Alias-mode extension: the mode of a path from two draws (Vose table).
+__device__ inline int sample_mode(const DevModes &m, Rng &rng)
+{
+    const double u0 = rng.next(), u1 = rng.next();
+    const int n = m.nb * m.nk;
+    int i = (int)(u0 * (double)n);
+    if (i >= n) i = n - 1;
+    return u1 < m.prob[i] ? i : m.alias[i];
+}
+
+// Sets up path pid: grid mode (cell of the reference's collapsed loop) or alias mode
+// (sampled mode, Lambertian incidence).  Returns start_path's verdict.
+template <int kMode>
+__device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, Path &ph, int *mode)
+{
+    rng.path = (uint64_t)pid;
+    rng.draw = 0;
+    if (kMode == TRACE_GRID) {
+        int ib, ik, ia;
+        cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
+        ph.omega = cell_omega(P.m, ib, ik);
+        ph.mfp = P.mfp;
+        *mode = (ib - 1) + P.m.nb * (ik - 1);
+        return start_path(P.g, P.cos_t[ia - 1], P.sin_t[ia - 1], false, rng, ph);
+    } else {
+        const int m = sample_mode(P.m, rng);
+        const int ib = m % P.m.nb;
+        ph.omega = P.m.omega[m];
+        ph.mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[ib] : P.m.mfp[m];
+        *mode = m;
+        return start_path(P.g, 0.0, 0.0, true, rng, ph);
+    }
+}
+
 }  // namespace
 
-template <bool kCount>
-__global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
+// Persistent trace kernel.  Each wave takes chunks of path ids from one global queue and
+// refills lanes whose path has ended (ballot + prefix count); lanes then advance one event
+// per iteration.  Grid mode writes one outcome byte per path (reduced per cell later);
+// alias mode adds to per-mode integer tallies (exact, order independent).
+template <int kMode, bool kCount>
+__global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
 {
     const int lane = threadIdx.x & 63;
     OpCount oc = {0ull, 0ull};
@@ -550,9 +597,9 @@ __global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
     Path ph;
     bool active = false;
     int64_t pid = 0;
+    int mode = 0;
     unsigned long long s_paths = 0, s_events = 0, s_draws = 0, s_lost = 0, s_mis = 0, s_lim = 0;
-    // wave-private slice of the global path queue
-    int64_t chunk_next = 0, chunk_end = 0;
+    int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
 
     while (true) {
@@ -572,15 +619,10 @@ __global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
                 const int nneed = __popcll(need);
                 if (!active && rank < avail) {
                     pid = chunk_next + rank;
-                    int ib, ik, ia;
-                    cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
-                    ph.omega = cell_omega(P.m, ib, ik);
-                    rng.path = (uint64_t)pid;
-                    rng.draw = 0;
-                    if (start_path(P.g, P.mfp, P.cos_t[ia - 1], P.sin_t[ia - 1], rng, ph)) {
+                    if (begin_path<kMode>(P, pid, rng, ph, &mode)) {
                         active = true;
                     } else {
-                        P.outcome[pid] = OUT_MISMATCH;
+                        if (kMode == TRACE_GRID) P.outcome[pid] = OUT_MISMATCH;
                         s_paths++;
                         s_mis++;
                         s_draws += rng.draw;
@@ -594,9 +636,14 @@ __global__ void __launch_bounds__(256) trace_grid_kernel(GridParams P)
             continue;
         }
         if (active) {
-            if (ph.end_side != 0 ||
-                step_path(P.g, P.mfp, P.m.max_omega, P.event_limit, rng, ph)) {
-                P.outcome[pid] = (ph.end_side != 1) ? OUT_TRANSMITTED : 0;
+            if (ph.end_side != 0 || step_path(P.g, P.m.max_omega, P.event_limit, rng, ph)) {
+                const bool transmitted = ph.end_side != 1;          // RT:1833
+                if (kMode == TRACE_GRID) {
+                    P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
+                } else {
+                    atomicAdd(P.mode_paths + mode, 1ull);
+                    if (transmitted) atomicAdd(P.mode_trans + mode, 1ull);
+                }
                 s_paths++;
                 s_events += (unsigned long long)ph.events;
                 s_draws += rng.draw;
@@ -671,24 +718,22 @@ __global__ void __launch_bounds__(256) reduce_grid_kernel(const uint8_t *__restr
     }
 }
 
-__global__ void __launch_bounds__(64) replay_kernel(ReplayParams P)
+// Single paths with full records (parity checks), one lane per path.
+template <int kMode>
+__global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first, int64_t n,
+                                                    pt_path_record *rec)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const int64_t pid = P.first + i;
-    int ib, ik, ia;
-    cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
+    if (i >= n) return;
     OpCount oc = {0ull, 0ull};
     Rng rng;
     rng.oc = &oc;
     rng.seed = P.seed;
-    rng.path = (uint64_t)pid;
-    rng.draw = 0;
     Path ph;
-    ph.omega = cell_omega(P.m, ib, ik);
-    if (start_path(P.g, P.mfp, P.cos_t[ia - 1], P.sin_t[ia - 1], rng, ph)) {
+    int mode = 0;
+    if (begin_path<kMode>(P, first + i, rng, ph, &mode)) {
         while (ph.end_side == 0) {
-            if (step_path(P.g, P.mfp, P.m.max_omega, P.event_limit, rng, ph)) break;
+            if (step_path(P.g, P.m.max_omega, P.event_limit, rng, ph)) break;
         }
     }
     pt_path_record r;
@@ -699,23 +744,26 @@ __global__ void __launch_bounds__(64) replay_kernel(ReplayParams P)
     r.which_grain = ph.which;
     r.start_grain = ph.start_grain;
     for (int k = 0; k < 3; ++k) { r.pos[k] = ph.pos[k]; r.dir[k] = ph.dir[k]; }
-    P.rec[i] = r;
+    rec[i] = r;
 }
 
-hipError_t launch_trace_grid(const GridParams &p, int blocks, int threads, bool count,
-                             hipStream_t s)
+hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
+                        hipStream_t s)
 {
-    if (count)
-        hipLaunchKernelGGL(trace_grid_kernel<true>, dim3(blocks), dim3(threads), 0, s, p);
-    else
-        hipLaunchKernelGGL(trace_grid_kernel<false>, dim3(blocks), dim3(threads), 0, s, p);
+    if (mode == TRACE_GRID) {
+        if (count) hipLaunchKernelGGL((trace_kernel<TRACE_GRID, true>), dim3(blocks), dim3(threads), 0, s, p);
+        else hipLaunchKernelGGL((trace_kernel<TRACE_GRID, false>), dim3(blocks), dim3(threads), 0, s, p);
+    } else {
+        if (count) hipLaunchKernelGGL((trace_kernel<TRACE_ALIAS, true>), dim3(blocks), dim3(threads), 0, s, p);
+        else hipLaunchKernelGGL((trace_kernel<TRACE_ALIAS, false>), dim3(blocks), dim3(threads), 0, s, p);
+    }
     return hipGetLastError();
 }
 
-hipError_t trace_grid_occupancy(int threads, int *blocks_per_cu)
+hipError_t trace_occupancy(int threads, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_grid_kernel<false>,
-                                                        threads, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        trace_kernel<TRACE_GRID, false>, threads, 0);
 }
 
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,
@@ -728,11 +776,15 @@ hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
     return hipGetLastError();
 }
 
-hipError_t launch_replay(const ReplayParams &p, hipStream_t s)
+hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n,
+                         pt_path_record *rec, hipStream_t s)
 {
-    if (p.n == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)((p.n + 63) / 64);
-    hipLaunchKernelGGL(replay_kernel, dim3(blocks), dim3(64), 0, s, p);
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 63) / 64);
+    if (mode == TRACE_GRID)
+        hipLaunchKernelGGL(replay_kernel<TRACE_GRID>, dim3(blocks), dim3(64), 0, s, p, first, n, rec);
+    else
+        hipLaunchKernelGGL(replay_kernel<TRACE_ALIAS>, dim3(blocks), dim3(64), 0, s, p, first, n, rec);
     return hipGetLastError();
 }
 

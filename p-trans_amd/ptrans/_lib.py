@@ -20,7 +20,7 @@ EXPORTS = (
     "pt_ctx_set_op_counting",
     "pt_geom_create", "pt_geom_destroy", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
-    "pt_trace_grid", "pt_replay",
+    "pt_trace_grid", "pt_replay", "pt_alias_build", "pt_trace_alias", "pt_replay_alias",
 )
 
 
@@ -75,6 +75,9 @@ def load():
     L.pt_modes_destroy.argtypes = [vp]
     L.pt_trace_grid.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i32, vp, vp]
     L.pt_replay.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i64, i64, vp]
+    L.pt_alias_build.argtypes = [i64, vp, vp, vp]
+    L.pt_trace_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
+    L.pt_replay_alias.argtypes = [vp, vp, vp, vp, i64, i64, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

@@ -198,20 +198,29 @@ class PtModesDesc(ctypes.Structure):
         ("nbranch", ctypes.c_int32), ("nkpoint", ctypes.c_int32),
         ("omega", ctypes.POINTER(ctypes.c_double)), ("vel", ctypes.POINTER(ctypes.c_double)),
         ("mfp", ctypes.POINTER(ctypes.c_double)), ("weight", ctypes.POINTER(ctypes.c_double)),
-        ("max_omega", ctypes.c_double),
+        ("max_omega", ctypes.c_double), ("temperature", ctypes.c_double),
     ]
 
 
-def modes_desc(omega, vel, mfp, weight, max_omega=None):
-    """A pt_modes_desc over (nb, nk) arrays; returns (desc, keepalive)."""
-    arrs = [np.asfortranarray(np.asarray(a, dtype=np.float64)) for a in (omega, vel, mfp, weight)]
-    nb, nk = arrs[0].shape
-    flat = [np.ascontiguousarray(a.reshape(-1, order="F")) for a in arrs]
+def modes_desc(omega, vel=None, mfp=None, weight=None, max_omega=None, temperature=300.0):
+    """A pt_modes_desc over (nb, nk) arrays (vel, mfp [nm], weight optional: the alias
+    trace needs them); returns (desc, keepalive)."""
+    omega = np.asarray(omega, dtype=np.float64)
+    nb, nk = omega.shape
+    flat = []
+    ptrs = []
+    for a in (omega, vel, mfp, weight):
+        if a is None:
+            ptrs.append(None)
+            continue
+        f = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, order="F"))
+        flat.append(f)
+        ptrs.append(f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     d = PtModesDesc()
     d.nbranch, d.nkpoint = nb, nk
-    ptr = [f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for f in flat]
-    d.omega, d.vel, d.mfp, d.weight = ptr
-    d.max_omega = float(np.max(arrs[0])) if max_omega is None else float(max_omega)
+    d.omega, d.vel, d.mfp, d.weight = ptrs
+    d.max_omega = float(np.max(omega)) if max_omega is None else float(max_omega)
+    d.temperature = float(temperature)
     return d, flat
 
 

@@ -231,3 +231,41 @@ def write_outputs(res: KappaResult, directory: str, nphonon: int, semfp: np.ndar
         for i in range(nf):
             f.write(fortran_e(freqs[i], 20, 10) + "".join(fortran_e(spec[i, j], 20, 10)
                                                           for j in range(nb)) + "\n")
+
+
+# ------------------------------------------------------------------ alias-mode estimator
+
+def kappa_bulk(omega, vel, mfp_nm, weight, temperature: float, vol: float) -> float:
+    """Bulk conductivity sum_m C_m w_m v_m Lambda_m / (3 V N_q), the kappa of fileoutput
+    (RT:2747-2755) with EMFP = bulk MFP; N_q = int(sum(w)/nb) (RT:2657)."""
+    omega = np.asarray(omega, dtype=np.float64)
+    nb = omega.shape[0]
+    nq = int(np.asarray(weight, dtype=np.float64).sum() / nb)
+    cap = np.vectorize(lambda o: heat_capacity(float(o), temperature))(omega)
+    return float((cap * weight * np.asarray(mfp_nm) * 1.0e-9 * vel).sum() / 3 / vol / nq)
+
+
+@dataclasses.dataclass
+class AliasKappa:
+    ratio: float            # kappa / kappa_bulk = E[X], X = (3L/4) T / Lambda_m
+    stderr: float           # standard error of ratio
+    paths: int
+    var_x: float            # per-path variance of X (for the paths needed at a target error)
+
+    def paths_for_rel_error(self, rel: float) -> float:
+        return self.var_x / (rel * self.ratio) ** 2 if self.ratio > 0 else math.inf
+
+
+def alias_kappa(mode_paths, mode_trans, mfp_nm, length: float) -> AliasKappa:
+    """kappa/kappa_bulk from alias-mode tallies.  A path of mode m that transmits scores
+    X = (3L/4)/Lambda_m, else 0: with modes drawn in proportion to C v Lambda w and the
+    Lambertian polar angle, E[X] = sum_m q_m Lambda_eff,m / Lambda_m with
+    Lambda_eff = (3L/2) int T sin cos dtheta (RT:2418-2451 in the continuum limit)."""
+    n = int(np.sum(mode_paths))
+    x = 0.75 * length / np.asarray(mfp_nm, dtype=np.float64)
+    t = np.asarray(mode_trans, dtype=np.float64)
+    s1 = float((t * x).sum())
+    s2 = float((t * x * x).sum())
+    mean = s1 / n
+    var = max(s2 / n - mean * mean, 0.0)
+    return AliasKappa(ratio=mean, stderr=math.sqrt(var / n), paths=n, var_x=var)

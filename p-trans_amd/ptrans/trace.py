@@ -36,14 +36,13 @@ class Geom:
 
 
 class Modes:
-    def __init__(self, ctx, omega, vel=None, mfp=None, weight=None, max_omega=None):
+    def __init__(self, ctx, omega, vel=None, mfp=None, weight=None, max_omega=None,
+                 temperature=300.0):
         self.ctx = ctx
         omega = np.asarray(omega, dtype=np.float64)
-        vel = omega * 0 if vel is None else vel
-        mfp = omega * 0 if mfp is None else mfp
-        weight = np.ones_like(omega) if weight is None else weight
         self.nbranch, self.nkpoint = omega.shape
-        self._desc, self._arrays = modes_desc(omega, vel, mfp, weight, max_omega)
+        self.mfp = None if mfp is None else np.asarray(mfp, dtype=np.float64)
+        self._desc, self._arrays = modes_desc(omega, vel, mfp, weight, max_omega, temperature)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().pt_modes_create(ctx.handle, ctypes.addressof(self._desc), ctypes.byref(h)))
         self.handle = h
@@ -92,11 +91,14 @@ class Context:
         self._owned.append(g)
         return g
 
-    def modes(self, table=None, omega=None, max_omega=None) -> Modes:
+    def modes(self, table=None, omega=None, max_omega=None, vel=None, mfp=None, weight=None,
+              temperature=300.0) -> Modes:
+        """Upload a bulk table (an Anphon, mfp column in nm) or bare arrays; vel, mfp and
+        weight enable the alias-mode trace."""
         if table is not None:
-            m = Modes(self, table.omega, table.vel, table.mfp, table.weight, max_omega)
+            m = Modes(self, table.omega, table.vel, table.mfp, table.weight, max_omega, temperature)
         else:
-            m = Modes(self, omega, max_omega=max_omega)
+            m = Modes(self, omega, vel, mfp, weight, max_omega, temperature)
         self._owned.append(m)
         return m
 
@@ -134,3 +136,43 @@ class Context:
             va.ctypes.data if va is not None else None, int(np_), int(first), int(n),
             out.ctypes.data))
         return out
+
+    def trace_alias(self, geom: Geom, modes: Modes, npaths: int, branch_scale=None,
+                    precision: int = PREC_FP64):
+        """pt_trace_alias: per-mode (nb, nk) int64 path and transmitted counts, stats."""
+        nb, nk = modes.nbranch, modes.nkpoint
+        paths = np.zeros(nb * nk, dtype=np.int64)
+        trans = np.zeros(nb * nk, dtype=np.int64)
+        stats = _lib.PtStats()
+        sc = None if branch_scale is None else np.ascontiguousarray(branch_scale, dtype=np.float64)
+        _lib.check(_lib.load().pt_trace_alias(
+            self.handle, geom.handle, modes.handle, int(npaths), int(precision),
+            sc.ctypes.data if sc is not None else None, paths.ctypes.data, trans.ctypes.data,
+            ctypes.byref(stats)))
+        return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"),
+                stats.as_dict())
+
+    def replay_alias(self, geom: Geom, modes: Modes, first: int, n: int, branch_scale=None):
+        out = np.zeros(n, dtype=_lib.RECORD_DTYPE)
+        sc = None if branch_scale is None else np.ascontiguousarray(branch_scale, dtype=np.float64)
+        _lib.check(_lib.load().pt_replay_alias(
+            self.handle, geom.handle, modes.handle, sc.ctypes.data if sc is not None else None,
+            int(first), int(n), out.ctypes.data))
+        return out
+
+
+def alias_table(weights):
+    """pt_alias_build on a flat weight vector: (prob, alias)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+    prob = np.zeros(w.size)
+    alias = np.zeros(w.size, dtype=np.int32)
+    _lib.check(_lib.load().pt_alias_build(w.size, w.ctypes.data, prob.ctypes.data, alias.ctypes.data))
+    return prob, alias
+
+
+def alias_probabilities(prob, alias):
+    """The distribution a Vose table samples: (prob[i] + sum_{alias[j]=i} (1-prob[j])) / n."""
+    n = prob.size
+    p = prob.astype(np.float64).copy()
+    np.add.at(p, alias, 1.0 - prob)          # an entry with prob = 1 gives nothing away
+    return p / n

@@ -33,7 +33,7 @@ def test_struct_layouts_match_header():
     # sizes printed by sizeof() on include/ptrans.h (gcc, x86-64)
     assert SURFACE_DTYPE.itemsize == 184
     assert ctypes.sizeof(PtGeomDesc) == 112
-    assert ctypes.sizeof(PtModesDesc) == 48
+    assert ctypes.sizeof(PtModesDesc) == 56
     assert _lib.RECORD_DTYPE.itemsize == 80
     assert ctypes.sizeof(_lib.PtStats) == 80
 
