@@ -30,8 +30,9 @@ for p in (os.path.join(ROOT, "p-trans_amd"), ROOT):
         sys.path.insert(0, p)
 
 from ptrans import gen, geometry  # noqa: E402
-from ptrans.anphon import synthetic_table  # noqa: E402
-from ptrans.constants import M_TWOPI  # noqa: E402
+from ptrans import post  # noqa: E402
+from ptrans.anphon import Anphon, synthetic_table  # noqa: E402
+from ptrans.constants import M_TWOPI, PREC_FP32_RETEST, PREC_FP64  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector, published
 METRIC = "phonon paths/s and κ-gradient evals/s on 1 MI355X at fixed κ std-error"
@@ -43,7 +44,21 @@ def workload(np_per_cell: int, nangle: int = 11, nb: int = 6, nk: int = 2000):
     geo = geometry.build_geometry(model)
     t = synthetic_table(nb, nk, seed=20)
     omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12          # RT:727
-    return model, geo, omega
+    table = Anphon(material="synthetic", rtemp=t.rtemp, vol=t.vol, omega=omega, tau=t.tau_ps,
+                   vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    return model, geo, omega, table
+
+
+def kappa_of(counts, model, table, np_per_cell):
+    """kappa, its binomial standard error and the paths needed for a 1e-3 relative error,
+    through effectiveMFP + Matthiessen as the reference's fileoutput (RT:2418-2755)."""
+    ang = post.vangle(model.nangle)
+    semfp = post.effective_mfp(counts / float(np_per_cell), ang, model.box[0])[..., None]
+    res = post.kappa(semfp, table, model.temperature, [1.0e10], True, counts=counts,
+                     np_=np_per_cell, angles=ang, boxx=model.box[0])
+    paths = counts.shape[0] * counts.shape[1] * (counts.shape[2] - 1) * np_per_cell
+    need = paths * (res.kappa_stderr / (1e-3 * res.kappa)) ** 2 if res.kappa > 0 else None
+    return res.kappa, res.kappa_stderr, need
 
 
 def cpu_baseline(geo, omega, mfp, nangle, np_per_cell, seed, seconds, threads):
@@ -79,10 +94,11 @@ def cpu_baseline(geo, omega, mfp, nangle, np_per_cell, seed, seconds, threads):
 
 
 def roofline(cstats, kernel_ms):
-    """FP64-VALU roofline of trace_grid_kernel: algorithmic FP64 operations of one
-    launch (instrumented twin kernel, same paths) over the timed kernel's average
-    duration (HIP events on the library's stream).  HBM traffic per launch comes from
-    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present."""
+    """VALU roofline of the trace kernel: algorithmic FP64 and FP32 operations of one
+    launch (instrumented twin kernel, same paths) over the timed kernel's average duration
+    (HIP events on the library's stream), against MI355X's FP64 vector peak (non-packed
+    FP32 issues at the same rate).  HBM traffic per launch comes from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present."""
     ops = cstats["fp64_ops"]
     achieved = ops / (kernel_ms * 1e-3) / 1e12
     traffic = None
@@ -90,7 +106,7 @@ def roofline(cstats, kernel_ms):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    return {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
             "fp64_ops_per_launch": ops, "fp64_ops_per_path": ops / max(1, cstats["paths"]),
             "philox_blocks_per_path": cstats["philox_blocks"] / max(1, cstats["paths"])}
@@ -127,31 +143,41 @@ def main():
     from ptrans.trace import Context
 
     nangle, mfp, seed = 11, 1.0e10, 12345
-    model, geo, omega = workload(args.np, nangle)
+    model, geo, omega, table = workload(args.np, nangle)
     nb, nk = omega.shape
     paths = nb * nk * (nangle - 1) * args.np
+    res = {}
     with Context(seed=seed, device=0) as ctx:
         g = ctx.geometry(geo)
         m = ctx.modes(omega=omega)
-        for _ in range(args.warmup):
-            ctx.trace_grid(g, m, mfp, nangle, args.np)
-        torch.cuda.synchronize()
-        kern_ms = []
-        stats = None
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            counts, stats = ctx.trace_grid(g, m, mfp, nangle, args.np)
-            kern_ms.append(stats["kernel_ms"])
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        # one more launch of the instrumented twin kernel (untimed) for the FP64 op count
-        ctx.set_op_counting(True)
-        _, cstats = ctx.trace_grid(g, m, mfp, nangle, args.np)
-        ctx.set_op_counting(False)
+        for prec in (PREC_FP64, PREC_FP32_RETEST):
+            for _ in range(args.warmup):
+                ctx.trace_grid(g, m, mfp, nangle, args.np, precision=prec)
+            torch.cuda.synchronize()
+            kern_ms = []
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                counts, stats = ctx.trace_grid(g, m, mfp, nangle, args.np, precision=prec)
+                kern_ms.append(stats["kernel_ms"])
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            # one more launch of the instrumented twin kernel (untimed) for the op count
+            ctx.set_op_counting(True)
+            _, cstats = ctx.trace_grid(g, m, mfp, nangle, args.np, precision=prec)
+            ctx.set_op_counting(False)
+            res[prec] = dict(dt=dt, kernel_ms=float(np.mean(kern_ms)), counts=counts, stats=stats,
+                             cstats=cstats)
+    fast, plain = res[PREC_FP32_RETEST], res[PREC_FP64]
+    dt = fast["dt"]
     value = paths * args.steps / dt
-    k_ms = float(np.mean(kern_ms))
+    k_ms = fast["kernel_ms"]
+    cstats = fast["cstats"]
+    stats = fast["stats"]
+    counts = fast["counts"]
     events_per_path = stats["events"] / stats["paths"]
     transmitted = counts.sum() / paths
+    kap, kap_err, need = kappa_of(counts, model, table, args.np)
+    kap64, _, _ = kappa_of(plain["counts"], model, table, args.np)
 
     cpu = None
     if not args.no_cpu:
@@ -174,7 +200,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic (seeded film geometry and bulk table)",
         "config": {"workload": "thin film 600x600x200 nm, 3 z-cylinder holes r=50 nm p=0.3, "
                                "6x2000 modes x 10 angles x %d particles, ballistic FP64 grid "
@@ -182,6 +208,18 @@ def main():
                    "paths_per_step": paths, "events_per_path": events_per_path,
                    "transmission": transmitted},
         "kernel_ms": k_ms,
+        "precision": "FP32 intersection filter, FP64 retest and FP64 state/tallies",
+        "fp64_plain": {"value": paths * args.steps / plain["dt"],
+                       "ms_per_step": plain["dt"] / args.steps * 1e3,
+                       "kernel_ms": plain["kernel_ms"],
+                       "roofline": roofline(plain["cstats"], plain["kernel_ms"])},
+        "speedup_vs_fp64": plain["dt"] / dt,
+        "kappa": {"value_W_mK": kap, "stderr": kap_err, "rel_err_vs_fp64": abs(kap - kap64) / kap64,
+                  "counts_identical_to_fp64": bool(np.array_equal(counts, plain["counts"])),
+                  "paths_for_1e-3_rel_stderr": need,
+                  "seconds_for_1e-3_rel_stderr": need / value if need else None},
+        "retests_per_event": stats["retests"] / max(1, stats["events"]),
+        "fallbacks_per_event": stats["fallbacks"] / max(1, stats["events"]),
         "roofline": roofline(cstats, k_ms),
         "cpu_baseline": cpu,
     }

@@ -62,6 +62,10 @@ extern "C" {
 
 /* arithmetic of the trace kernel */
 #define PT_PREC_FP64 0           /* every step in FP64 (the reference's arithmetic)      */
+#define PT_PREC_FP32_RETEST 1    /* FP32 intersection filter with bounded errors; the
+                                    event's surface is certified or re-tested in FP64
+                                    and its distance recomputed in FP64, so the state
+                                    evolves with FP64 numbers                           */
 
 /*
  * One surface record: the fields of Surface_type (src/RTMCphon.f90:94-111) that the
@@ -138,6 +142,8 @@ typedef struct pt_stats {
     double  reduce_ms;       /* device time of the per-cell tally reduction          */
     int64_t fp64_ops;        /* algorithmic FP64 operations (op counting on only)    */
     int64_t philox_blocks;   /* Philox4x32-10 blocks evaluated (op counting on only) */
+    int64_t retests;         /* PT_PREC_FP32_RETEST: candidates settled by FP64       */
+    int64_t fallbacks;       /* PT_PREC_FP32_RETEST: events decided by the FP64 scan  */
 } pt_stats;
 
 /* Per-path record of pt_replay(). */
@@ -229,7 +235,7 @@ int pt_trace_grid(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
  */
 int pt_replay(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
               double mfp_nm, int32_t nangle, const double *vangle, int64_t np,
-              int64_t first_path, int64_t n, pt_path_record *records);
+              int64_t first_path, int64_t n, int32_t precision, pt_path_record *records);
 
 /*
  * Alias-mode trace (an extension of the reference's estimator, whose (k, b, angle)
@@ -247,7 +253,7 @@ int pt_trace_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, int6
 
 /* Per-path records of alias-mode paths first .. first+n-1 (parity checks). */
 int pt_replay_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
-                    const double *branch_scale, int64_t first, int64_t n,
+                    const double *branch_scale, int64_t first, int64_t n, int32_t precision,
                     pt_path_record *records);
 
 #ifdef __cplusplus

@@ -216,6 +216,30 @@ int pt_surface_setup_polygon(int32_t nv, const double *v, double center[3], doub
     return PT_OK;
 }
 
+// True when grain 0 is exactly the six walls setupBoxGrain builds (RT:1266-1331): order
+// x-, x+, y-, y+, z-, z+, internal quads on the box faces with axis-aligned unit normals.
+static bool box_is_axis_aligned(const pt_geom_desc *d)
+{
+    if (d->grain_offset[1] != 6) return false;
+    for (int w = 0; w < 6; ++w) {
+        const pt_surface &s = d->surfaces[w];
+        const int a = w / 2;
+        const double P = (w & 1) ? d->box[a] : 0.0;
+        if (s.kind != PT_SHAPE_POLYGON || s.nvertex != 4 || !s.internal) return false;
+        for (int k = 0; k < 3; ++k) {
+            const double want = k == a ? ((w & 1) ? 1.0 : -1.0) : 0.0;
+            if (s.normal[k] != want) return false;
+        }
+        for (int i = 0; i < 4; ++i) {
+            const double *v = d->vertices + 3 * (s.vertex_offset + i);
+            if (v[a] != P) return false;
+            for (int k = 0; k < 3; ++k)
+                if (v[k] != 0.0 && v[k] != d->box[k]) return false;
+        }
+    }
+    return true;
+}
+
 int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
 {
     if (!c || !d || !out) return fail(PT_E_INVALID, "pt_geom_create: NULL argument");
@@ -326,6 +350,7 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     g.nmaterial = d->nmaterial;
     g.ngrain_total = ng;
     g.nsurface = ns;
+    g.box_aligned = box_is_axis_aligned(d) ? 1 : 0;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         pt_geom_destroy(G);
@@ -490,6 +515,7 @@ TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
     P.seed = c->seed;
     P.chunk = c->chunk;
     P.event_limit = c->event_limit;
+    P.precision = PT_PREC_FP64;
     P.queue = c->counters;
     P.stats = c->counters + 1;
     return P;
@@ -525,8 +551,10 @@ void fill_stats(pt_stats *stats, const unsigned long long *h, float ms, float ms
     stats->skipped = (int64_t)h[7];
     stats->kernel_ms = ms;
     stats->reduce_ms = ms_red;
-    stats->fp64_ops = (int64_t)h[9];
-    stats->philox_blocks = (int64_t)h[10];
+    stats->fp64_ops = (int64_t)h[8];
+    stats->philox_blocks = (int64_t)h[9];
+    stats->retests = (int64_t)h[11];
+    stats->fallbacks = (int64_t)h[12];
 }
 
 int replay_common(pt_ctx *c, TraceParams &P, int mode, int64_t first, int64_t n,
@@ -555,7 +583,8 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
 {
     int rc = check_trace_args(c, G, M, mfp, nangle, np);
     if (rc) return rc;
-    if (precision != PT_PREC_FP64) return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (!counts) return fail(PT_E_INVALID, "counts is NULL");
     const int64_t nb = M->nb, nk = M->nk;
     const int64_t ncell = nb * nk * (nangle - 1);
@@ -583,6 +612,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     std::vector<double> ang(ct);
     ang.insert(ang.end(), st.begin(), st.end());
     TraceParams P = base_params(c, G, M);
+    P.precision = precision;
     P.cos_t = d_ang;
     P.sin_t = d_ang + nangle;
     P.mfp = mfp;
@@ -620,11 +650,13 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
 
 int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
               const double *vangle, int64_t np, int64_t first_path, int64_t n,
-              pt_path_record *records)
+              int32_t precision, pt_path_record *records)
 {
     int rc = check_trace_args(c, G, M, mfp, nangle, np);
     if (rc) return rc;
     if (np < 1) return fail(PT_E_INVALID, "np must be >= 1");
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (n < 0 || first_path < 0) return fail(PT_E_INVALID, "bad path range");
     if (n == 0) return PT_OK;
     if (!records) return fail(PT_E_INVALID, "records is NULL");
@@ -640,6 +672,7 @@ int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_
     hipError_t e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "pt_replay: %s", hipGetErrorString(e)); }
     TraceParams P = base_params(c, G, M);
+    P.precision = precision;
     P.cos_t = d_ang;
     P.sin_t = d_ang + nangle;
     P.mfp = mfp;
@@ -657,7 +690,8 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
     if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
     if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
-    if (precision != PT_PREC_FP64) return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
     if (!mode_paths || !mode_trans) return fail(PT_E_INVALID, "tally arrays are NULL");
     if (stats) memset(stats, 0, sizeof *stats);
@@ -665,6 +699,7 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     const size_t nm = (size_t)M->nb * M->nk;
     PT_HIP(hipSetDevice(c->device));
     TraceParams P = base_params(c, G, M);
+    P.precision = precision;
     int rc = set_scale(c, M, branch_scale, P);
     if (rc) return rc;
     P.total = npaths;
@@ -702,8 +737,10 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
 }
 
 int pt_replay_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, const double *branch_scale,
-                    int64_t first, int64_t n, pt_path_record *records)
+                    int64_t first, int64_t n, int32_t precision, pt_path_record *records)
 {
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
     if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
     if (n < 0 || first < 0) return fail(PT_E_INVALID, "bad path range");
@@ -711,6 +748,7 @@ int pt_replay_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, const double
     if (!records) return fail(PT_E_INVALID, "records is NULL");
     PT_HIP(hipSetDevice(c->device));
     TraceParams P = base_params(c, G, M);
+    P.precision = precision;
     int rc = set_scale(c, M, branch_scale, P);
     if (rc) return rc;
     return replay_common(c, P, TRACE_ALIAS, first, n, records);

@@ -46,6 +46,7 @@ struct DevGeom {
     int32_t nmaterial;
     int32_t ngrain_total;
     int32_t nsurface;
+    int32_t box_aligned;       // grain 0 is the six axis-aligned walls of setupBoxGrain
 };
 
 struct DevModes {
@@ -78,9 +79,11 @@ struct TraceParams {
     int32_t nangle;            // grid
     int32_t chunk;             // paths a wave takes from the queue at once
     int64_t event_limit;
+    int32_t precision;         // PT_PREC_FP64 or PT_PREC_FP32_RETEST
     unsigned long long *queue; // next path id
-    unsigned long long *stats; // paths, events, draws, lost, mismatch, limit, -, -,
-                               // fp64 ops, philox blocks (counting kernel only)
+    unsigned long long *stats; // paths, events, draws, lost, mismatch, limit, (skipped),
+                               // fp64 ops, philox blocks (counting kernel only),
+                               // FP64 retests, full-scan fallbacks
     uint8_t *outcome;          // grid: [total]
     unsigned long long *mode_paths;   // alias: [nb*nk]
     unsigned long long *mode_trans;   // alias: [nb*nk]

@@ -309,7 +309,203 @@ struct Path {
     int flags;
     int start_grain;
     int64_t events;
+    unsigned retests;       // fast path: candidates evaluated in FP64 by the filter
+    unsigned fallbacks;     // fast path: events decided by the full FP64 scan
 };
+
+// ---------------------------------------------------------------------------------
+// FP32 filter of the fast path.  Each candidate gets a distance interval [lo, hi] that
+// contains the exact FP64 result of the reference's routine, from FP32 arithmetic on
+// coordinates relative to the current position, or the verdict "unsure" (grazing, near
+// an edge, tangent, near zero), which is then settled by the FP64 routine.  The event's
+// surface is certified when one candidate's interval lies entirely below all others;
+// its distance is then recomputed by the FP64 routine, so the trace evolves with exactly
+// the FP64 numbers.  Anything else falls back to the full FP64 scan.
+// Error bounds: the FP32 quantities carry a handful of roundings of relative size
+// 2^-24 (conversion, products, v_rcp/v_rsq/v_sqrt at ~1 ulp); kMarg = 256 * 2^-24
+// covers them with a wide safety factor.
+constexpr float kEps32 = 5.9604645e-8f;
+constexpr float kMarg = 256.0f * kEps32;
+enum { F_MISS = 0, F_HIT = 1, F_UNSURE = 2 };
+
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Axis-aligned box wall facing the ray on axis a (the box grain of setupBoxGrain,
+// RT:1255-1332): cos and the sign of L are exact; the in-face test has margins.
+__device__ int filt_box_axis(const DevGeom &g, int a, const double *rs, const double *nd,
+                             int *widx, float *lo, float *hi)
+{
+    const double nda = nd[a];
+    if (nda == 0.0) return F_MISS;                       // both walls back-facing
+    const bool minus = nda < 0.0;
+    *widx = 2 * a + (minus ? 0 : 1);
+    const double P = minus ? 0.0 : g.box[a];
+    const double num = P - rs[a];
+    if ((num < 0.0 && !minus) || (num > 0.0 && minus)) return F_MISS;   // L < 0
+    const float L = (float)num * frcp((float)nda);
+    const int u = a == 2 ? 0 : a + 1, v = a == 0 ? 2 : (a == 1 ? 0 : 1);
+    const float du = (float)g.box[u], dv = (float)g.box[v];
+    const float ru = (float)rs[u], rv = (float)rs[v];
+    const float hu = ru + (float)nd[u] * L, hv = rv + (float)nd[v] * L;
+    const float mu = kMarg * (fabsf(ru) + fabsf(hu - ru) + du) + 1e-30f;
+    const float mv = kMarg * (fabsf(rv) + fabsf(hv - rv) + dv) + 1e-30f;
+    if (hu < -mu || hu > du + mu || hv < -mv || hv > dv + mv) return F_MISS;
+    if (hu < mu || hu > du - mu || hv < mv || hv > dv - mv) return F_UNSURE;
+    *lo = L * (1.0f - kMarg);
+    *hi = L * (1.0f + kMarg) + 1e-30f;
+    return F_HIT;
+}
+
+// z-cylinder (calcSurfaceDistanceCylinder, RT:3040-3088)
+__device__ int filt_cylinder(const DSurf &s, const double *rs, const double *nd, double boxz,
+                             float *lo, float *hi)
+{
+    const float dx = (float)(s.c[0] - rs[0]), dy = (float)(s.c[1] - rs[1]);
+    const float nx = (float)nd[0], ny = (float)nd[1];
+    const float n2 = nx * nx + ny * ny;
+    if (n2 < 1e-10f) return F_UNSURE;                    // near axis-parallel (0/0 in FP64)
+    const float inv = frsq(n2);
+    const float tx = nx * inv, ty = ny * inv;
+    const float rd = dx * tx + dy * ty;
+    const float rr = dx * dx + dy * dy;
+    const float r = (float)s.r, r2 = r * r;
+    const float rd2 = rd * rd;
+    const float D = rd2 + r2 - rr;
+    const float mD = kMarg * (rd2 + r2 + rr) + 1e-30f;
+    if (D < -mD) return F_MISS;
+    if (D <= mD) return F_UNSURE;
+    const float sq = fsqrt(D);
+    const float mrd = kMarg * fsqrt(rr);
+    const float ml = mrd + mD / sq + kMarg * (fabsf(rd) + sq) + 1e-30f;
+    const float l1 = rd - sq, l2 = rd + sq;
+    float l;
+    if (l1 > ml) l = l1;
+    else if (l1 >= -ml) return F_UNSURE;
+    else if (l2 > ml) l = l2;
+    else if (l2 >= -ml) return F_UNSURE;
+    else return F_MISS;
+    const float L = l * inv;
+    const float mL = ml * inv + kMarg * L;
+    const float nz = (float)nd[2], rz = (float)rs[2];
+    const float zh = rz + nz * L;
+    const float mz = kMarg * (fabsf(rz) + fabsf(nz * L)) + fabsf(nz) * mL + 1e-30f;
+    const float bz = (float)boxz;
+    if (zh < -mz || zh > bz + mz) return F_MISS;
+    if (zh < mz || zh > bz - mz) return F_UNSURE;
+    *lo = L - mL;
+    *hi = L + mL;
+    return F_HIT;
+}
+
+// sphere (calcSurfaceDistanceSphere, RT:2985-3038), including its outside-and-leaving guard
+__device__ int filt_sphere(const DSurf &s, const double *rs, const double *nd, float *lo,
+                           float *hi)
+{
+    const float dx = (float)(s.c[0] - rs[0]), dy = (float)(s.c[1] - rs[1]),
+                dz = (float)(s.c[2] - rs[2]);
+    const float nx = (float)nd[0], ny = (float)nd[1], nz = (float)nd[2];
+    const float rd = dx * nx + dy * ny + dz * nz;
+    const float rr = dx * dx + dy * dy + dz * dz;
+    const float r = (float)s.r, r2 = r * r;
+    const float mrr = kMarg * (rr + r2) + 1e-30f;
+    const float mrd = kMarg * fsqrt(rr) + 1e-30f;
+    const float out = rr - r2;                           // guard: rr > r^2 - 1e-10
+    if (out > mrr) {
+        if (rd < -mrd) return F_MISS;
+        if (rd <= mrd) return F_UNSURE;
+    } else if (out >= -mrr) {
+        return F_UNSURE;
+    }
+    const float rd2 = rd * rd;
+    const float D = rd2 + r2 - rr;
+    const float mD = kMarg * (rd2 + r2 + rr) + 1e-30f;
+    if (D < -mD) return F_MISS;
+    if (D <= mD) return F_UNSURE;
+    const float sq = fsqrt(D);
+    const float ml = mrd + mD / sq + kMarg * (fabsf(rd) + sq) + 1e-30f;
+    const float l1 = rd - sq, l2 = rd + sq;
+    float l;
+    if (l1 > ml) l = l1;
+    else if (l1 >= -ml) return F_UNSURE;
+    else if (l2 > ml) l = l2;
+    else if (l2 >= -ml) return F_UNSURE;
+    else return F_MISS;
+    *lo = l - ml;
+    *hi = l + ml;
+    return F_HIT;
+}
+
+// Running selection over candidate intervals in scan order.
+struct Pick {
+    double hi_min, lo1, lo2;
+    int idx_hi, idx_lo1;
+    double exact_l;       // FP64 distance of idx_hi when it was evaluated exactly
+    bool exact;
+    __device__ void init() { hi_min = lo1 = lo2 = 1.0e300; idx_hi = idx_lo1 = -1; exact = false; }
+    __device__ void add(int idx, double lo, double hi, bool is_exact)
+    {
+        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = is_exact; exact_l = lo; }
+        if (lo < lo1) { lo2 = lo1; lo1 = lo; idx_lo1 = idx; }
+        else if (lo < lo2) lo2 = lo;
+    }
+};
+
+// The fast path's findCollisionSurface.  Returns false when the FP64 full scan must
+// decide; else sets *dmin and *best exactly as the reference's scan would.
+__device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, OpCount &oc)
+{
+    Pick pk;
+    pk.init();
+    const double *rs = ph.pos, *nd = ph.dir;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ig = pass == 0 ? 0 : ph.which;
+        if (pass == 1 && ig == 0) break;
+        if (ig == 0 && g.box_aligned) {
+            for (int a = 0; a < 3; ++a) {
+                int w;
+                float lo, hi;
+                const int r = filt_box_axis(g, a, rs, nd, &w, &lo, &hi);
+                oc.flops += 10;
+                if (r == F_HIT) {
+                    pk.add(w, lo, hi, false);
+                } else if (r == F_UNSURE) {
+                    double l;
+                    ph.retests++;
+                    if (surface_distance(g, g.surf[w], rs, nd, &l, oc)) pk.add(w, l, l, true);
+                }
+            }
+            continue;
+        }
+        const int j1 = g.cand_off[ig + 1];
+        for (int j = g.cand_off[ig]; j < j1; ++j) {
+            const int idx = g.cand_idx[j];
+            const DSurf &s = g.surf[idx];
+            float lo = 0.f, hi = 0.f;
+            int r;
+            if (s.kind == PT_SHAPE_CYLINDER) { r = filt_cylinder(s, rs, nd, g.box[2], &lo, &hi); oc.flops += 30; }
+            else if (s.kind == PT_SHAPE_SPHERE) { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
+            else r = F_UNSURE;                                // polygons: FP64 routine
+            if (r == F_HIT) {
+                pk.add(idx, lo, hi, false);
+            } else if (r == F_UNSURE) {
+                double l;
+                if (s.kind != PT_SHAPE_POLYGON) ph.retests++;
+                if (surface_distance(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
+            }
+        }
+    }
+    if (pk.idx_hi < 0 || pk.idx_lo1 != pk.idx_hi || !(pk.lo2 > pk.hi_min)) return false;
+    double l = pk.exact_l;
+    if (!pk.exact) {
+        if (!surface_distance(g, g.surf[pk.idx_hi], rs, nd, &l, oc)) return false;
+    }
+    if (!(1.0e10 > l)) return false;                      // reference: dmin starts at 1e10
+    *dmin = l;
+    *best = pk.idx_hi;
+    return true;
+}
 
 // determinTransmit, RT:2384-2413
 __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, double max_omega,
@@ -369,6 +565,8 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     ph.cur = 0;                                                             // igrain=0, isurface=1
     ph.flags = 0;
     ph.events = 0;
+    ph.retests = 0;
+    ph.fallbacks = 0;
     int which = 0;
     if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
     else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos, oc);
@@ -384,6 +582,7 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
 }
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
+template <bool kFast>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
                           Path &ph)
 {
@@ -406,16 +605,27 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     // strictly nearest hit wins; with no hit the previous surface is kept.
     double dmin = 1.0e10;
     int best = ph.cur;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int ig = pass == 0 ? 0 : ph.which;
-        if (pass == 1 && ig == 0) break;
-        const int j1 = g.cand_off[ig + 1];
-        for (int j = g.cand_off[ig]; j < j1; ++j) {
-            const int idx = g.cand_idx[j];
-            double l;
-            if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) && dmin > l) {
-                dmin = l;
-                best = idx;
+    bool decided = false;
+    if (kFast) {
+        decided = scan_fast(g, ph, &dmin, &best, oc);
+        if (!decided) {
+            dmin = 1.0e10;
+            best = ph.cur;
+            ph.fallbacks++;
+        }
+    }
+    if (!decided) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const int ig = pass == 0 ? 0 : ph.which;
+            if (pass == 1 && ig == 0) break;
+            const int j1 = g.cand_off[ig + 1];
+            for (int j = g.cand_off[ig]; j < j1; ++j) {
+                const int idx = g.cand_idx[j];
+                double l;
+                if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) && dmin > l) {
+                    dmin = l;
+                    best = idx;
+                }
             }
         }
     }
@@ -586,7 +796,7 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
 // refills lanes whose path has ended (ballot + prefix count); lanes then advance one event
 // per iteration.  Grid mode writes one outcome byte per path (reduced per cell later);
 // alias mode adds to per-mode integer tallies (exact, order independent).
-template <int kMode, bool kCount>
+template <int kMode, bool kCount, bool kFast>
 __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
 {
     const int lane = threadIdx.x & 63;
@@ -599,6 +809,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
     int64_t pid = 0;
     int mode = 0;
     unsigned long long s_paths = 0, s_events = 0, s_draws = 0, s_lost = 0, s_mis = 0, s_lim = 0;
+    unsigned long long s_ret = 0, s_fb = 0;
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
 
@@ -636,7 +847,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
             continue;
         }
         if (active) {
-            if (ph.end_side != 0 || step_path(P.g, P.m.max_omega, P.event_limit, rng, ph)) {
+            if (ph.end_side != 0 || step_path<kFast>(P.g, P.m.max_omega, P.event_limit, rng, ph)) {
                 const bool transmitted = ph.end_side != 1;          // RT:1833
                 if (kMode == TRACE_GRID) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
@@ -649,6 +860,8 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
                 s_draws += rng.draw;
                 if (ph.flags & PT_PATH_LOST) s_lost++;
                 if (ph.flags & PT_PATH_EVENT_LIMIT) s_lim++;
+                s_ret += ph.retests;
+                s_fb += ph.fallbacks;
                 active = false;
             }
         }
@@ -659,6 +872,8 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
     s_lost = wave_sum(s_lost);
     s_mis = wave_sum(s_mis);
     s_lim = wave_sum(s_lim);
+    s_ret = wave_sum(s_ret);
+    s_fb = wave_sum(s_fb);
     if (lane == 0) {
         atomicAdd(P.stats + 0, s_paths);
         atomicAdd(P.stats + 1, s_events);
@@ -666,12 +881,14 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
         atomicAdd(P.stats + 3, s_lost);
         atomicAdd(P.stats + 4, s_mis);
         atomicAdd(P.stats + 5, s_lim);
+        atomicAdd(P.stats + 10, s_ret);
+        atomicAdd(P.stats + 11, s_fb);
     }
     if (kCount) {
         const unsigned long long f = wave_sum(oc.flops), b = wave_sum(oc.philox);
         if (lane == 0) {
-            atomicAdd(P.stats + 8, f);
-            atomicAdd(P.stats + 9, b);
+            atomicAdd(P.stats + 7, f);
+            atomicAdd(P.stats + 8, b);
         }
     }
 }
@@ -719,7 +936,7 @@ __global__ void __launch_bounds__(256) reduce_grid_kernel(const uint8_t *__restr
 }
 
 // Single paths with full records (parity checks), one lane per path.
-template <int kMode>
+template <int kMode, bool kFast>
 __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first, int64_t n,
                                                     pt_path_record *rec)
 {
@@ -733,7 +950,7 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
     int mode = 0;
     if (begin_path<kMode>(P, first + i, rng, ph, &mode)) {
         while (ph.end_side == 0) {
-            if (step_path(P.g, P.m.max_omega, P.event_limit, rng, ph)) break;
+            if (step_path<kFast>(P.g, P.m.max_omega, P.event_limit, rng, ph)) break;
         }
     }
     pt_path_record r;
@@ -750,20 +967,24 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
 hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
                         hipStream_t s)
 {
+    const dim3 gr(blocks), bl(threads);
+    const bool fast = p.precision == PT_PREC_FP32_RETEST;
+#define PT_LAUNCH(M, C, F) hipLaunchKernelGGL((trace_kernel<M, C, F>), gr, bl, 0, s, p)
     if (mode == TRACE_GRID) {
-        if (count) hipLaunchKernelGGL((trace_kernel<TRACE_GRID, true>), dim3(blocks), dim3(threads), 0, s, p);
-        else hipLaunchKernelGGL((trace_kernel<TRACE_GRID, false>), dim3(blocks), dim3(threads), 0, s, p);
+        if (fast) { if (count) PT_LAUNCH(TRACE_GRID, true, true); else PT_LAUNCH(TRACE_GRID, false, true); }
+        else { if (count) PT_LAUNCH(TRACE_GRID, true, false); else PT_LAUNCH(TRACE_GRID, false, false); }
     } else {
-        if (count) hipLaunchKernelGGL((trace_kernel<TRACE_ALIAS, true>), dim3(blocks), dim3(threads), 0, s, p);
-        else hipLaunchKernelGGL((trace_kernel<TRACE_ALIAS, false>), dim3(blocks), dim3(threads), 0, s, p);
+        if (fast) { if (count) PT_LAUNCH(TRACE_ALIAS, true, true); else PT_LAUNCH(TRACE_ALIAS, false, true); }
+        else { if (count) PT_LAUNCH(TRACE_ALIAS, true, false); else PT_LAUNCH(TRACE_ALIAS, false, false); }
     }
+#undef PT_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t trace_occupancy(int threads, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                        trace_kernel<TRACE_GRID, false>, threads, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, trace_kernel<TRACE_GRID, false, false>, threads, 0);
 }
 
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,
@@ -781,10 +1002,14 @@ hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t 
 {
     if (n == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n + 63) / 64);
-    if (mode == TRACE_GRID)
-        hipLaunchKernelGGL(replay_kernel<TRACE_GRID>, dim3(blocks), dim3(64), 0, s, p, first, n, rec);
-    else
-        hipLaunchKernelGGL(replay_kernel<TRACE_ALIAS>, dim3(blocks), dim3(64), 0, s, p, first, n, rec);
+    const bool fast = p.precision == PT_PREC_FP32_RETEST;
+    if (mode == TRACE_GRID) {
+        if (fast) hipLaunchKernelGGL((replay_kernel<TRACE_GRID, true>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
+        else hipLaunchKernelGGL((replay_kernel<TRACE_GRID, false>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
+    } else {
+        if (fast) hipLaunchKernelGGL((replay_kernel<TRACE_ALIAS, true>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
+        else hipLaunchKernelGGL((replay_kernel<TRACE_ALIAS, false>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
+    }
     return hipGetLastError();
 }
 

@@ -29,7 +29,8 @@ class PtStats(ctypes.Structure):
                 ("lost", ctypes.c_int64), ("start_mismatch", ctypes.c_int64),
                 ("skipped", ctypes.c_int64), ("kernel_ms", ctypes.c_double),
                 ("reduce_ms", ctypes.c_double), ("fp64_ops", ctypes.c_int64),
-                ("philox_blocks", ctypes.c_int64)]
+                ("philox_blocks", ctypes.c_int64), ("retests", ctypes.c_int64),
+                ("fallbacks", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,10 +75,10 @@ def load():
     L.pt_modes_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
     L.pt_modes_destroy.argtypes = [vp]
     L.pt_trace_grid.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i32, vp, vp]
-    L.pt_replay.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i64, i64, vp]
+    L.pt_replay.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i64, i64, i32, vp]
     L.pt_alias_build.argtypes = [i64, vp, vp, vp]
     L.pt_trace_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
-    L.pt_replay_alias.argtypes = [vp, vp, vp, vp, i64, i64, vp]
+    L.pt_replay_alias.argtypes = [vp, vp, vp, vp, i64, i64, i32, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

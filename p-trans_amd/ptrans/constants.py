@@ -23,6 +23,7 @@ PT_E_UNSUPPORTED = -5
 PT_E_NODEVICE = -6
 
 PREC_FP64 = 0
+PREC_FP32_RETEST = 1
 
 PATH_LOST = 1
 PATH_START_MISMATCH = 2

@@ -84,6 +84,8 @@ def kappa(semfp: np.ndarray, table, temperature: float, mfp_samples: List[float]
     nqfbz = int(weight.sum(dtype=np.float64) / nb)
     # per-branch exchange sort by omega (RT:2658-2691)
     for ii in range(nb):
+        if np.all(omega[ii, :-1] <= omega[ii, 1:]):
+            continue                    # already ordered: the exchange sort swaps nothing
         for jj in range(nk - 1):
             for kk in range(jj + 1, nk):
                 if omega[ii, jj] > omega[ii, kk]:

@@ -125,7 +125,7 @@ class Context:
         return transmit, counts, stats
 
     def replay(self, geom: Geom, modes: Modes, mfp_nm: float, nangle: int, np_: int,
-               first: int, n: int, vangle=None):
+               first: int, n: int, vangle=None, precision: int = PREC_FP64):
         """pt_replay: per-path records of paths first .. first+n-1."""
         out = np.zeros(n, dtype=_lib.RECORD_DTYPE)
         va = None
@@ -134,7 +134,7 @@ class Context:
         _lib.check(_lib.load().pt_replay(
             self.handle, geom.handle, modes.handle, float(mfp_nm), int(nangle),
             va.ctypes.data if va is not None else None, int(np_), int(first), int(n),
-            out.ctypes.data))
+            int(precision), out.ctypes.data))
         return out
 
     def trace_alias(self, geom: Geom, modes: Modes, npaths: int, branch_scale=None,
@@ -152,12 +152,13 @@ class Context:
         return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"),
                 stats.as_dict())
 
-    def replay_alias(self, geom: Geom, modes: Modes, first: int, n: int, branch_scale=None):
+    def replay_alias(self, geom: Geom, modes: Modes, first: int, n: int, branch_scale=None,
+                     precision: int = PREC_FP64):
         out = np.zeros(n, dtype=_lib.RECORD_DTYPE)
         sc = None if branch_scale is None else np.ascontiguousarray(branch_scale, dtype=np.float64)
         _lib.check(_lib.load().pt_replay_alias(
             self.handle, geom.handle, modes.handle, sc.ctypes.data if sc is not None else None,
-            int(first), int(n), out.ctypes.data))
+            int(first), int(n), int(precision), out.ctypes.data))
         return out
 
 

@@ -106,3 +106,47 @@ def test_results_independent_of_run(gpu_ctx):
     c2, _ = gpu_ctx.trace_grid(g, m, 500.0, 5, 500)
     np.testing.assert_array_equal(a, b)
     assert not np.array_equal(a, c2)
+
+
+# ---------------------------------------------------------------- FP32 filter + FP64 retest
+
+FAST = 1
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fast_replay_identical_to_fp64(gpu_ctx, name):
+    """The FP32-filter path certifies each event's surface (or re-tests it in FP64) and
+    recomputes its distance in FP64, so every path evolves exactly as in FP64."""
+    c = Case(name)
+    mt = c.meta
+    print("case", name, flush=True)
+    g, m = _gpu_case(gpu_ctx, c, mt["paths_seed"])
+    fast = gpu_ctx.replay(g, m, mt["mfp"], mt["nangle"], mt["nphonon"], 0, mt["npaths"], precision=FAST)
+    ref = gpu_ctx.replay(g, m, mt["mfp"], mt["nangle"], mt["nphonon"], 0, mt["npaths"])
+    for f in ("end_side", "flags", "events", "draws", "which_grain"):
+        np.testing.assert_array_equal(fast[f], ref[f], err_msg=f)
+    np.testing.assert_array_equal(fast["pos"].view(np.uint64), ref["pos"].view(np.uint64))
+    np.testing.assert_array_equal(fast["dir"].view(np.uint64), ref["dir"].view(np.uint64))
+
+
+@pytest.mark.parametrize("builder", ["film", "inclusions"])
+def test_fast_grid_counts_identical_to_fp64_large(gpu_ctx, builder):
+    """Half a million to a few million paths: FP32-filter counts, events and draws equal the
+    FP64 trace's exactly; the fraction of events needing the FP64 full scan is small."""
+    if builder == "film":
+        model = gen.film_holes(seed=2, nphonon=20000, nangle=11, mfp=(300.0,))
+    else:
+        model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.4, nphonon=20000, nangle=11,
+                               mfp=(500.0,))
+    geo = geometry.build_geometry(model)
+    om = np.linspace(1e12, 9e13, 6).reshape(2, 3)
+    gpu_ctx.set_seed(31)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    a, sa = gpu_ctx.trace_grid(g, m, model.mfp[0], 11, 20000)
+    b, sb = gpu_ctx.trace_grid(g, m, model.mfp[0], 11, 20000, precision=FAST)
+    print(builder, "events", sa["events"], "retests", sb["retests"], "fallbacks", sb["fallbacks"],
+          "fp64 ms", sa["kernel_ms"], "fast ms", sb["kernel_ms"], flush=True)
+    np.testing.assert_array_equal(a, b)
+    assert sa["events"] == sb["events"] and sa["draws"] == sb["draws"]
+    assert sb["fallbacks"] < 0.01 * sb["events"]
