@@ -32,7 +32,7 @@ for p in (os.path.join(ROOT, "p-trans_amd"), ROOT):
 from ptrans import gen, geometry  # noqa: E402
 from ptrans import post  # noqa: E402
 from ptrans.anphon import Anphon, synthetic_table  # noqa: E402
-from ptrans.constants import M_TWOPI, PREC_FP32_RETEST, PREC_FP64  # noqa: E402
+from ptrans.constants import M_TWOPI, PREC_FP32_FILTER, PREC_FP32_RETEST, PREC_FP64  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector, published
 METRIC = "phonon paths/s and κ-gradient evals/s on 1 MI355X at fixed κ std-error"
@@ -150,7 +150,7 @@ def main():
     with Context(seed=seed, device=0) as ctx:
         g = ctx.geometry(geo)
         m = ctx.modes(omega=omega)
-        for prec in (PREC_FP64, PREC_FP32_RETEST):
+        for prec in (PREC_FP64, PREC_FP32_FILTER, PREC_FP32_RETEST):
             for _ in range(args.warmup):
                 ctx.trace_grid(g, m, mfp, nangle, args.np, precision=prec)
             torch.cuda.synchronize()
@@ -167,7 +167,7 @@ def main():
             ctx.set_op_counting(False)
             res[prec] = dict(dt=dt, kernel_ms=float(np.mean(kern_ms)), counts=counts, stats=stats,
                              cstats=cstats)
-    fast, plain = res[PREC_FP32_RETEST], res[PREC_FP64]
+    fast, plain, filt = res[PREC_FP32_RETEST], res[PREC_FP64], res[PREC_FP32_FILTER]
     dt = fast["dt"]
     value = paths * args.steps / dt
     k_ms = fast["kernel_ms"]
@@ -208,7 +208,11 @@ def main():
                    "paths_per_step": paths, "events_per_path": events_per_path,
                    "transmission": transmitted},
         "kernel_ms": k_ms,
-        "precision": "FP32 intersection filter, FP64 retest and FP64 state/tallies",
+        "precision": "FP32 intersection filter and FP32 direction sampling/reflection; FP64 "
+                     "retests, hit distance, positions, free paths and tallies",
+        "fp32_filter_bitexact": {"value": paths * args.steps / filt["dt"],
+                                 "kernel_ms": filt["kernel_ms"],
+                                 "counts_identical_to_fp64": bool(np.array_equal(filt["counts"], plain["counts"]))},
         "fp64_plain": {"value": paths * args.steps / plain["dt"],
                        "ms_per_step": plain["dt"] / args.steps * 1e3,
                        "kernel_ms": plain["kernel_ms"],

@@ -62,10 +62,14 @@ extern "C" {
 
 /* arithmetic of the trace kernel */
 #define PT_PREC_FP64 0           /* every step in FP64 (the reference's arithmetic)      */
-#define PT_PREC_FP32_RETEST 1    /* FP32 intersection filter with bounded errors; the
-                                    event's surface is certified or re-tested in FP64
-                                    and its distance recomputed in FP64, so the state
-                                    evolves with FP64 numbers                           */
+#define PT_PREC_FP32_RETEST 1    /* FP32 intersection filter (bounded errors; grazing,
+                                    edge, tangent and near-tie cases re-tested in FP64),
+                                    the event's hit distance recomputed in FP64, and
+                                    direction sampling and reflection in FP32; positions,
+                                    free paths and tallies in FP64.  Statistically
+                                    equivalent to FP64, not bit-identical.              */
+#define PT_PREC_FP32_FILTER 2    /* the same FP32 filter with every other step in FP64:
+                                    bit-identical to PT_PREC_FP64                       */
 
 /*
  * One surface record: the fields of Surface_type (src/RTMCphon.f90:94-111) that the
@@ -187,6 +191,10 @@ int pt_ctx_set_event_limit(pt_ctx *ctx, int64_t limit);
 /* on != 0: later trace calls run the instrumented kernel that also fills
  * pt_stats.fp64_ops and philox_blocks (for roofline accounting; slower). */
 int pt_ctx_set_op_counting(pt_ctx *ctx, int32_t on);
+/* Launch tuning of the persistent trace kernels: paths a wave takes from the work queue
+ * at once (chunk), idle lanes that trigger a refill (refill_min, 1..64) and workgroups
+ * per CU (0: the occupancy maximum).  Results do not depend on these. */
+int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t blocks_per_cu);
 
 /* ---- geometry ------------------------------------------------------------------ */
 /* Uploads vgrain (RT:122) as flattened by the host.  Validates indices. */

@@ -63,6 +63,7 @@ struct pt_ctx {
     int blocks_per_cu = 0;
     int threads = 256;
     int chunk = 256;
+    int refill_min = 16;
     int64_t event_limit = 100000000;
     // grow-only scratch of the grid trace
     uint8_t *outcome = nullptr;
@@ -163,6 +164,24 @@ int pt_ctx_set_event_limit(pt_ctx *c, int64_t limit)
 {
     if (!c || limit < 1) return fail(PT_E_INVALID, "pt_ctx_set_event_limit: bad argument");
     c->event_limit = limit;
+    return PT_OK;
+}
+
+int pt_ctx_set_tuning(pt_ctx *c, int32_t chunk, int32_t refill_min, int32_t blocks_per_cu)
+{
+    if (!c) return fail(PT_E_INVALID, "pt_ctx_set_tuning: NULL context");
+    if (chunk < 1 || refill_min < 1 || refill_min > 64 || blocks_per_cu < 0)
+        return fail(PT_E_INVALID, "pt_ctx_set_tuning: chunk >= 1, 1 <= refill_min <= 64, "
+                                  "blocks_per_cu >= 0 (0: occupancy maximum)");
+    c->chunk = chunk;
+    c->refill_min = refill_min;
+    int occ = 1;
+    if (blocks_per_cu == 0) {
+        PT_HIP(trace_occupancy(c->threads, &occ));
+        c->blocks_per_cu = occ < 1 ? 1 : occ;
+    } else {
+        c->blocks_per_cu = blocks_per_cu;
+    }
     return PT_OK;
 }
 
@@ -514,6 +533,7 @@ TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
     P.m = M->dm;
     P.seed = c->seed;
     P.chunk = c->chunk;
+    P.refill_min = c->refill_min;
     P.event_limit = c->event_limit;
     P.precision = PT_PREC_FP64;
     P.queue = c->counters;
@@ -583,7 +603,8 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
 {
     int rc = check_trace_args(c, G, M, mfp, nangle, np);
     if (rc) return rc;
-    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (!counts) return fail(PT_E_INVALID, "counts is NULL");
     const int64_t nb = M->nb, nk = M->nk;
@@ -655,7 +676,8 @@ int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_
     int rc = check_trace_args(c, G, M, mfp, nangle, np);
     if (rc) return rc;
     if (np < 1) return fail(PT_E_INVALID, "np must be >= 1");
-    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (n < 0 || first_path < 0) return fail(PT_E_INVALID, "bad path range");
     if (n == 0) return PT_OK;
@@ -690,7 +712,8 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
     if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
     if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
-    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
     if (!mode_paths || !mode_trans) return fail(PT_E_INVALID, "tally arrays are NULL");
@@ -739,7 +762,8 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
 int pt_replay_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, const double *branch_scale,
                     int64_t first, int64_t n, int32_t precision, pt_path_record *records)
 {
-    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST)
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
     if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");

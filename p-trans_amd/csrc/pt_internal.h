@@ -78,6 +78,7 @@ struct TraceParams {
     int64_t total;             // paths of the launch
     int32_t nangle;            // grid
     int32_t chunk;             // paths a wave takes from the queue at once
+    int32_t refill_min;        // idle lanes that trigger a refill (64: only when all idle)
     int64_t event_limit;
     int32_t precision;         // PT_PREC_FP64 or PT_PREC_FP32_RETEST
     unsigned long long *queue; // next path id

@@ -297,6 +297,8 @@ __device__ int incident_collision_grain(const DevGeom &g, const double *p, OpCou
 }
 
 // State of one phonon path.
+struct Path;
+__device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *best, OpCount &oc);
 struct Path {
     double pos[3];
     double dir[3];
@@ -507,6 +509,25 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
     return true;
 }
 
+// findCollisionSurface, RT:1989-2096: box grain, then the current grain; the first
+// strictly nearest hit wins; with no hit the previous surface is kept (*best unchanged).
+__device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *best, OpCount &oc)
+{
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ig = pass == 0 ? 0 : ph.which;
+        if (pass == 1 && ig == 0) break;
+        const int j1 = g.cand_off[ig + 1];
+        for (int j = g.cand_off[ig]; j < j1; ++j) {
+            const int idx = g.cand_idx[j];
+            double l;
+            if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) && *dmin > l) {
+                *dmin = l;
+                *best = idx;
+            }
+        }
+    }
+}
+
 // determinTransmit, RT:2384-2413
 __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, double max_omega,
                                   Rng &rng, double rand_in, bool present)
@@ -535,10 +556,71 @@ __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, d
     return value <= threshold;
 }
 
+// Kernel arithmetic variants (PT_PREC_*): the FP64 trace; the FP32 filter with FP64
+// certification (bit-identical to FP64); and the fast trace, which also samples and
+// reflects directions in FP32 (hardware v_sin/v_cos/v_log/v_sqrt) while positions, free
+// paths, the certified hit distance and the tallies stay FP64.
+constexpr int kPrecFP64 = PT_PREC_FP64;
+constexpr int kPrecFast = PT_PREC_FP32_RETEST;
+constexpr int kPrecFilter = PT_PREC_FP32_FILTER;
+
+// sin and cos of 2*pi*u for u in [0, 1): V_SIN_F32/V_COS_F32 take revolutions.
+__device__ inline void fsincos_rev(float u, float *sn, float *cs)
+{
+    *sn = __builtin_amdgcn_sinf(u);
+    *cs = __builtin_amdgcn_cosf(u);
+}
+
+// -mfp * ln(u) with V_LOG_F32 (log2); u = 0 gives +inf as in the FP64 trace
+__device__ inline double free_path32(double mfp, double u)
+{
+    const float l2 = __builtin_amdgcn_logf((float)u);
+    return -(mfp * (double)(l2 * 0.69314718f));
+}
+
+// Surface normal and frame at the hit point in FP32 (calcSurfaceNormal*/Rotation*,
+// RT:3119-3301), relative to the surface centre.
+__device__ void surface_frame32(const DevGeom &g, int idx, const DSurf &s, const double *pos,
+                                bool want_rot, float *n, float *rot)
+{
+    if (s.kind == PT_SHAPE_POLYGON) {
+        n[0] = (float)s.n[0]; n[1] = (float)s.n[1]; n[2] = (float)s.n[2];
+        if (want_rot) {
+            const double *t = g.tang + 9 * (int64_t)idx;
+            for (int k = 0; k < 9; ++k) rot[k] = (float)t[k];
+        }
+        return;
+    }
+    float a[3] = {(float)(pos[0] - s.c[0]), (float)(pos[1] - s.c[1]), (float)(pos[2] - s.c[2])};
+    if (s.kind == PT_SHAPE_CYLINDER) a[2] = 0.0f;
+    const float dd = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    const float inv = __builtin_amdgcn_rsqf(dd);
+    for (int k = 0; k < 3; ++k) a[k] = a[k] * inv;
+    const float r = (float)s.r;
+    const float sg = dd > r * r ? -1.0f : 1.0f;
+    for (int k = 0; k < 3; ++k) n[k] = sg * a[k];
+    if (!want_rot) return;
+    rot[6] = a[0]; rot[7] = a[1]; rot[8] = a[2];
+    const bool axis = (s.kind == PT_SHAPE_SPHERE) ? (fabsf(a[2]) == 1.0f) : (fabsf(a[0]) == 1.0f);
+    if (axis || (a[0] * a[0] == 0.0f && a[1] * a[1] == 0.0f)) {
+        rot[3] = 0.0f; rot[4] = 1.0f; rot[5] = 0.0f;
+        rot[0] = 1.0f; rot[1] = 0.0f; rot[2] = 0.0f;
+    } else {
+        const float ir = __builtin_amdgcn_rsqf(a[0] * a[0] + a[1] * a[1]);
+        rot[3] = -a[1] * ir;
+        rot[4] = a[0] * ir;
+        rot[5] = 0.0f;
+        rot[0] = rot[4] * rot[8] - rot[5] * rot[7];
+        rot[1] = rot[5] * rot[6] - rot[3] * rot[8];
+        rot[2] = rot[3] * rot[7] - rot[4] * rot[6];
+    }
+}
+
 // One iteration of the particle loop up to the event loop (RT:1632-1690).
 // Returns false if the start-grain check fails (the reference EXITs the particle loop).
 // lambert: the alias-mode extension draws the polar angle, cos(theta) = sqrt(u),
 // after the position instead of taking the cell's fixed angle.
+template <int kPrec>
 __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool lambert, Rng &rng,
                            Path &ph)
 {
@@ -548,19 +630,33 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     ph.pos[0] = 0.0;
     ph.pos[1] = y * g.box[1] * g.scope[0];
     ph.pos[2] = z * g.box[2] * g.scope[1];
-    if (lambert) {
-        const double u = rng.next();
-        cos_t = sqrt(u);
-        sin_t = sqrt(1.0 - u);
-        oc.flops += 3;
+    if (kPrec == kPrecFast) {
+        float ct = (float)cos_t, st = (float)sin_t;
+        if (lambert) {
+            const float u = (float)rng.next();
+            ct = __builtin_amdgcn_sqrtf(u);
+            st = __builtin_amdgcn_sqrtf(1.0f - u);
+        }
+        float sp, cp;
+        fsincos_rev((float)rng.next(), &sp, &cp);                           // RT:1866
+        ph.dir[0] = ct;
+        ph.dir[1] = st * cp;
+        ph.dir[2] = st * sp;
+    } else {
+        if (lambert) {
+            const double u = rng.next();
+            cos_t = sqrt(u);
+            sin_t = sqrt(1.0 - u);
+            oc.flops += 3;
+        }
+        double phi = rng.next();                                            // RT:1866
+        phi = phi * kTwoPi;
+        double sp, cp;
+        pt_sincos(phi, &sp, &cp);
+        ph.dir[0] = cos_t;
+        ph.dir[1] = sin_t * cp;
+        ph.dir[2] = sin_t * sp;
     }
-    double phi = rng.next();                                                // RT:1866
-    phi = phi * kTwoPi;
-    double sp, cp;
-    pt_sincos(phi, &sp, &cp);
-    ph.dir[0] = cos_t;
-    ph.dir[1] = sin_t * cp;
-    ph.dir[2] = sin_t * sp;
     ph.end_side = (ph.dir[0] <= 0.0) ? 1 : 0;                               // RT:1644-1652
     ph.cur = 0;                                                             // igrain=0, isurface=1
     ph.flags = 0;
@@ -577,12 +673,13 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
         return false;
     }
     if (ph.mfp == kBallistic) ph.fp = kBallistic;                           // RT:1686-1690
+    else if (kPrec == kPrecFast) ph.fp = free_path32(ph.mfp, rng.next());
     else { ph.fp = -(ph.mfp * pt_log(rng.next())); oc.flops += 1 + kFlopsLog; } // RT:2171-2172
     return true;
 }
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
-template <bool kFast>
+template <int kPrec>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
                           Path &ph)
 {
@@ -606,7 +703,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     double dmin = 1.0e10;
     int best = ph.cur;
     bool decided = false;
-    if (kFast) {
+    if (kPrec != kPrecFP64) {
         decided = scan_fast(g, ph, &dmin, &best, oc);
         if (!decided) {
             dmin = 1.0e10;
@@ -615,19 +712,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         }
     }
     if (!decided) {
-        for (int pass = 0; pass < 2; ++pass) {
-            const int ig = pass == 0 ? 0 : ph.which;
-            if (pass == 1 && ig == 0) break;
-            const int j1 = g.cand_off[ig + 1];
-            for (int j = g.cand_off[ig]; j < j1; ++j) {
-                const int idx = g.cand_idx[j];
-                double l;
-                if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) && dmin > l) {
-                    dmin = l;
-                    best = idx;
-                }
-            }
-        }
+        scan_full(g, ph, &dmin, &best, oc);
     }
     double distance = dmin;
     ph.cur = best;
@@ -638,14 +723,22 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         distance = ph.fp;
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
-        double ct = rng.next();                                             // RT:2197
-        double ph_ = rng.next();                                            // RT:2198
-        ct = ct * 2.0 - 1.0;
-        ph_ = ph_ * kTwoPi;
-        const double st = sqrt(1.0 - ct * ct);
-        double sph, cph;
-        pt_sincos(ph_, &sph, &cph);
-        ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
+        if (kPrec == kPrecFast) {
+            const float ct = (float)rng.next() * 2.0f - 1.0f;
+            float sph, cph;
+            fsincos_rev((float)rng.next(), &sph, &cph);
+            const float st = __builtin_amdgcn_sqrtf(1.0f - ct * ct);
+            ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
+        } else {
+            double ct = rng.next();                                         // RT:2197
+            double ph_ = rng.next();                                        // RT:2198
+            ct = ct * 2.0 - 1.0;
+            ph_ = ph_ * kTwoPi;
+            const double st = sqrt(1.0 - ct * ct);
+            double sph, cph;
+            pt_sincos(ph_, &sph, &cph);
+            ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
+        }
         ph.cur = -1;
         oc.flops += 8 + kFlopsSincos;    // + 6 above
     } else {
@@ -666,6 +759,12 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
                 // reflectSpecular, RT:2275-2328
                 if (determin_transmit(s, g.nmaterial, ph.omega, max_omega, rng, 0.0, false)) {
                     jreflect = 0;
+                } else if (kPrec == kPrecFast) {
+                    float n[3], d[3] = {(float)ph.dir[0], (float)ph.dir[1], (float)ph.dir[2]};
+                    surface_frame32(g, ph.cur, s, ph.pos, false, n, nullptr);
+                    float dd = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+                    if (dd < 0.0f) { for (int k = 0; k < 3; ++k) n[k] = -n[k]; dd = -dd; }
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = d[k] - 2.0f * n[k] * dd;
                 } else {
                     double n[3];
                     surface_normal(s, ph.pos, n, oc);
@@ -674,6 +773,29 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
                     const double dd = dot3(n, ph.dir);
                     for (int k = 0; k < 3; ++k) ph.dir[k] = ph.dir[k] - 2.0 * n[k] * dd;
                     oc.flops += 19;
+                }
+            } else if (kPrec == kPrecFast) {
+                // reflectDiffusive (RT:2332-2380) in FP32
+                float n[3], rot[9], d[3] = {(float)ph.dir[0], (float)ph.dir[1], (float)ph.dir[2]};
+                surface_frame32(g, ph.cur, s, ph.pos, true, n, rot);
+                if (n[0] * d[0] + n[1] * d[1] + n[2] * d[2] < 0.0f)
+                    for (int k = 0; k < 3; ++k) n[k] = -n[k];
+                const float st = __builtin_amdgcn_sqrtf((float)rng.next());  // RT:2213
+                const float ct = __builtin_amdgcn_sqrtf(1.0f - st * st);
+                float sph, cph;
+                fsincos_rev((float)rng.next(), &sph, &cph);                 // RT:2218
+                const float loc[3] = {st * cph, st * sph, ct};
+                float md[3];
+                for (int k = 0; k < 3; ++k)
+                    md[k] = rot[k] * loc[0] + rot[3 + k] * loc[1] + rot[6 + k] * loc[2];
+                const float sg1 = (d[0] * n[0] + d[1] * n[1] + d[2] * n[2] > 0.0f) ? -1.0f : 1.0f;
+                for (int k = 0; k < 3; ++k) md[k] = sg1 * md[k];
+                const float sg2 = (md[0] * n[0] + md[1] * n[1] + md[2] * n[2] > 0.0f) ? -1.0f : 1.0f;
+                for (int k = 0; k < 3; ++k) ph.dir[k] = sg2 * md[k];
+                const double u = rng.next();                                // RT:2361
+                if (determin_transmit(s, g.nmaterial, ph.omega, max_omega, rng, u, true)) {
+                    for (int k = 0; k < 3; ++k) ph.dir[k] = -ph.dir[k];
+                    jreflect = 0;
                 }
             } else {
                 // reflectDiffusive, RT:2332-2380
@@ -718,8 +840,12 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         ph.fp = ph.fp - distance;
         oc.flops += 1;
         if (ph.fp < 1.0e-10) {
-            ph.fp = -(ph.mfp * pt_log(rng.next()));                          // RT:2171
-            oc.flops += 1 + kFlopsLog;
+            if (kPrec == kPrecFast) {
+                ph.fp = free_path32(ph.mfp, rng.next());
+            } else {
+                ph.fp = -(ph.mfp * pt_log(rng.next()));                      // RT:2171
+                oc.flops += 1 + kFlopsLog;
+            }
         }
     }
     // whichV on transmission, RT:1793-1796
@@ -768,7 +894,7 @@ __device__ inline int sample_mode(const DevModes &m, Rng &rng)
 
 // Sets up path pid: grid mode (cell of the reference's collapsed loop) or alias mode
 // (sampled mode, Lambertian incidence).  Returns start_path's verdict.
-template <int kMode>
+template <int kMode, int kPrec>
 __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, Path &ph, int *mode)
 {
     rng.path = (uint64_t)pid;
@@ -779,14 +905,14 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
         ph.omega = cell_omega(P.m, ib, ik);
         ph.mfp = P.mfp;
         *mode = (ib - 1) + P.m.nb * (ik - 1);
-        return start_path(P.g, P.cos_t[ia - 1], P.sin_t[ia - 1], false, rng, ph);
+        return start_path<kPrec>(P.g, P.cos_t[ia - 1], P.sin_t[ia - 1], false, rng, ph);
     } else {
         const int m = sample_mode(P.m, rng);
         const int ib = m % P.m.nb;
         ph.omega = P.m.omega[m];
         ph.mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[ib] : P.m.mfp[m];
         *mode = m;
-        return start_path(P.g, 0.0, 0.0, true, rng, ph);
+        return start_path<kPrec>(P.g, 0.0, 0.0, true, rng, ph);
     }
 }
 
@@ -796,7 +922,7 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
 // refills lanes whose path has ended (ballot + prefix count); lanes then advance one event
 // per iteration.  Grid mode writes one outcome byte per path (reduced per cell later);
 // alias mode adds to per-mode integer tallies (exact, order independent).
-template <int kMode, bool kCount, bool kFast>
+template <int kMode, bool kCount, int kPrec>
 __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
 {
     const int lane = threadIdx.x & 63;
@@ -815,7 +941,8 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
 
     while (true) {
         const unsigned long long need = __ballot(!active);
-        if (need != 0ull && !exhausted) {
+        // refill in batches: start_path is long, so issue it for many lanes at once
+        if (need != 0ull && !exhausted && (__popcll(need) >= P.refill_min || need == ~0ull)) {
             if (chunk_next >= chunk_end) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(P.queue, (unsigned long long)P.chunk);
@@ -830,7 +957,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
                 const int nneed = __popcll(need);
                 if (!active && rank < avail) {
                     pid = chunk_next + rank;
-                    if (begin_path<kMode>(P, pid, rng, ph, &mode)) {
+                    if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
                         active = true;
                     } else {
                         if (kMode == TRACE_GRID) P.outcome[pid] = OUT_MISMATCH;
@@ -847,7 +974,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
             continue;
         }
         if (active) {
-            if (ph.end_side != 0 || step_path<kFast>(P.g, P.m.max_omega, P.event_limit, rng, ph)) {
+            if (ph.end_side != 0 || step_path<kPrec>(P.g, P.m.max_omega, P.event_limit, rng, ph)) {
                 const bool transmitted = ph.end_side != 1;          // RT:1833
                 if (kMode == TRACE_GRID) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
@@ -936,7 +1063,7 @@ __global__ void __launch_bounds__(256) reduce_grid_kernel(const uint8_t *__restr
 }
 
 // Single paths with full records (parity checks), one lane per path.
-template <int kMode, bool kFast>
+template <int kMode, int kPrec>
 __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first, int64_t n,
                                                     pt_path_record *rec)
 {
@@ -948,9 +1075,9 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
     rng.seed = P.seed;
     Path ph;
     int mode = 0;
-    if (begin_path<kMode>(P, first + i, rng, ph, &mode)) {
+    if (begin_path<kMode, kPrec>(P, first + i, rng, ph, &mode)) {
         while (ph.end_side == 0) {
-            if (step_path<kFast>(P.g, P.m.max_omega, P.event_limit, rng, ph)) break;
+            if (step_path<kPrec>(P.g, P.m.max_omega, P.event_limit, rng, ph)) break;
         }
     }
     pt_path_record r;
@@ -968,15 +1095,18 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
                         hipStream_t s)
 {
     const dim3 gr(blocks), bl(threads);
-    const bool fast = p.precision == PT_PREC_FP32_RETEST;
-#define PT_LAUNCH(M, C, F) hipLaunchKernelGGL((trace_kernel<M, C, F>), gr, bl, 0, s, p)
-    if (mode == TRACE_GRID) {
-        if (fast) { if (count) PT_LAUNCH(TRACE_GRID, true, true); else PT_LAUNCH(TRACE_GRID, false, true); }
-        else { if (count) PT_LAUNCH(TRACE_GRID, true, false); else PT_LAUNCH(TRACE_GRID, false, false); }
-    } else {
-        if (fast) { if (count) PT_LAUNCH(TRACE_ALIAS, true, true); else PT_LAUNCH(TRACE_ALIAS, false, true); }
-        else { if (count) PT_LAUNCH(TRACE_ALIAS, true, false); else PT_LAUNCH(TRACE_ALIAS, false, false); }
-    }
+#define PT_LAUNCH(M, C, R) hipLaunchKernelGGL((trace_kernel<M, C, R>), gr, bl, 0, s, p)
+#define PT_LAUNCH_C(M, R) do { if (count) PT_LAUNCH(M, true, R); else PT_LAUNCH(M, false, R); } while (0)
+#define PT_LAUNCH_R(M)                                                    \
+    do {                                                                  \
+        if (p.precision == kPrecFast) PT_LAUNCH_C(M, kPrecFast);          \
+        else if (p.precision == kPrecFilter) PT_LAUNCH_C(M, kPrecFilter); \
+        else PT_LAUNCH_C(M, kPrecFP64);                                   \
+    } while (0)
+    if (mode == TRACE_GRID) PT_LAUNCH_R(TRACE_GRID);
+    else PT_LAUNCH_R(TRACE_ALIAS);
+#undef PT_LAUNCH_R
+#undef PT_LAUNCH_C
 #undef PT_LAUNCH
     return hipGetLastError();
 }
@@ -984,7 +1114,7 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
 hipError_t trace_occupancy(int threads, int *blocks_per_cu)
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, trace_kernel<TRACE_GRID, false, false>, threads, 0);
+        blocks_per_cu, trace_kernel<TRACE_GRID, false, kPrecFilter>, threads, 0);
 }
 
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,
@@ -1001,15 +1131,18 @@ hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t 
                          pt_path_record *rec, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)((n + 63) / 64);
-    const bool fast = p.precision == PT_PREC_FP32_RETEST;
-    if (mode == TRACE_GRID) {
-        if (fast) hipLaunchKernelGGL((replay_kernel<TRACE_GRID, true>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
-        else hipLaunchKernelGGL((replay_kernel<TRACE_GRID, false>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
-    } else {
-        if (fast) hipLaunchKernelGGL((replay_kernel<TRACE_ALIAS, true>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
-        else hipLaunchKernelGGL((replay_kernel<TRACE_ALIAS, false>), dim3(blocks), dim3(64), 0, s, p, first, n, rec);
-    }
+    const dim3 gr((unsigned)((n + 63) / 64)), bl(64);
+#define PT_REPLAY(M, R) hipLaunchKernelGGL((replay_kernel<M, R>), gr, bl, 0, s, p, first, n, rec)
+#define PT_REPLAY_R(M)                                                   \
+    do {                                                                 \
+        if (p.precision == kPrecFast) PT_REPLAY(M, kPrecFast);           \
+        else if (p.precision == kPrecFilter) PT_REPLAY(M, kPrecFilter);  \
+        else PT_REPLAY(M, kPrecFP64);                                    \
+    } while (0)
+    if (mode == TRACE_GRID) PT_REPLAY_R(TRACE_GRID);
+    else PT_REPLAY_R(TRACE_ALIAS);
+#undef PT_REPLAY_R
+#undef PT_REPLAY
     return hipGetLastError();
 }
 

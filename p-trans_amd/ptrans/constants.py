@@ -24,6 +24,7 @@ PT_E_NODEVICE = -6
 
 PREC_FP64 = 0
 PREC_FP32_RETEST = 1
+PREC_FP32_FILTER = 2
 
 PATH_LOST = 1
 PATH_START_MISMATCH = 2

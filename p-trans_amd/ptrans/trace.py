@@ -83,6 +83,9 @@ class Context:
     def set_event_limit(self, limit: int):
         _lib.check(_lib.load().pt_ctx_set_event_limit(self.handle, limit))
 
+    def set_tuning(self, chunk: int = 256, refill_min: int = 16, blocks_per_cu: int = 0):
+        _lib.check(_lib.load().pt_ctx_set_tuning(self.handle, chunk, refill_min, blocks_per_cu))
+
     def set_op_counting(self, on: bool):
         _lib.check(_lib.load().pt_ctx_set_op_counting(self.handle, int(bool(on))))
 

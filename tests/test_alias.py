@@ -119,7 +119,7 @@ def test_gpu_alias_paths_bitexact_vs_oracle(gpu_ctx, name):
     gp, gt, st = gpu_ctx.trace_alias(g, m, 400, branch_scale=scale)
     np.testing.assert_array_equal(gp, mp)
     np.testing.assert_array_equal(gt, mt)
-    fast = gpu_ctx.replay_alias(g, m, 0, 400, branch_scale=scale, precision=1)
+    fast = gpu_ctx.replay_alias(g, m, 0, 400, branch_scale=scale, precision=2)
     for f in ("end_side", "events", "draws"):
         np.testing.assert_array_equal(fast[f], orec[f], err_msg="fast " + f)
     np.testing.assert_array_equal(fast["pos"].view(np.uint64), orec["pos"].view(np.uint64))

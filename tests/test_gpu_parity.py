@@ -110,7 +110,8 @@ def test_results_independent_of_run(gpu_ctx):
 
 # ---------------------------------------------------------------- FP32 filter + FP64 retest
 
-FAST = 1
+FAST = 2       # PT_PREC_FP32_FILTER: FP32 intersection filter, every other step FP64
+FP32 = 1       # PT_PREC_FP32_RETEST: FP32 filter and FP32 direction sampling/reflection
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -150,3 +151,40 @@ def test_fast_grid_counts_identical_to_fp64_large(gpu_ctx, builder):
     np.testing.assert_array_equal(a, b)
     assert sa["events"] == sb["events"] and sa["draws"] == sb["draws"]
     assert sb["fallbacks"] < 0.01 * sb["events"]
+
+
+@pytest.mark.parametrize("builder", ["film", "inclusions", "voronoi"])
+def test_fp32_trace_statistically_equals_fp64(gpu_ctx, builder):
+    """PT_PREC_FP32_RETEST reflects and samples directions in FP32, so paths drift from
+    the FP64 ones by rounding; with common random numbers most paths still end on the
+    same side, and the transmitted total agrees within 3 sigma of the paired difference."""
+    if builder == "film":
+        model = gen.film_holes(seed=2, nphonon=20000, nangle=11, mfp=(300.0,))
+    elif builder == "inclusions":
+        model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.4, nphonon=20000, nangle=11,
+                               mfp=(500.0,))
+    else:
+        model = gen.voronoi_cube(seed=4, kind="Gamma", value=1.5, p=0.2, nphonon=20000, nangle=11)
+    geo = geometry.build_geometry(model)
+    om = np.linspace(1e12, 9e13, 6).reshape(2, 3)
+    mfp = model.mfp[0] if model.mfp else 1.0e10
+    gpu_ctx.set_seed(37)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    a = gpu_ctx.replay(g, m, mfp, 11, 20000, 0, 20000)
+    b = gpu_ctx.replay(g, m, mfp, 11, 20000, 0, 20000, precision=FP32)
+    ta = (a["end_side"] != 1)
+    tb = (b["end_side"] != 1)
+    agree = (a["end_side"] == b["end_side"]).mean()
+    print(builder, "per-path exit agreement", agree, flush=True)
+    assert agree > 0.99
+    ca, sa = gpu_ctx.trace_grid(g, m, mfp, 11, 20000)
+    cb, sb = gpu_ctx.trace_grid(g, m, mfp, 11, 20000, precision=FP32)
+    n = sa["paths"]
+    diff = int(cb.sum()) - int(ca.sum())
+    d = (tb.astype(int) - ta.astype(int))
+    sig = np.sqrt(max(d.var(), 1.0 / len(d)) * n)
+    print(builder, "transmitted fp64", ca.sum(), "fp32", cb.sum(), "diff", diff, "3 sigma", 3 * sig,
+          "fp64 ms", sa["kernel_ms"], "fp32 ms", sb["kernel_ms"], flush=True)
+    assert abs(diff) <= 3 * sig + 1
+    assert abs(diff) / max(1, ca.sum()) < 0.005
