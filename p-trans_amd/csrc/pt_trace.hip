@@ -74,14 +74,21 @@ __device__ inline void cross3(const double *a, const double *b, double *c)
 
 // calcSurfaceDistancePolygon, RT:2907-2959: one-sided plane hit, then the convex
 // point-in-polygon test with on-edge counted inside.
-__device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
-                             const double *nd, double *lhit, OpCount &oc)
+// Geometry read through the constant address space: with a wave-uniform index the
+// loads become scalar (SMEM) loads into SGPRs.
+typedef const __attribute__((address_space(4))) double cdouble;
+typedef const __attribute__((address_space(4))) DSurf CSurf;
+typedef const __attribute__((address_space(4))) int32_t cint;
+
+template <class VT>
+__device__ bool dist_polygon_v(VT *vtx, const DSurf &s, const double *rs, const double *nd,
+                               double *lhit, OpCount &oc)
 {
     const double cosang = nd[0] * s.n[0] + nd[1] * s.n[1] + nd[2] * s.n[2];
     oc.flops += 5;
     if (cosang <= 0.0) return false;
     oc.flops += 9;
-    const double *v = g.vtx + 3 * (int64_t)s.voff;
+    VT *v = vtx + 3 * (int64_t)s.voff;
     double d[3] = {v[0] - rs[0], v[1] - rs[1], v[2] - rs[2]};
     const double l = (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]) / cosang;
     if (l < 0.0) return false;
@@ -90,8 +97,8 @@ __device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
     oc.flops += 6;
     double c1[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < s.nv; ++i) {
-        const double *vi = v + 3 * i;
-        const double *vn = v + 3 * (i + 1 == s.nv ? 0 : i + 1);
+        VT *vi = v + 3 * i;
+        VT *vn = v + 3 * (i + 1 == s.nv ? 0 : i + 1);
         double ra[3], rb[3], ci[3];
         for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = h[k] - vi[k]; }
         cross3(ra, rb, ci);
@@ -104,6 +111,12 @@ __device__ bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
     }
     *lhit = l;
     return true;
+}
+
+__device__ inline bool dist_polygon(const DevGeom &g, const DSurf &s, const double *rs,
+                                    const double *nd, double *lhit, OpCount &oc)
+{
+    return dist_polygon_v(g.vtx, s, rs, nd, lhit, oc);
 }
 
 // isInsidePolygon, RT:2961-2983
@@ -177,6 +190,18 @@ __device__ bool dist_cylinder(const DSurf &s, const double *rs, const double *nd
     if (zh < 0.0 || boxz < zh) return false;
     *lhit = l;
     return true;
+}
+
+// surface distance with the vertex pool read through the constant address space
+__device__ inline bool surface_distance_c(const DevGeom &g, const DSurf &s, const double *p,
+                                          const double *d, double *l, OpCount &oc)
+{
+    switch (s.kind) {
+    case PT_SHAPE_POLYGON: return dist_polygon_v((cdouble *)g.vtx, s, p, d, l, oc);
+    case PT_SHAPE_SPHERE: return dist_sphere(s, p, d, l, oc);
+    case PT_SHAPE_CYLINDER: return dist_cylinder(s, p, d, g.box[2], l, oc);
+    }
+    return false;
 }
 
 __device__ inline bool surface_distance(const DevGeom &g, const DSurf &s, const double *p,
@@ -454,47 +479,74 @@ struct Pick {
     }
 };
 
+// DSurf record through a scalar (constant address space) load.
+__device__ inline DSurf load_surf_c(const DSurf *base, int idx)
+{
+    CSurf *p = (CSurf *)base + idx;
+    DSurf o;
+    for (int k = 0; k < 3; ++k) { o.n[k] = p->n[k]; o.c[k] = p->c[k]; }
+    o.r = p->r;
+    o.p = p->p;
+    o.tparam = p->tparam;
+    o.kind = p->kind;
+    o.nv = p->nv;
+    o.voff = p->voff;
+    o.gout = p->gout;
+    o.tkind = p->tkind;
+    return o;
+}
+
 // The fast path's findCollisionSurface.  Returns false when the FP64 full scan must
 // decide; else sets *dmin and *best exactly as the reference's scan would.
+// The current grain's candidates are scanned group by group ("waterfall"): lanes that
+// share a grain run its candidate list together with wave-uniform indices, so the
+// surface records come in through scalar loads.
 __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, OpCount &oc)
 {
     Pick pk;
     pk.init();
     const double *rs = ph.pos, *nd = ph.dir;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int ig = pass == 0 ? 0 : ph.which;
-        if (pass == 1 && ig == 0) break;
-        if (ig == 0 && g.box_aligned) {
-            for (int a = 0; a < 3; ++a) {
-                int w;
-                float lo, hi;
-                const int r = filt_box_axis(g, a, rs, nd, &w, &lo, &hi);
-                oc.flops += 10;
-                if (r == F_HIT) {
-                    pk.add(w, lo, hi, false);
-                } else if (r == F_UNSURE) {
-                    double l;
-                    ph.retests++;
-                    if (surface_distance(g, g.surf[w], rs, nd, &l, oc)) pk.add(w, l, l, true);
-                }
-            }
-            continue;
-        }
-        const int j1 = g.cand_off[ig + 1];
-        for (int j = g.cand_off[ig]; j < j1; ++j) {
-            const int idx = g.cand_idx[j];
-            const DSurf &s = g.surf[idx];
-            float lo = 0.f, hi = 0.f;
-            int r;
-            if (s.kind == PT_SHAPE_CYLINDER) { r = filt_cylinder(s, rs, nd, g.box[2], &lo, &hi); oc.flops += 30; }
-            else if (s.kind == PT_SHAPE_SPHERE) { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
-            else r = F_UNSURE;                                // polygons: FP64 routine
+    bool todo0 = true, todo1 = ph.which != 0;
+    if (g.box_aligned) {
+        todo0 = false;
+        for (int a = 0; a < 3; ++a) {
+            int w;
+            float lo, hi;
+            const int r = filt_box_axis(g, a, rs, nd, &w, &lo, &hi);
+            oc.flops += 10;
             if (r == F_HIT) {
-                pk.add(idx, lo, hi, false);
+                pk.add(w, lo, hi, false);
             } else if (r == F_UNSURE) {
                 double l;
-                if (s.kind != PT_SHAPE_POLYGON) ph.retests++;
-                if (surface_distance(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
+                ph.retests++;
+                if (surface_distance(g, g.surf[w], rs, nd, &l, oc)) pk.add(w, l, l, true);
+            }
+        }
+    }
+    while (__any(todo0 || todo1)) {
+        if (todo0 || todo1) {
+            const int mine = todo0 ? 0 : ph.which;
+            const int ig = __builtin_amdgcn_readfirstlane(mine);
+            if (mine == ig) {
+                const int j0 = ((cint *)g.cand_off)[ig], j1 = ((cint *)g.cand_off)[ig + 1];
+                for (int j = j0; j < j1; ++j) {
+                    const int idx = ((cint *)g.cand_idx)[j];
+                    const DSurf s = load_surf_c(g.surf, idx);
+                    float lo = 0.f, hi = 0.f;
+                    int r;
+                    if (s.kind == PT_SHAPE_CYLINDER) { r = filt_cylinder(s, rs, nd, g.box[2], &lo, &hi); oc.flops += 30; }
+                    else if (s.kind == PT_SHAPE_SPHERE) { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
+                    else r = F_UNSURE;                        // polygons: FP64 routine
+                    if (r == F_HIT) {
+                        pk.add(idx, lo, hi, false);
+                    } else if (r == F_UNSURE) {
+                        double l;
+                        if (s.kind != PT_SHAPE_POLYGON) ph.retests++;
+                        if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
+                    }
+                }
+                if (todo0) todo0 = false;
+                else todo1 = false;
             }
         }
     }
@@ -511,18 +563,28 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
 
 // findCollisionSurface, RT:1989-2096: box grain, then the current grain; the first
 // strictly nearest hit wins; with no hit the previous surface is kept (*best unchanged).
+// Each grain's candidate list is run by the lanes in that grain together (waterfall over
+// the distinct grains of the wave), with wave-uniform, scalar-loaded records.
 __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *best, OpCount &oc)
 {
-    for (int pass = 0; pass < 2; ++pass) {
-        const int ig = pass == 0 ? 0 : ph.which;
-        if (pass == 1 && ig == 0) break;
-        const int j1 = g.cand_off[ig + 1];
-        for (int j = g.cand_off[ig]; j < j1; ++j) {
-            const int idx = g.cand_idx[j];
-            double l;
-            if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) && *dmin > l) {
-                *dmin = l;
-                *best = idx;
+    bool todo0 = true, todo1 = ph.which != 0;
+    while (__any(todo0 || todo1)) {
+        if (todo0 || todo1) {
+            const int mine = todo0 ? 0 : ph.which;
+            const int ig = __builtin_amdgcn_readfirstlane(mine);
+            if (mine == ig) {
+                const int j0 = ((cint *)g.cand_off)[ig], j1 = ((cint *)g.cand_off)[ig + 1];
+                for (int j = j0; j < j1; ++j) {
+                    const int idx = ((cint *)g.cand_idx)[j];
+                    const DSurf s = load_surf_c(g.surf, idx);
+                    double l;
+                    if (surface_distance_c(g, s, ph.pos, ph.dir, &l, oc) && *dmin > l) {
+                        *dmin = l;
+                        *best = idx;
+                    }
+                }
+                if (todo0) todo0 = false;
+                else todo1 = false;
             }
         }
     }
@@ -569,6 +631,19 @@ __device__ inline void fsincos_rev(float u, float *sn, float *cs)
 {
     *sn = __builtin_amdgcn_sinf(u);
     *cs = __builtin_amdgcn_cosf(u);
+}
+
+// Directions made in FP32 have |d| = 1 only to ~1e-7.  The sphere distance
+// (RT:2985-3038) assumes |d| = 1 and its error grows as l^2 / sqrt(D): at l ~ 700 nm a
+// 4e-8 error in |d|^2 puts the hit point ~5e-4 nm inside the sphere, far beyond the
+// 1e-7 nm nudge, and the phonon is trapped there.  Scaling by the second-order series
+// of (1 + e)^(-1/2), e = |d|^2 - 1 (exact by Sterbenz), leaves |d|^2 - 1 = O(e^3), i.e.
+// |d| = 1 to FP64 rounding.
+__device__ inline void renorm_dir(double *d)
+{
+    const double e = (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - 1.0;
+    const double f = 1.0 + e * (-0.5 + 0.375 * e);
+    for (int k = 0; k < 3; ++k) d[k] = d[k] * f;
 }
 
 // -mfp * ln(u) with V_LOG_F32 (log2); u = 0 gives +inf as in the FP64 trace
@@ -642,6 +717,7 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
         ph.dir[0] = ct;
         ph.dir[1] = st * cp;
         ph.dir[2] = st * sp;
+        renorm_dir(ph.dir);
     } else {
         if (lambert) {
             const double u = rng.next();
@@ -729,6 +805,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             fsincos_rev((float)rng.next(), &sph, &cph);
             const float st = __builtin_amdgcn_sqrtf(1.0f - ct * ct);
             ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
+            renorm_dir(ph.dir);
         } else {
             double ct = rng.next();                                         // RT:2197
             double ph_ = rng.next();                                        // RT:2198
@@ -765,6 +842,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
                     float dd = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
                     if (dd < 0.0f) { for (int k = 0; k < 3; ++k) n[k] = -n[k]; dd = -dd; }
                     for (int k = 0; k < 3; ++k) ph.dir[k] = d[k] - 2.0f * n[k] * dd;
+                    renorm_dir(ph.dir);
                 } else {
                     double n[3];
                     surface_normal(s, ph.pos, n, oc);
@@ -792,6 +870,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
                 for (int k = 0; k < 3; ++k) md[k] = sg1 * md[k];
                 const float sg2 = (md[0] * n[0] + md[1] * n[1] + md[2] * n[2] > 0.0f) ? -1.0f : 1.0f;
                 for (int k = 0; k < 3; ++k) ph.dir[k] = sg2 * md[k];
+                renorm_dir(ph.dir);
                 const double u = rng.next();                                // RT:2361
                 if (determin_transmit(s, g.nmaterial, ph.omega, max_omega, rng, u, true)) {
                     for (int k = 0; k < 3; ++k) ph.dir[k] = -ph.dir[k];
