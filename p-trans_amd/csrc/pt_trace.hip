@@ -496,11 +496,30 @@ __device__ inline DSurf load_surf_c(const DSurf *base, int idx)
     return o;
 }
 
+// FP64 distance to box wall w of an axis-aligned box.  With n = +-e_a and v1[a] = P
+// (both checked when the geometry is created) every other term of the reference's
+// ((v1 - p).n) / (d.n) (RT:2928-2934) is a signed zero, so the quotient is exactly
+// (P - p[a]) / d[a].  Valid where the FP32 filter certified the hit.
+__device__ inline double box_wall_distance(const DevGeom &g, int w, const double *rs,
+                                           const double *nd)
+{
+    // selects, not indexing: a dynamic index would move the path state to scratch
+    const int a = w >> 1;
+    const double b = a == 0 ? g.box[0] : (a == 1 ? g.box[1] : g.box[2]);
+    const double r = a == 0 ? rs[0] : (a == 1 ? rs[1] : rs[2]);
+    const double d = a == 0 ? nd[0] : (a == 1 ? nd[1] : nd[2]);
+    const double P = (w & 1) ? b : 0.0;
+    return (P - r) / d;
+}
+
 // The fast path's findCollisionSurface.  Returns false when the FP64 full scan must
 // decide; else sets *dmin and *best exactly as the reference's scan would.
 // The current grain's candidates are scanned group by group ("waterfall"): lanes that
 // share a grain run its candidate list together with wave-uniform indices, so the
-// surface records come in through scalar loads.
+// surface records come in through scalar loads.  Curved candidates the FP32 filter
+// cannot settle (mostly the surface the phonon has just left) are queued and settled
+// afterwards by one FP64 evaluation site shared by all lanes, instead of one FP64 site
+// per candidate of the list.
 __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, OpCount &oc)
 {
     Pick pk;
@@ -523,14 +542,16 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
             }
         }
     }
+    int q0 = -1, q1 = -1;                                 // queued unsure curved candidates
     while (__any(todo0 || todo1)) {
         if (todo0 || todo1) {
             const int mine = todo0 ? 0 : ph.which;
             const int ig = __builtin_amdgcn_readfirstlane(mine);
             if (mine == ig) {
-                const int j0 = ((cint *)g.cand_off)[ig], j1 = ((cint *)g.cand_off)[ig + 1];
+                const int j0 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig]);
+                const int j1 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig + 1]);
                 for (int j = j0; j < j1; ++j) {
-                    const int idx = ((cint *)g.cand_idx)[j];
+                    const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
                     const DSurf s = load_surf_c(g.surf, idx);
                     float lo = 0.f, hi = 0.f;
                     int r;
@@ -540,9 +561,15 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                     if (r == F_HIT) {
                         pk.add(idx, lo, hi, false);
                     } else if (r == F_UNSURE) {
-                        double l;
-                        if (s.kind != PT_SHAPE_POLYGON) ph.retests++;
-                        if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
+                        if (s.kind != PT_SHAPE_POLYGON && q0 < 0) {
+                            q0 = idx;
+                        } else if (s.kind != PT_SHAPE_POLYGON && q1 < 0) {
+                            q1 = idx;
+                        } else {
+                            double l;
+                            if (s.kind != PT_SHAPE_POLYGON) ph.retests++;
+                            if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
+                        }
                     }
                 }
                 if (todo0) todo0 = false;
@@ -550,10 +577,24 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
             }
         }
     }
+    while (__any(q0 >= 0)) {
+        if (q0 >= 0) {
+            double l;
+            ph.retests++;
+            if (surface_distance(g, g.surf[q0], rs, nd, &l, oc)) pk.add(q0, l, l, true);
+            q0 = q1;
+            q1 = -1;
+        }
+    }
     if (pk.idx_hi < 0 || pk.idx_lo1 != pk.idx_hi || !(pk.lo2 > pk.hi_min)) return false;
     double l = pk.exact_l;
     if (!pk.exact) {
-        if (!surface_distance(g, g.surf[pk.idx_hi], rs, nd, &l, oc)) return false;
+        if (g.box_aligned && pk.idx_hi < 6) {
+            l = box_wall_distance(g, pk.idx_hi, rs, nd);
+            oc.flops += 2;
+        } else if (!surface_distance(g, g.surf[pk.idx_hi], rs, nd, &l, oc)) {
+            return false;
+        }
     }
     if (!(1.0e10 > l)) return false;                      // reference: dmin starts at 1e10
     *dmin = l;
@@ -573,9 +614,10 @@ __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *b
             const int mine = todo0 ? 0 : ph.which;
             const int ig = __builtin_amdgcn_readfirstlane(mine);
             if (mine == ig) {
-                const int j0 = ((cint *)g.cand_off)[ig], j1 = ((cint *)g.cand_off)[ig + 1];
+                const int j0 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig]);
+                const int j1 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig + 1]);
                 for (int j = j0; j < j1; ++j) {
-                    const int idx = ((cint *)g.cand_idx)[j];
+                    const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
                     const DSurf s = load_surf_c(g.surf, idx);
                     double l;
                     if (surface_distance_c(g, s, ph.pos, ph.dir, &l, oc) && *dmin > l) {
