@@ -201,6 +201,12 @@ int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t bl
 int pt_geom_create(pt_ctx *ctx, const pt_geom_desc *desc, pt_geom **out);
 int pt_geom_destroy(pt_geom *geom);
 
+/* Sets surface%specularity of every surface (nsurface values in [0, 1], descriptor
+ * order) without rebuilding the geometry: the parameters of the specularity gradient.
+ * The reference draws u <= specularity at each reflection (determinSpecular,
+ * RT:2225-2235); its values come from param%specularity (RT:1150, 1388). */
+int pt_geom_set_specularity(pt_geom *geom, const double *specularity);
+
 /*
  * setupSurfaceGeometry for a polygon (src/RTMCphon.f90:1433-1491): centroid, the
  * averaged (not renormalised) fan normal and the tangent frame, from nvertex >= 3
@@ -263,6 +269,31 @@ int pt_trace_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, int6
 int pt_replay_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
                     const double *branch_scale, int64_t first, int64_t n, int32_t precision,
                     pt_path_record *records);
+
+/*
+ * Adjoint (an extension; the reference has no derivatives): the alias-mode trace of
+ * pt_trace_alias plus likelihood-ratio scores, in one pass with nothing stored per path.
+ * With x = T / mfp_m for a path of mode m (T = 1 if transmitted), kappa = c * E[x] with
+ * c = kappa_bulk * 3L/4, and for each parameter theta
+ *     d kappa / d theta = c * E[(x - baseline) * d log P(path) / d theta],
+ * P(path) the path's probability: every specular decision on a surface of group g
+ * contributes 1/p (specular) or -1/(1-p) (diffuse), and the free flights of branch b
+ * (rate branch_scale[b] / mfp_m) contribute n_scat / branch_scale[b] - L / mfp_m
+ * (n_scat internal scatterings, L the path length).  Any constant baseline keeps the
+ * estimate unbiased; a value near E[x] lowers its variance.
+ * surface_group: nsurface entries, the gradient group of each surface or -1 (several
+ *   surfaces may share a group: the derivative is then with respect to a common shift
+ *   of their specularities); grouped surfaces need 0 < p < 1.  ngroup <= 32.
+ * batch_sums: nbatch x (3 + ngroup + nbranch) doubles, accumulated; path i adds to batch
+ *   i mod nbatch the values [1, x, x^2, (x - baseline) * score_0, ...], the scores
+ *   ordered as the ngroup specularity groups, then the nbranch branch scales.
+ * mode_paths / mode_trans are accumulated as by pt_trace_alias.
+ */
+int pt_adjoint_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, int64_t npaths,
+                     int32_t precision, const double *branch_scale,
+                     const int32_t *surface_group, int32_t ngroup, double baseline,
+                     int32_t nbatch, int64_t *mode_paths, int64_t *mode_trans,
+                     double *batch_sums, pt_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,8 @@ def lib():
                                         vp, i64, vp, vp, i64, vp]
         L.oracle_build_alias.argtypes = [i64, vp, vp, vp]
         L.oracle_trace_alias.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp, vp, vp]
+        L.oracle_adjoint_alias.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp, i32,
+                                           dbl, i32, vp]
         L.oracle_philox_draw.restype = dbl
         L.oracle_philox_draw.argtypes = [u64, u64, u64]
         L.oracle_pt_log.restype = dbl
@@ -134,6 +136,26 @@ def trace_alias(gdesc, mdesc, mfp_nm, prob, alias, seed, first, n, scale=None, m
                              _p(mp), _p(mt), _p(rec))
     out = (mp.reshape((nb, nk), order="F"), mt.reshape((nb, nk), order="F"))
     return out + (rec,) if records else out
+
+
+def adjoint_alias(gdesc, mdesc, mfp_nm, prob, alias, seed, first, n, group=None, ngroup=0,
+                  baseline=0.0, nbatch=64, scale=None, math=MATH_OWN):
+    """oracle_adjoint_alias: per-batch sums (nbatch, 3 + ngroup + nb) of
+    [1, x, x^2, (x - baseline) * score_k], paths in order."""
+    nb = mdesc.nbranch
+    mfp = np.ascontiguousarray(np.asarray(mfp_nm, dtype=np.float64).reshape(-1, order="F"))
+    prob = np.ascontiguousarray(prob, dtype=np.float64)
+    alias = np.ascontiguousarray(alias, dtype=np.int32)
+    sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+    grp = None if not ngroup else np.ascontiguousarray(group, dtype=np.int32)
+    sums = np.zeros((nbatch, 3 + ngroup + nb), dtype=np.float64)
+    rc = lib().oracle_adjoint_alias(ctypes.addressof(gdesc), ctypes.addressof(mdesc), _p(mfp),
+                                    _p(prob), _p(alias), _p(sc), int(seed), int(math), int(first),
+                                    int(n), _p(grp), int(ngroup), float(baseline), int(nbatch),
+                                    _p(sums))
+    if rc != 0:
+        raise ValueError("oracle_adjoint_alias: bad arguments")
+    return sums
 
 
 def philox_draw(seed, path, draw):

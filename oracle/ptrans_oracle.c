@@ -56,6 +56,12 @@ typedef struct {
      * the reference source line of every random number */
     double *tlog; int64_t tcap, tn;
     int16_t *slog; int64_t scap, sn;
+    /* adjoint bookkeeping (oracle_adjoint_alias): group of each surface (global index)
+     * or NULL, the path's score per group, internal scatterings and travelled length */
+    const int32_t *group;
+    double *score;
+    int64_t nscat;
+    double len;
 } ostate;
 
 /* site = the line of the RANDOM_NUMBER call in src/RTMCphon.f90 that this draw mirrors */
@@ -516,6 +522,7 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
             /* internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204 */
             double ct, st, ph, sph, cph;
             distance = freepath;
+            s->nscat++;
             for (k = 0; k < 3; ++k) pos[k] = pos[k] + distance * dir[k];
             ct = o_rand(s, 2197);
             ph = o_rand(s, 2198);
@@ -539,6 +546,13 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
             } else {
                 const pt_surface *sf = g->surfaces + g->grain_offset[igrain] + (isurface - 1);
                 double spec = o_rand(s, 2230);                    /* determinSpecular RT:2230 */
+                if (s->group) {
+                    /* d/dp log P: 1/p for a specular draw, -1/(1-p) for a diffuse one */
+                    int gi = s->group[g->grain_offset[igrain] + (isurface - 1)];
+                    if (gi >= 0)
+                        s->score[gi] += spec <= sf->specularity ? 1.0 / sf->specularity
+                                                                : -1.0 / (1.0 - sf->specularity);
+                }
                 if (spec <= sf->specularity) {
                     /* reflectSpecular, RT:2275-2328 */
                     if (o_determin_transmit(s, sf, 0.0, 0)) {
@@ -587,6 +601,7 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
             }
         }
         /* nudge, RT:1775-1780 */
+        s->len += distance;
         for (k = 0; k < 3; ++k) pos[k] = pos[k] + O_NUDGE * dir[k];
 
         /* FreePath update, RT:1783-1790 */
@@ -836,6 +851,64 @@ int oracle_trace_alias(const pt_geom_desc *g, const pt_modes_desc *m, const doub
             if (mode_trans && r.end_side != 1) mode_trans[mode]++;
         }
         if (records) records[i] = r;
+    }
+    return 0;
+}
+
+/*
+ * Likelihood-ratio adjoint of the alias-mode trace, paths [first, first+n) in order.
+ * x = T / mfp_m (T = 1 if the path did not end on the x- side); the path's scores are
+ *   group gi:  sum over its specular decisions on surfaces of group gi of 1/p
+ *              (specular) or -1/(1-p) (diffuse);
+ *   branch b:  n_scat / scale_b - L / mfp_m for the path's branch, 0 for the others
+ *              (free flights of rate scale_b / mfp_m: log P = n log scale_b -
+ *              scale_b L / mfp_m + const).
+ * Path i adds [1, x, x^2, (x - baseline) * score_k ...] to sums[(i mod nbatch) * K ...],
+ * K = 3 + ngroup + nbranch.  Paths failing the start-grain check add nothing.
+ */
+int oracle_adjoint_alias(const pt_geom_desc *g, const pt_modes_desc *m, const double *mfp_nm,
+                         const double *prob, const int32_t *alias, const double *scale,
+                         uint64_t seed, int math, int64_t first, int64_t n,
+                         const int32_t *group, int ngroup, double baseline, int nbatch,
+                         double *sums)
+{
+    ostate s;
+    int64_t i, nm = (int64_t)m->nbranch * m->nkpoint;
+    const int nb = m->nbranch, K = 3 + ngroup + nb;
+    double score[64];
+    if (ngroup > 64 || nbatch < 1) return -1;
+    memset(&s, 0, sizeof s);
+    s.g = g; s.m = m; s.math = math; s.seed = seed;
+    s.group = ngroup > 0 ? group : NULL;
+    s.score = score;
+    for (i = 0; i < n; ++i) {
+        pt_path_record r;
+        int64_t mode, pid = first + i;
+        int ib, k;
+        double lam, sb, x, w, *acc;
+        s.path = (uint64_t)pid;
+        s.draw = 0;
+        s.nscat = 0;
+        s.len = 0.0;
+        for (k = 0; k < ngroup; ++k) score[k] = 0.0;
+        mode = o_sample_mode(&s, nm, prob, alias);
+        ib = (int)(mode % nb);
+        s.cbranch = ib + 1;
+        s.ckpoint = (int)(mode / nb) + 1;
+        sb = scale ? scale[ib] : 1.0;
+        lam = mfp_nm[mode];
+        s.mfp = scale ? lam / scale[ib] : lam;
+        memset(&r, 0, sizeof r);
+        o_trace_particle(&s, 0.0, 0.0, 1, &r);
+        if (r.flags & PT_PATH_START_MISMATCH) continue;
+        x = r.end_side != 1 ? 1.0 / lam : 0.0;
+        w = x - baseline;
+        acc = sums + (pid % nbatch) * K;
+        acc[0] += 1.0;
+        acc[1] += x;
+        acc[2] += x * x;
+        for (k = 0; k < ngroup; ++k) acc[3 + k] += w * score[k];
+        acc[3 + ngroup + ib] += w * ((double)s.nscat / sb - s.len / lam);
     }
     return 0;
 }

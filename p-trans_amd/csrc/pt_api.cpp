@@ -60,7 +60,7 @@ struct pt_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     bool count_ops = false;
     int ncu = 0;
-    int blocks_per_cu = 0;
+    int blocks_per_cu = 0;                     // 0: occupancy maximum of each kernel
     int threads = 256;
     int chunk = 256;
     int refill_min = 16;
@@ -76,6 +76,7 @@ struct pt_ctx {
 struct pt_geom {
     pt_ctx *ctx = nullptr;
     DevGeom dg{};
+    std::vector<DSurf> surf;        // host copy of the surface records
     std::vector<void *> owned;
 };
 
@@ -127,12 +128,10 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = trace_occupancy(c->threads, &c->blocks_per_cu);
     if (e != hipSuccess) {
         pt_ctx_destroy(c);
         return fail(PT_E_HIP, "pt_ctx_create: %s", hipGetErrorString(e));
     }
-    if (c->blocks_per_cu < 1) c->blocks_per_cu = 1;
     *out = c;
     return PT_OK;
 }
@@ -175,13 +174,7 @@ int pt_ctx_set_tuning(pt_ctx *c, int32_t chunk, int32_t refill_min, int32_t bloc
                                   "blocks_per_cu >= 0 (0: occupancy maximum)");
     c->chunk = chunk;
     c->refill_min = refill_min;
-    int occ = 1;
-    if (blocks_per_cu == 0) {
-        PT_HIP(trace_occupancy(c->threads, &occ));
-        c->blocks_per_cu = occ < 1 ? 1 : occ;
-    } else {
-        c->blocks_per_cu = blocks_per_cu;
-    }
+    c->blocks_per_cu = blocks_per_cu;
     return PT_OK;
 }
 
@@ -349,6 +342,7 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
         return rc;
     }
     G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc};
+    G->surf = surf;
     DevGeom &g = G->dg;
     g.surf = d_surf;
     g.tang = d_tang;
@@ -376,6 +370,22 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
         return fail(PT_E_HIP, "pt_geom_create: %s", hipGetErrorString(e));
     }
     *out = G;
+    return PT_OK;
+}
+
+int pt_geom_set_specularity(pt_geom *G, const double *p)
+{
+    if (!G || !p) return fail(PT_E_INVALID, "pt_geom_set_specularity: NULL argument");
+    const int ns = G->dg.nsurface;
+    for (int i = 0; i < ns; ++i)
+        if (!(p[i] >= 0.0 && p[i] <= 1.0))
+            return fail(PT_E_INVALID, "pt_geom_set_specularity: specularity %d is %g, outside [0, 1]", i, p[i]);
+    for (int i = 0; i < ns; ++i) G->surf[i].p = p[i];
+    PT_HIP(hipSetDevice(G->ctx->device));
+    hipError_t e = hipMemcpyAsync((void *)G->dg.surf, G->surf.data(), ns * sizeof(DSurf),
+                                  hipMemcpyHostToDevice, G->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G->ctx->stream);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_geom_set_specularity: %s", hipGetErrorString(e));
     return PT_OK;
 }
 
@@ -541,6 +551,20 @@ TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
     return P;
 }
 
+// Workgroups of a persistent trace launch: CUs x (the tuned count, or the occupancy
+// maximum of this kernel variant with `lds` bytes of dynamic LDS).
+hipError_t launch_blocks(const pt_ctx *c, int mode, int precision, size_t lds, int *blocks)
+{
+    int per_cu = c->blocks_per_cu;
+    if (per_cu <= 0) {
+        hipError_t e = trace_occupancy(mode, precision, c->threads, lds, &per_cu);
+        if (e != hipSuccess) return e;
+        if (per_cu < 1) per_cu = 1;
+    }
+    *blocks = c->ncu * per_cu;
+    return hipSuccess;
+}
+
 int set_scale(pt_ctx *c, const pt_modes *M, const double *scale, TraceParams &P)
 {
     P.m.scale = nullptr;
@@ -643,13 +667,14 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     P.outcome = c->outcome;
     unsigned long long host_ctr[16] = {0};
     float ms = 0.f, ms_red = 0.f;
-    const int blocks = c->ncu * c->blocks_per_cu;
+    int blocks = 0;
     do {
+        if ((e = launch_blocks(c, TRACE_GRID, precision, 0, &blocks))) break;
         if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_GRID, blocks, c->threads, c->count_ops, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_GRID, blocks, c->threads, c->count_ops, 0, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
         if ((e = hipEventRecord(c->ev2, c->stream))) break;
@@ -733,13 +758,14 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     std::vector<unsigned long long> h_tal(2 * nm);
     unsigned long long host_ctr[16] = {0};
     float ms = 0.f;
-    const int blocks = c->ncu * c->blocks_per_cu;
+    int blocks = 0;
     hipError_t e;
     do {
+        if ((e = launch_blocks(c, TRACE_ALIAS, precision, 0, &blocks))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, 0, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
@@ -776,6 +802,104 @@ int pt_replay_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, const double
     int rc = set_scale(c, M, branch_scale, P);
     if (rc) return rc;
     return replay_common(c, P, TRACE_ALIAS, first, n, records);
+}
+
+int pt_adjoint_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npaths,
+                     int32_t precision, const double *branch_scale, const int32_t *surface_group,
+                     int32_t ngroup, double baseline, int32_t nbatch, int64_t *mode_paths,
+                     int64_t *mode_trans, double *batch_sums, pt_stats *stats)
+{
+    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
+    if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
+    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
+    if (ngroup < 0 || ngroup > kAdjMaxGroups)
+        return fail(PT_E_UNSUPPORTED, "ngroup %d: at most %d specularity groups", ngroup, kAdjMaxGroups);
+    if (ngroup > 0 && !surface_group) return fail(PT_E_INVALID, "surface_group is NULL");
+    if (nbatch < 1 || nbatch > 4096) return fail(PT_E_INVALID, "nbatch must be in 1..4096");
+    if (!std::isfinite(baseline)) return fail(PT_E_INVALID, "baseline is not finite");
+    if (!mode_paths || !mode_trans || !batch_sums)
+        return fail(PT_E_INVALID, "tally or batch arrays are NULL");
+    const int ns = G->dg.nsurface;
+    std::vector<int32_t> grp(ns, -1);
+    for (int i = 0; i < ns && ngroup > 0; ++i) {
+        const int gi = surface_group[i];
+        if (gi < -1 || gi >= ngroup)
+            return fail(PT_E_INVALID, "surface_group[%d] = %d outside -1..%d", i, gi, ngroup - 1);
+        if (gi >= 0 && !(G->surf[i].p > 0.0 && G->surf[i].p < 1.0))
+            return fail(PT_E_INVALID, "surface %d of group %d has specularity %g: the score of a "
+                                      "specularity needs 0 < p < 1", i, gi, G->surf[i].p);
+        grp[i] = gi;
+    }
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (npaths == 0) return PT_OK;
+    const size_t nm = (size_t)M->nb * M->nk;
+    PT_HIP(hipSetDevice(c->device));
+    TraceParams P = base_params(c, G, M);
+    P.precision = precision;
+    int rc = set_scale(c, M, branch_scale, P);
+    if (rc) return rc;
+    if (branch_scale)
+        for (int b = 0; b < M->nb; ++b)
+            if (!(branch_scale[b] > 0.0)) return fail(PT_E_INVALID, "branch_scale[%d] must be > 0", b);
+    P.total = npaths;
+    P.ngroup = ngroup;
+    P.nbatch = nbatch;
+    P.baseline = baseline;
+    const size_t nsum = (size_t)nbatch * adj_stride(P);
+    const size_t lds = adj_lds_bytes(P, c->threads);
+    hipDeviceProp_t prop;
+    PT_HIP(hipGetDeviceProperties(&prop, c->device));
+    if (lds > prop.sharedMemPerBlock)
+        return fail(PT_E_UNSUPPORTED, "adjoint needs %zu bytes of LDS per workgroup (limit %zu): "
+                                      "fewer groups or batches", lds, (size_t)prop.sharedMemPerBlock);
+    unsigned long long *d_tal = nullptr;
+    int32_t *d_grp = nullptr;
+    double *d_sum = nullptr;
+    PT_HIP(hipMalloc((void **)&d_tal, 2 * nm * sizeof(unsigned long long)));
+    hipError_t e = hipMalloc((void **)&d_grp, (size_t)ns * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_sum, nsum * sizeof(double));
+    std::vector<unsigned long long> h_tal(2 * nm);
+    std::vector<double> h_sum(nsum);
+    unsigned long long host_ctr[16] = {0};
+    float ms = 0.f;
+    int blocks = 0;
+    do {
+        if (e) break;
+        P.mode_paths = d_tal;
+        P.mode_trans = d_tal + nm;
+        P.surf_group = d_grp;
+        P.adj = d_sum;
+        if ((e = launch_blocks(c, TRACE_ADJOINT, precision, lds, &blocks))) break;
+        if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(d_sum, 0, nsum * sizeof(double), c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_ADJOINT, blocks, c->threads, c->count_ops, lds, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(h_sum.data(), d_sum, nsum * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipStreamSynchronize(c->stream))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    for (void *p : {(void *)d_tal, (void *)d_grp, (void *)d_sum})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_adjoint_alias: %s", hipGetErrorString(e));
+    for (size_t i = 0; i < nm; ++i) {
+        mode_paths[i] += (int64_t)h_tal[i];
+        mode_trans[i] += (int64_t)h_tal[nm + i];
+    }
+    for (size_t i = 0; i < nsum; ++i) batch_sums[i] += h_sum[i];
+    fill_stats(stats, host_ctr, ms, 0.f);
+    if ((int64_t)host_ctr[1] != npaths)
+        return fail(PT_E_HIP, "pt_adjoint_alias: %lld of %lld paths completed",
+                    (long long)host_ctr[1], (long long)npaths);
+    return PT_OK;
 }
 
 }  // extern "C"

@@ -65,7 +65,12 @@ enum : uint8_t {
     OUT_MISMATCH = 2,          // start-grain check failed (RT:1673-1682)
 };
 
-enum { TRACE_GRID = 0, TRACE_ALIAS = 1 };
+// TRACE_ADJOINT: the alias-mode trace plus likelihood-ratio scores of each path
+enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2 };
+
+// Largest number of specularity groups of one adjoint launch (each lane keeps one FP64
+// score per group in LDS).
+constexpr int kAdjMaxGroups = 32;
 
 struct TraceParams {
     DevGeom g;
@@ -88,16 +93,31 @@ struct TraceParams {
     uint8_t *outcome;          // grid: [total]
     unsigned long long *mode_paths;   // alias: [nb*nk]
     unsigned long long *mode_trans;   // alias: [nb*nk]
+    // adjoint: per batch (path id mod nbatch) the sums [n, sum x, sum x^2,
+    // sum (x - baseline) * score_k for k < ngroup + nb], x = T / mfp_m
+    const int32_t *surf_group; // [nsurface]: specularity group of each surface, or -1
+    int32_t ngroup;
+    int32_t nbatch;
+    double baseline;
+    double *adj;               // [nbatch * (3 + ngroup + nb)]
 };
+
+__host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
+inline size_t adj_lds_bytes(const TraceParams &p, int threads)
+{
+    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p)) * sizeof(double);
+}
 
 // kernels' launch wrappers (pt_trace.hip)
 hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
-                        hipStream_t s);
+                        size_t lds, hipStream_t s);
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
                               int64_t nb, int64_t nk, int32_t nangle,
                               long long *counts, unsigned long long *skipped, hipStream_t s);
 hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n,
                          pt_path_record *rec, hipStream_t s);
-hipError_t trace_occupancy(int threads, int *blocks_per_cu);
+// workgroups per CU the trace kernel of (mode, precision) can hold with `lds` bytes of
+// dynamic LDS
+hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int *blocks_per_cu);
 
 }  // namespace pt

@@ -796,10 +796,23 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     return true;
 }
 
+// Likelihood-ratio bookkeeping of one adjoint path.  Free flights are exponential with
+// rate s_b / mfp_m, so a path with n internal scatterings over a travelled length L has
+// log-likelihood n log s_b - s_b L / mfp_m + const; each specular decision on a surface of
+// group g (u <= p specular, RT:2230) adds log p or log(1 - p).  The lane's group scores
+// live in LDS, slot g at score[g * stride].
+struct AdjLane {
+    double *score;
+    int stride;
+    const int32_t *group;
+    int nscat;
+    double len;
+};
+
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
-template <int kPrec>
+template <int kPrec, bool kAdj>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
-                          Path &ph)
+                          Path &ph, AdjLane &al)
 {
     OpCount &oc = *rng.oc;
     oc.flops += 3;                       // box + SMALL_VAL
@@ -839,6 +852,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     if (ph.fp < distance) {
         // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
         distance = ph.fp;
+        if (kAdj) al.nscat++;
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
         if (kPrec == kPrecFast) {
@@ -874,6 +888,10 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         } else {
             const DSurf s = g.surf[ph.cur];
             const double spec = rng.next();                                 // RT:2230
+            if (kAdj) {
+                const int gi = al.group[ph.cur];
+                if (gi >= 0) al.score[gi * al.stride] += spec <= s.p ? 1.0 / s.p : -1.0 / (1.0 - s.p);
+            }
             if (spec <= s.p) {
                 // reflectSpecular, RT:2275-2328
                 if (determin_transmit(s, g.nmaterial, ph.omega, max_omega, rng, 0.0, false)) {
@@ -951,6 +969,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             }
         }
     }
+    if (kAdj) al.len += distance;
     // nudge, RT:1775-1780
     for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + PT_NUDGE * ph.dir[k];
     oc.flops += 6;
@@ -1042,11 +1061,29 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
 // Persistent trace kernel.  Each wave takes chunks of path ids from one global queue and
 // refills lanes whose path has ended (ballot + prefix count); lanes then advance one event
 // per iteration.  Grid mode writes one outcome byte per path (reduced per cell later);
-// alias mode adds to per-mode integer tallies (exact, order independent).
+// alias mode adds to per-mode integer tallies (exact, order independent).  Adjoint mode
+// is alias mode plus likelihood-ratio scores: each lane keeps its path's specularity
+// scores in LDS, and a finished path adds (x - baseline) * score to its batch's sums in
+// LDS, flushed to HBM once at the end.
 template <int kMode, bool kCount, int kPrec>
 __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
 {
+    constexpr bool kAdj = kMode == TRACE_ADJOINT;
+    extern __shared__ double adj_lds[];
     const int lane = threadIdx.x & 63;
+    AdjLane al;
+    al.stride = blockDim.x;
+    al.score = adj_lds + threadIdx.x;
+    al.group = P.surf_group;
+    al.nscat = 0;
+    al.len = 0.0;
+    double *adj_acc = adj_lds + (kAdj ? P.ngroup * blockDim.x : 0);
+    const int adj_k = kAdj ? adj_stride(P) : 0;
+    if (kAdj) {
+        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k;
+        for (int i = threadIdx.x; i < nlds; i += blockDim.x) adj_lds[i] = 0.0;
+        __syncthreads();
+    }
     OpCount oc = {0ull, 0ull};
     Rng rng;
     rng.oc = &oc;
@@ -1078,6 +1115,8 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
                 const int nneed = __popcll(need);
                 if (!active && rank < avail) {
                     pid = chunk_next + rank;
+                    al.nscat = 0;
+                    al.len = 0.0;
                     if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
                         active = true;
                     } else {
@@ -1095,13 +1134,31 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
             continue;
         }
         if (active) {
-            if (ph.end_side != 0 || step_path<kPrec>(P.g, P.m.max_omega, P.event_limit, rng, ph)) {
+            if (ph.end_side != 0 ||
+                step_path<kPrec, kAdj>(P.g, P.m.max_omega, P.event_limit, rng, ph, al)) {
                 const bool transmitted = ph.end_side != 1;          // RT:1833
                 if (kMode == TRACE_GRID) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
                 } else {
                     atomicAdd(P.mode_paths + mode, 1ull);
                     if (transmitted) atomicAdd(P.mode_trans + mode, 1ull);
+                }
+                if (kAdj) {
+                    const double lam = P.m.mfp[mode];
+                    const int ib = mode % P.m.nb;
+                    const double sb = P.m.scale ? P.m.scale[ib] : 1.0;
+                    const double x = transmitted ? 1.0 / lam : 0.0;
+                    const double w = x - P.baseline;
+                    double *acc = adj_acc + (int)(pid % P.nbatch) * adj_k;
+                    atomicAdd(acc + 0, 1.0);
+                    atomicAdd(acc + 1, x);
+                    atomicAdd(acc + 2, x * x);
+                    for (int gi = 0; gi < P.ngroup; ++gi) {
+                        const double v = al.score[gi * al.stride];
+                        if (v != 0.0) atomicAdd(acc + 3 + gi, w * v);
+                        al.score[gi * al.stride] = 0.0;
+                    }
+                    atomicAdd(acc + 3 + P.ngroup + ib, w * ((double)al.nscat / sb - al.len / lam));
                 }
                 s_paths++;
                 s_events += (unsigned long long)ph.events;
@@ -1113,6 +1170,11 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceParams P)
                 active = false;
             }
         }
+    }
+    if (kAdj) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < P.nbatch * adj_k; i += blockDim.x)
+            if (adj_acc[i] != 0.0) atomicAdd(P.adj + i, adj_acc[i]);
     }
     s_paths = wave_sum(s_paths);
     s_events = wave_sum(s_events);
@@ -1195,10 +1257,11 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
     rng.oc = &oc;
     rng.seed = P.seed;
     Path ph;
+    AdjLane al = {nullptr, 0, nullptr, 0, 0.0};
     int mode = 0;
     if (begin_path<kMode, kPrec>(P, first + i, rng, ph, &mode)) {
         while (ph.end_side == 0) {
-            if (step_path<kPrec>(P.g, P.m.max_omega, P.event_limit, rng, ph)) break;
+            if (step_path<kPrec, false>(P.g, P.m.max_omega, P.event_limit, rng, ph, al)) break;
         }
     }
     pt_path_record r;
@@ -1213,10 +1276,10 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
 }
 
 hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
-                        hipStream_t s)
+                        size_t lds, hipStream_t s)
 {
     const dim3 gr(blocks), bl(threads);
-#define PT_LAUNCH(M, C, R) hipLaunchKernelGGL((trace_kernel<M, C, R>), gr, bl, 0, s, p)
+#define PT_LAUNCH(M, C, R) hipLaunchKernelGGL((trace_kernel<M, C, R>), gr, bl, lds, s, p)
 #define PT_LAUNCH_C(M, R) do { if (count) PT_LAUNCH(M, true, R); else PT_LAUNCH(M, false, R); } while (0)
 #define PT_LAUNCH_R(M)                                                    \
     do {                                                                  \
@@ -1225,17 +1288,25 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
         else PT_LAUNCH_C(M, kPrecFP64);                                   \
     } while (0)
     if (mode == TRACE_GRID) PT_LAUNCH_R(TRACE_GRID);
-    else PT_LAUNCH_R(TRACE_ALIAS);
+    else if (mode == TRACE_ALIAS) PT_LAUNCH_R(TRACE_ALIAS);
+    else PT_LAUNCH_R(TRACE_ADJOINT);
 #undef PT_LAUNCH_R
 #undef PT_LAUNCH_C
 #undef PT_LAUNCH
     return hipGetLastError();
 }
 
-hipError_t trace_occupancy(int threads, int *blocks_per_cu)
+hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, trace_kernel<TRACE_GRID, false, kPrecFilter>, threads, 0);
+#define PT_OCC(M, R) hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_kernel<M, false, R>, threads, lds)
+#define PT_OCC_R(M)                                                                  \
+    (precision == kPrecFast ? PT_OCC(M, kPrecFast)                                   \
+                            : precision == kPrecFilter ? PT_OCC(M, kPrecFilter) : PT_OCC(M, kPrecFP64))
+    if (mode == TRACE_GRID) return PT_OCC_R(TRACE_GRID);
+    if (mode == TRACE_ALIAS) return PT_OCC_R(TRACE_ALIAS);
+    return PT_OCC_R(TRACE_ADJOINT);
+#undef PT_OCC_R
+#undef PT_OCC
 }
 
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,

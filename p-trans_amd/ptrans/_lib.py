@@ -18,9 +18,10 @@ EXPORTS = (
     "pt_version", "pt_build_info", "pt_last_error",
     "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_set_seed", "pt_ctx_set_event_limit",
     "pt_ctx_set_op_counting", "pt_ctx_set_tuning",
-    "pt_geom_create", "pt_geom_destroy", "pt_surface_setup_polygon",
+    "pt_geom_create", "pt_geom_destroy", "pt_geom_set_specularity", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
     "pt_trace_grid", "pt_replay", "pt_alias_build", "pt_trace_alias", "pt_replay_alias",
+    "pt_adjoint_alias",
 )
 
 
@@ -80,6 +81,8 @@ def load():
     L.pt_alias_build.argtypes = [i64, vp, vp, vp]
     L.pt_trace_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
     L.pt_replay_alias.argtypes = [vp, vp, vp, vp, i64, i64, i32, vp]
+    L.pt_geom_set_specularity.argtypes = [vp, vp]
+    L.pt_adjoint_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, i32, dbl, i32, vp, vp, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

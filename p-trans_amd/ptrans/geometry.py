@@ -368,3 +368,30 @@ def _fill_grain(grains, ngrain, igrain_fill, kind_t, trans_t, spec_t) -> Grain:
             _set_trans(s, kind_t[(s.imaterial2, s.imaterial1)], trans_t[(s.imaterial2, s.imaterial1)])
             fill.surfaces.append(s)
     return fill
+
+
+def inclusion_groups(geo: Geometry):
+    """Specularity groups of a bricks geometry, one per brick: a brick's own surfaces
+    (grain_inside = brick) and the fill grain's reversed copies of them (grain_outside =
+    brick, setupFillGrain RT:1336-1398) form group brick-1; box walls and the rest get
+    -1.  Returns (group per surface in descriptor order, number of groups)."""
+    surf = geo.flatten()["surfaces"]
+    gin = surf["grain_inside"].astype(np.int64)
+    gout = surf["grain_outside"].astype(np.int64)
+    nb = geo.ngrain
+    own = (gin >= 1) & (gin <= nb)
+    copy = ~own & (gout >= 1) & (gout <= nb) & (gin == geo.igrain_fill)
+    group = np.full(len(surf), -1, dtype=np.int32)
+    group[own] = gin[own] - 1
+    group[copy] = gout[copy] - 1
+    return group, nb
+
+
+def group_specularity(specularity, group, values):
+    """Per-surface specularities with every surface of group g set to values[g]."""
+    p = np.array(specularity, dtype=np.float64, copy=True)
+    group = np.asarray(group)
+    values = np.asarray(values, dtype=np.float64)
+    sel = group >= 0
+    p[sel] = values[group[sel]]
+    return p

@@ -271,3 +271,47 @@ def alias_kappa(mode_paths, mode_trans, mfp_nm, length: float) -> AliasKappa:
     mean = s1 / n
     var = max(s2 / n - mean * mean, 0.0)
     return AliasKappa(ratio=mean, stderr=math.sqrt(var / n), paths=n, var_x=var)
+
+
+@dataclasses.dataclass
+class AliasGradient:
+    ratio: float              # kappa / kappa_bulk = (3L/4) E[x], x = T / Lambda_m
+    stderr: float
+    grad: np.ndarray          # d ratio / d theta: ngroup specularities, then nb branch scales
+    grad_stderr: np.ndarray
+    paths: int
+    ngroup: int
+
+    @property
+    def grad_p(self):
+        return self.grad[:self.ngroup]
+
+    @property
+    def grad_s(self):
+        return self.grad[self.ngroup:]
+
+
+def _ratio_of_sums(num, den):
+    """sum(num)/sum(den) over batches and its batch-means standard error (ratio estimator:
+    residuals num_j - r den_j, variance B/(B-1) sum r_j^2 / (sum den)^2)."""
+    num = np.asarray(num, dtype=np.float64)
+    den = np.asarray(den, dtype=np.float64)
+    tot = den.sum()
+    r = num.sum(axis=0) / tot
+    res = num - np.multiply.outer(den, r) if num.ndim > 1 else num - den * r
+    b = den.size
+    var = (res * res).sum(axis=0) * b / max(b - 1, 1) / tot ** 2
+    return r, np.sqrt(var)
+
+
+def alias_gradients(batch_sums, length: float, ngroup: int) -> AliasGradient:
+    """kappa/kappa_bulk and its likelihood-ratio gradient from pt_adjoint_alias batch sums
+    [n, sum x, sum x^2, sum (x - b) score_k] (x = T / Lambda_m): ratio = (3L/4) E[x],
+    d ratio / d theta_k = (3L/4) E[(x - b) score_k], errors by batch means."""
+    s = np.asarray(batch_sums, dtype=np.float64)
+    n = s[:, 0]
+    c = 0.75 * length
+    ratio, se = _ratio_of_sums(s[:, 1], n)
+    grad, gse = _ratio_of_sums(s[:, 3:], n)
+    return AliasGradient(ratio=float(c * ratio), stderr=float(c * se), grad=c * grad,
+                         grad_stderr=c * gse, paths=int(n.sum()), ngroup=int(ngroup))

@@ -29,6 +29,18 @@ class Geom:
         _lib.check(_lib.load().pt_geom_create(ctx.handle, ctypes.addressof(self._desc), ctypes.byref(h)))
         self.handle = h
 
+    def set_specularity(self, specularity):
+        """pt_geom_set_specularity: new surface%specularity for every surface."""
+        p = np.ascontiguousarray(specularity, dtype=np.float64).reshape(-1)
+        if p.size != len(self._arrays["surfaces"]):
+            raise ValueError("need one specularity per surface (%d)" % len(self._arrays["surfaces"]))
+        _lib.check(_lib.load().pt_geom_set_specularity(self.handle, p.ctypes.data))
+        self._arrays["surfaces"]["specularity"] = p
+
+    @property
+    def specularity(self):
+        return self._arrays["surfaces"]["specularity"].copy()
+
     def close(self):
         if self.handle:
             _lib.load().pt_geom_destroy(self.handle)
@@ -163,6 +175,32 @@ class Context:
             self.handle, geom.handle, modes.handle, sc.ctypes.data if sc is not None else None,
             int(first), int(n), int(precision), out.ctypes.data))
         return out
+
+
+def _adjoint_alias(self, geom: Geom, modes: Modes, npaths: int, surface_group=None,
+                   ngroup: int = 0, baseline: float = 0.0, nbatch: int = 64, branch_scale=None,
+                   precision: int = PREC_FP64):
+    """pt_adjoint_alias: per-mode tallies, per-batch sums (nbatch, 3 + ngroup + nb) of
+    [1, x, x^2, (x - baseline) * score_k] with x = T / mfp_m, and the stats."""
+    nb, nk = modes.nbranch, modes.nkpoint
+    paths = np.zeros(nb * nk, dtype=np.int64)
+    trans = np.zeros(nb * nk, dtype=np.int64)
+    sums = np.zeros((nbatch, 3 + ngroup + nb), dtype=np.float64)
+    stats = _lib.PtStats()
+    sc = None if branch_scale is None else np.ascontiguousarray(branch_scale, dtype=np.float64)
+    grp = None
+    if ngroup:
+        grp = np.ascontiguousarray(surface_group, dtype=np.int32)
+    _lib.check(_lib.load().pt_adjoint_alias(
+        self.handle, geom.handle, modes.handle, int(npaths), int(precision),
+        sc.ctypes.data if sc is not None else None, grp.ctypes.data if grp is not None else None,
+        int(ngroup), float(baseline), int(nbatch), paths.ctypes.data, trans.ctypes.data,
+        sums.ctypes.data, ctypes.byref(stats)))
+    return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"), sums,
+            stats.as_dict())
+
+
+Context.adjoint_alias = _adjoint_alias
 
 
 def alias_table(weights):
