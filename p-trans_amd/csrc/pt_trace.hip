@@ -338,6 +338,8 @@ struct Path {
     int64_t events;
     unsigned retests;       // fast path: candidates evaluated in FP64 by the filter
     unsigned fallbacks;     // fast path: events decided by the full FP64 scan
+    float bc[3];            // safety ball: centre and radius (br <= 0: none); no surface
+    float br;               //   the scan can hit lies within it
 };
 
 // ---------------------------------------------------------------------------------
@@ -632,6 +634,71 @@ __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *b
     }
 }
 
+// Safety ball of a diffusive path (an acceleration with no counterpart in the
+// reference).  At the current position, R = the smallest distance to any surface the
+// collision scan tests (box walls and the current grain's candidates): the plane of a
+// polygon (its hit points lie on that plane, RT:2928-2938), | |p - c| - r | for a
+// sphere, and the radial distance for a z-cylinder.  While a free flight starts and ends
+// inside the ball no candidate can be hit before it ends, so the reference's scan would
+// return a distance beyond the free path and the event is an internal scattering: the
+// event then skips the scan and is otherwise traced unchanged.  Distances are bounded in
+// FP32 and the radius shrunk by 1e-5 relative + 1e-2 nm, far above the rounding of both
+// the bound and the reference's FP64 hit distances.
+__device__ void ball_setup(const DevGeom &g, Path &ph)
+{
+    const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
+    float r = 3.0e38f;
+    bool todo0 = true, todo1 = ph.which != 0;
+    while (__any(todo0 || todo1)) {
+        if (todo0 || todo1) {
+            const int mine = todo0 ? 0 : ph.which;
+            const int ig = __builtin_amdgcn_readfirstlane(mine);
+            if (mine == ig) {
+                const int j0 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig]);
+                const int j1 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig + 1]);
+                for (int j = j0; j < j1; ++j) {
+                    const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
+                    const DSurf s = load_surf_c(g.surf, idx);
+                    float d;
+                    if (s.kind == PT_SHAPE_POLYGON) {
+                        cdouble *v = (cdouble *)g.vtx + 3 * (int64_t)s.voff;
+                        const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
+                        const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny +
+                                        (pz - (float)v[2]) * nz;
+                        d = fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+                    } else {
+                        const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
+                        const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
+                        d = fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
+                    }
+                    r = fminf(r, d);
+                }
+                if (todo0) todo0 = false;
+                else todo1 = false;
+            }
+        }
+    }
+    ph.bc[0] = px;
+    ph.bc[1] = py;
+    ph.bc[2] = pz;
+    ph.br = r * (1.0f - 1.0e-5f) - 1.0e-2f;
+}
+
+// True when the free flight from the current position stays inside the safety ball.
+__device__ inline bool in_ball(const Path &ph)
+{
+    if (!(ph.br > 0.0f)) return false;
+    const float r2 = ph.br * ph.br;
+    const float ax = (float)ph.pos[0] - ph.bc[0], ay = (float)ph.pos[1] - ph.bc[1],
+                az = (float)ph.pos[2] - ph.bc[2];
+    if (!(ax * ax + ay * ay + az * az < r2)) return false;
+    if (!(ph.fp < (double)ph.br * 2.0)) return false;
+    const float ex = (float)(ph.pos[0] + ph.fp * ph.dir[0]) - ph.bc[0];
+    const float ey = (float)(ph.pos[1] + ph.fp * ph.dir[1]) - ph.bc[1];
+    const float ez = (float)(ph.pos[2] + ph.fp * ph.dir[2]) - ph.bc[2];
+    return ex * ex + ey * ey + ez * ez < r2;
+}
+
 // determinTransmit, RT:2384-2413
 __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, double max_omega,
                                   Rng &rng, double rand_in, bool present)
@@ -781,6 +848,7 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     ph.events = 0;
     ph.retests = 0;
     ph.fallbacks = 0;
+    ph.br = -1.0f;
     int which = 0;
     if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
     else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos, oc);
@@ -834,7 +902,11 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     double dmin = 1.0e10;
     int best = ph.cur;
     bool decided = false;
-    if (kPrec != kPrecFP64) {
+    // accelerated precisions: inside its safety ball the free flight ends first (dmin
+    // stays beyond it and the event is an internal scattering, as the scan would find)
+    const bool in_safe = kPrec != kPrecFP64 && in_ball(ph);
+    if (in_safe) decided = true;
+    if (kPrec != kPrecFP64 && !in_safe) {
         decided = scan_fast(g, ph, &dmin, &best, oc);
         if (!decided) {
             dmin = 1.0e10;
@@ -849,9 +921,11 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     ph.cur = best;
 
     int jreflect = 1;
+    bool scattered = false;
     if (ph.fp < distance) {
         // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
         distance = ph.fp;
+        scattered = true;
         if (kAdj) al.nscat++;
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
@@ -988,8 +1062,15 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             }
         }
     }
+    // a diffusive path (mean free path under 5% of the smallest box side) that scattered
+    // outside its ball takes a new ball around the scattering point
+    if (kPrec != kPrecFP64) {
+        const double lim = 0.05 * fmin(g.box[0], fmin(g.box[1], g.box[2]));
+        if (scattered && !in_safe && ph.mfp < lim) ball_setup(g, ph);
+    }
     // whichV on transmission, RT:1793-1796
     if (jreflect == 0) {
+        ph.br = -1.0f;                  // the candidates change with the grain
         ph.which = g.surf[ph.cur].gout;
         if (ph.which <= 0) {            // DO ig=0,whichV,whichV with whichV=0 is undefined
             ph.flags |= PT_PATH_LOST;

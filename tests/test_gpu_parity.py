@@ -219,3 +219,59 @@ def test_fp32_directions_unit_length_on_specular_spheres(gpu_ctx):
             assert np.all(np.linalg.norm(rec["pos"] - c, axis=1) > r - 1.0e-6)
     assert np.max(np.abs(np.linalg.norm(b["dir"], axis=1) - 1.0)) < 1.0e-14
     assert abs(b["events"].mean() / a["events"].mean() - 1.0) < 0.05
+
+
+def test_safety_ball_diffusive_paths_bitexact(gpu_ctx):
+    """Deeply diffusive paths (2 nm mean free path in a 200 x 100 x 100 nm wire): the
+    accelerated precisions skip the collision scan while a free flight stays inside the
+    path's safety ball; per-path records still equal the oracle's (which always scans) bit
+    for bit, and the counts of a larger run equal the FP64 trace's."""
+    model = gen.nanowire(length=200.0, width=100.0, height=100.0, p_wall=0.4, nphonon=200,
+                         nangle=5)
+    geo = geometry.build_geometry(model)
+    om = np.linspace(1e12, 9e13, 2).reshape(2, 1)
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(om)
+    gpu_ctx.set_seed(23)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    gpu_ctx.set_event_limit(10_000_000)
+    try:
+        rec = gpu_ctx.replay(g, m, 2.0, 5, 200, 0, 1200, precision=FAST)
+        orec = O.replay(gdesc, mdesc, 2.0, 5, 200, 23, 0, 1200, math=O.MATH_OWN)
+        print("mean events", rec["events"].mean(), "max", rec["events"].max(), flush=True)
+        assert rec["events"].mean() > 20
+        for f in ("end_side", "flags", "events", "draws", "which_grain"):
+            np.testing.assert_array_equal(rec[f], orec[f], err_msg=f)
+        np.testing.assert_array_equal(rec["pos"].view(np.uint64), orec["pos"].view(np.uint64))
+        np.testing.assert_array_equal(rec["dir"].view(np.uint64), orec["dir"].view(np.uint64))
+        a, sa = gpu_ctx.trace_grid(g, m, 2.0, 5, 20000)
+        b, sb = gpu_ctx.trace_grid(g, m, 2.0, 5, 20000, precision=FAST)
+        print("events", sa["events"], "fp64 ms", sa["kernel_ms"], "filter+ball ms", sb["kernel_ms"],
+              flush=True)
+        np.testing.assert_array_equal(a, b)
+        assert sa["events"] == sb["events"] and sa["draws"] == sb["draws"]
+    finally:
+        gpu_ctx.set_event_limit(100_000_000)
+
+
+def test_safety_ball_alias_inclusions_bitexact(gpu_ctx):
+    """Alias mode on the C3 inclusions with mean free paths 1 - 40 nm: curved candidates
+    bound the ball too; FP32-filter records equal the oracle's."""
+    from test_alias import _weights
+    model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.4, nphonon=1, nangle=3)
+    geo = geometry.build_geometry(model)
+    t = synthetic_table(2, 4, 9, mfp_min_nm=1.0, mfp_max_nm=40.0)
+    omega = t.omega_cm() * 0.03 * 3.14159265358979323846 * 2.0 * 1.0e12
+    gpu_ctx.set_seed(31)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    rec = gpu_ctx.replay_alias(g, m, 0, 600, precision=FAST)
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
+    prob, alias = O.build_alias(_weights(omega, t.vel, t.mfp_nm, t.weight))
+    _, _, orec = O.trace_alias(gdesc, mdesc, t.mfp_nm, prob, alias, 31, 0, 600, records=True)
+    print("mean events", rec["events"].mean(), "max", rec["events"].max(), flush=True)
+    for f in ("end_side", "flags", "events", "draws", "which_grain"):
+        np.testing.assert_array_equal(rec[f], orec[f], err_msg=f)
+    np.testing.assert_array_equal(rec["pos"].view(np.uint64), orec["pos"].view(np.uint64))
