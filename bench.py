@@ -12,6 +12,13 @@ before timing starts.
 
 Prints one JSON line (metric, value = paths/s, roofline of the trace kernel and the
 CPU baseline: the oracle's C restatement on the host's cores, on a bounded sample).
+
+The "adjoint" object measures the second half of the metric on the inclusions
+configuration (1000 nm cube, 8 spheres + 4 z-cylinders, each inclusion with its own
+specularity; same bulk table): the alias-mode forward trace and the likelihood-ratio
+adjoint (12 specularity + 6 branch-scale derivatives of kappa) over 1e8 paths, and
+kappa-gradient evaluations/s = adjoint paths/s / the paths that make the gradient
+2-sigma significant (|stderr| = |grad| / 2).
 """
 from __future__ import annotations
 
@@ -93,6 +100,64 @@ def cpu_baseline(geo, omega, mfp, nangle, np_per_cell, seed, seconds, threads):
     return state["paths"] / dt, state["paths"], min(state["next"], ncell), dt
 
 
+def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
+    """C3 inclusions: forward alias trace (FP32 trace and FP64) and the LR adjoint."""
+    from ptrans.constants import PREC_FP32_RETEST as FAST
+    model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.5, nphonon=1, nangle=3)
+    geo = geometry.build_geometry(model)
+    group, ngroup = geometry.inclusion_groups(geo)
+    pvals = np.random.default_rng(3).uniform(0.05, 0.95, ngroup)
+    p0 = geometry.group_specularity(geo.flatten()["surfaces"]["specularity"], group, pvals)
+    t = synthetic_table(6, 2000, seed=table_seed)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    g = ctx.geometry(geo)
+    g.set_specularity(p0)
+    m = ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    length = geo.box[0]
+    ctx.set_seed(777)
+    ctx.trace_alias(g, m, 1_000_000, precision=FAST)                       # warm-up
+    ctx.adjoint_alias(g, m, 1_000_000, surface_group=group, ngroup=ngroup, precision=FAST)
+    t0 = time.perf_counter()
+    mp, mt, fst = ctx.trace_alias(g, m, npaths, precision=FAST)
+    t_fwd = time.perf_counter() - t0
+    k32 = post.alias_kappa(mp, mt, t.mfp_nm, length)
+    ctx.set_seed(778)
+    _, _, pilot, _ = ctx.adjoint_alias(g, m, 1_000_000, nbatch=1, precision=FAST)
+    base = pilot[0, 1] / pilot[0, 0]
+    t0 = time.perf_counter()
+    _, _, sums, ast = ctx.adjoint_alias(g, m, npaths, surface_group=group, ngroup=ngroup,
+                                        baseline=base, nbatch=256, precision=FAST)
+    t_adj = time.perf_counter() - t0
+    gr = post.alias_gradients(sums, length, ngroup)
+    ctx.set_seed(777)
+    mp64, mt64, st64 = ctx.trace_alias(g, m, npaths, precision=PREC_FP64)
+    k64 = post.alias_kappa(mp64, mt64, t.mfp_nm, length)
+    gnorm = float(np.linalg.norm(gr.grad))
+    snorm = float(np.linalg.norm(gr.grad_stderr))
+    n2s = npaths * (snorm / (0.5 * gnorm)) ** 2 if gnorm > 0 else None
+    adj_rate = npaths / t_adj
+    return {
+        "workload": "1000 nm cube, 8 spheres + 4 z-cylinders (Gamma law, per-inclusion "
+                    "specularity U(0.05, 0.95)), 6x2000 modes, alias mode, FP32 trace",
+        "paths": npaths,
+        "forward_paths_per_s": npaths / t_fwd,
+        "forward_kernel_ms": fst["kernel_ms"],
+        "adjoint_paths_per_s": adj_rate,
+        "adjoint_kernel_ms": ast["kernel_ms"],
+        "adjoint_cost_vs_forward": t_adj / t_fwd,
+        "components": ngroup + 6,
+        "kappa_ratio": gr.ratio, "kappa_ratio_stderr": gr.stderr,
+        "kappa_ratio_fp64": k64.ratio, "kappa_ratio_fp64_stderr": k64.stderr,
+        "kappa_rel_err_vs_fp64": abs(k32.ratio - k64.ratio) / k64.ratio,
+        "kappa_diff_in_sigma": abs(k32.ratio - k64.ratio) / float(np.hypot(k32.stderr, k64.stderr)),
+        "fp64_forward_paths_per_s": npaths / max(1e-9, st64["kernel_ms"] * 1e-3),
+        "grad_norm": gnorm, "grad_stderr_norm": snorm,
+        "paths_for_2sigma_gradient": n2s,
+        "kappa_gradient_evals_per_s": adj_rate / n2s if n2s else None,
+        "events_per_path": ast["events"] / max(1, ast["paths"]),
+    }
+
+
 def roofline(cstats, kernel_ms):
     """VALU roofline of the trace kernel: algorithmic FP64 and FP32 operations of one
     launch (instrumented twin kernel, same paths) over the timed kernel's average duration
@@ -121,6 +186,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-paths", type=float, default=1e8,
+                    help="paths of the inclusions adjoint leg (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
     args = ap.parse_args()
@@ -167,6 +234,7 @@ def main():
             ctx.set_op_counting(False)
             res[prec] = dict(dt=dt, kernel_ms=float(np.mean(kern_ms)), counts=counts, stats=stats,
                              cstats=cstats)
+        adj = adjoint_leg(ctx, int(args.adjoint_paths)) if args.adjoint_paths > 0 else None
     fast, plain, filt = res[PREC_FP32_RETEST], res[PREC_FP64], res[PREC_FP32_FILTER]
     dt = fast["dt"]
     value = paths * args.steps / dt
@@ -203,8 +271,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded film geometry and bulk table)",
         "config": {"workload": "thin film 600x600x200 nm, 3 z-cylinder holes r=50 nm p=0.3, "
-                               "6x2000 modes x 10 angles x %d particles, ballistic FP64 grid "
-                               "trace" % args.np,
+                               "6x2000 modes x 10 angles x %d particles, ballistic grid "
+                               "trace (TransmissionFunction)" % args.np,
                    "paths_per_step": paths, "events_per_path": events_per_path,
                    "transmission": transmitted},
         "kernel_ms": k_ms,
@@ -226,6 +294,7 @@ def main():
         "fallbacks_per_event": stats["fallbacks"] / max(1, stats["events"]),
         "roofline": roofline(cstats, k_ms),
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(out))
 
