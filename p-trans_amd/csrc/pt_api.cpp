@@ -529,6 +529,11 @@ int check_trace_args(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp,
     if (nangle < 2) return fail(PT_E_INVALID, "nangle must be >= 2");
     if (np < 0) return fail(PT_E_INVALID, "np must be >= 0");
     if (!(mfp > 0)) return fail(PT_E_INVALID, "mean free path must be positive");
+    // the kernels number cells in 32 bits and divide path ids in FP64
+    const int64_t ncell = (int64_t)M->nb * M->nk * (nangle - 1);
+    if (ncell >= ((int64_t)1 << 31)) return fail(PT_E_UNSUPPORTED, "more than 2^31 cells");
+    if (np > 0 && ncell > (((int64_t)1 << 53) - 1) / np)
+        return fail(PT_E_UNSUPPORTED, "more than 2^53 paths in one call");
     return PT_OK;
 }
 
