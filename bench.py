@@ -158,23 +158,28 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
     }
 
 
-def roofline(cstats, kernel_ms):
+def roofline(cstats, kernel_ms, kernel_key):
     """VALU roofline of the trace kernel: algorithmic FP64 and FP32 operations of one
     launch (instrumented twin kernel, same paths) over the timed kernel's average duration
     (HIP events on the library's stream), against MI355X's FP64 vector peak (non-packed
-    FP32 issues at the same rate).  HBM traffic per launch comes from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present."""
+    FP32 issues at the same rate).  From the committed rocprofv3 summary of the same
+    kernel: HBM traffic per launch (FETCH_SIZE/WRITE_SIZE scaled by a device copy) and
+    the VALU issue rate, i.e. wave-instructions per second over 256 CUs x 2.4 GHz."""
     ops = cstats["fp64_ops"]
     achieved = ops / (kernel_ms * 1e-3) / 1e12
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "trace_grid_film_traffic.json")
+    prof = {}
+    tf = os.path.join(ROOT, "profiles", "trace_kernel_film_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f).get("kernels", {}).get(kernel_key, {})
     return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": prof.get("hbm_bytes_per_launch"),
             "fp64_ops_per_launch": ops, "fp64_ops_per_path": ops / max(1, cstats["paths"]),
-            "philox_blocks_per_path": cstats["philox_blocks"] / max(1, cstats["paths"])}
+            "philox_blocks_per_path": cstats["philox_blocks"] / max(1, cstats["paths"]),
+            "valu_issue_fraction_profiled": prof.get("valu_issue_fraction"),
+            "valu_wave_instr_per_path_profiled": prof.get("valu_wave_instr_per_path"),
+            "profiled_kernel_us": prof.get("average_us")}
 
 
 def main():
@@ -284,7 +289,7 @@ def main():
         "fp64_plain": {"value": paths * args.steps / plain["dt"],
                        "ms_per_step": plain["dt"] / args.steps * 1e3,
                        "kernel_ms": plain["kernel_ms"],
-                       "roofline": roofline(plain["cstats"], plain["kernel_ms"])},
+                       "roofline": roofline(plain["cstats"], plain["kernel_ms"], "fp64")},
         "speedup_vs_fp64": plain["dt"] / dt,
         "kappa": {"value_W_mK": kap, "stderr": kap_err, "rel_err_vs_fp64": abs(kap - kap64) / kap64,
                   "counts_identical_to_fp64": bool(np.array_equal(counts, plain["counts"])),
@@ -292,7 +297,7 @@ def main():
                   "seconds_for_1e-3_rel_stderr": need / value if need else None},
         "retests_per_event": stats["retests"] / max(1, stats["events"]),
         "fallbacks_per_event": stats["fallbacks"] / max(1, stats["events"]),
-        "roofline": roofline(cstats, k_ms),
+        "roofline": roofline(cstats, k_ms, "fp32_trace"),
         "cpu_baseline": cpu,
         "adjoint": adj,
     }
