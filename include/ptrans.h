@@ -55,6 +55,15 @@ extern "C" {
 #define PT_TRANS_VALUE   1   /* t = trans_param                                    */
 #define PT_TRANS_GAMMA   2   /* t = 1/(gamma*omega/omega_max + 1), gamma=trans_param */
 #define PT_TRANS_TBC     3   /* as GAMMA; gamma was solved from a TBC by the host    */
+/* extensions (no counterpart in the reference, whose laws are the three above):      */
+#define PT_TRANS_LORENTZ 4   /* t = 1/(1 + (omega/omega0)^2), omega0 = trans_param
+                                (rad/s), decided by u <= t as the reference's laws    */
+#define PT_TRANS_PERIODIC 5  /* box side walls only (surfaces 2..5 of grain 0, in
+                                pairs): a phonon reaching the wall continues from the
+                                opposite wall with the same direction and no draw;
+                                inclusions must not cross a periodic wall.  The
+                                reference has no periodic boundary (its box walls
+                                reflect with CellSpecularity, RT:1255-1331).          */
 
 /* grain kinds, src/RTMCphon.f90:131-132 */
 #define PT_GRAIN_VORONOI 1
@@ -294,6 +303,22 @@ int pt_adjoint_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, in
                      const int32_t *surface_group, int32_t ngroup, double baseline,
                      int32_t nbatch, int64_t *mode_paths, int64_t *mode_trans,
                      double *batch_sums, pt_stats *stats);
+
+/*
+ * Alias-mode trace with per-grain heat-flux tallies (an extension; the reference tallies
+ * only the transmitted count per cell, RT:1833-1838).  As pt_trace_alias, and for every
+ * grain g = 0 .. ngrain_total-1 adds over all flights spent in grain g (whichV during the
+ * flight) of every path of mode m:
+ *     grain_tally[2*g]   += dx / mfp_m     (signed x displacement: the heat flux)
+ *     grain_tally[2*g+1] += l  / mfp_m     (path length)
+ * in FP64.  A path's displacements sum to L T (up to the 1e-7 nm nudges, RT:1779), so
+ * sum_g grain_tally[2g] = L sum_paths T / mfp_m: kappa/kappa_bulk = (3/4) sum_g
+ * grain_tally[2g] / npaths splits into per-grain shares.
+ */
+int pt_trace_alias_grains(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
+                          int64_t npaths, int32_t precision, const double *branch_scale,
+                          int64_t *mode_paths, int64_t *mode_trans, double *grain_tally,
+                          pt_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,8 @@ def lib():
         L.oracle_trace_alias.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp, vp, vp]
         L.oracle_adjoint_alias.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp, i32,
                                            dbl, i32, vp]
+        L.oracle_trace_alias_grains.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp,
+                                                vp, vp]
         L.oracle_philox_draw.restype = dbl
         L.oracle_philox_draw.argtypes = [u64, u64, u64]
         L.oracle_pt_log.restype = dbl
@@ -156,6 +158,24 @@ def adjoint_alias(gdesc, mdesc, mfp_nm, prob, alias, seed, first, n, group=None,
     if rc != 0:
         raise ValueError("oracle_adjoint_alias: bad arguments")
     return sums
+
+
+def trace_alias_grains(gdesc, mdesc, mfp_nm, prob, alias, seed, first, n, ngrain_total,
+                       scale=None, math=MATH_OWN):
+    """oracle_trace_alias_grains: per-mode (nb, nk) paths, transmitted, and per-grain
+    (ngrain_total, 2) sums of [dx / mfp_m, l / mfp_m]."""
+    nb, nk = mdesc.nbranch, mdesc.nkpoint
+    mfp = np.ascontiguousarray(np.asarray(mfp_nm, dtype=np.float64).reshape(-1, order="F"))
+    prob = np.ascontiguousarray(prob, dtype=np.float64)
+    alias = np.ascontiguousarray(alias, dtype=np.int32)
+    sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+    mp = np.zeros(nb * nk, dtype=np.int64)
+    mt = np.zeros(nb * nk, dtype=np.int64)
+    gt = np.zeros((ngrain_total, 2), dtype=np.float64)
+    lib().oracle_trace_alias_grains(ctypes.addressof(gdesc), ctypes.addressof(mdesc), _p(mfp),
+                                    _p(prob), _p(alias), _p(sc), int(seed), int(math), int(first),
+                                    int(n), _p(mp), _p(mt), _p(gt))
+    return mp.reshape((nb, nk), order="F"), mt.reshape((nb, nk), order="F"), gt
 
 
 def philox_draw(seed, path, draw):

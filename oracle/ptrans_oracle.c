@@ -62,6 +62,9 @@ typedef struct {
     double *score;
     int64_t nscat;
     double len;
+    /* per-grain tallies (oracle_trace_alias_grains): [2g] += w dx, [2g+1] += w l */
+    double *gtally;
+    double gw;
 } ostate;
 
 /* site = the line of the RANDOM_NUMBER call in src/RTMCphon.f90 that this draw mirrors */
@@ -337,6 +340,13 @@ static int o_determin_transmit(ostate *s, const pt_surface *sf, double rand_in, 
             threshold = t11 + om * (1 - t11);
         }
         break;
+    case PT_TRANS_LORENTZ: {
+        /* extension: t = 1/(1 + (omega/omega0)^2) for the path's mode */
+        double w = s->m->omega[(s->cbranch - 1) + (int64_t)s->m->nbranch * (s->ckpoint - 1)];
+        double r = w / sf->trans_param;
+        threshold = 1.0 / (1.0 + r * r);
+        break;
+    }
     default:
         threshold = 0.0;
     }
@@ -523,6 +533,10 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
             double ct, st, ph, sph, cph;
             distance = freepath;
             s->nscat++;
+            if (s->gtally) {
+                s->gtally[2 * which] += s->gw * (distance * dir[0]);
+                s->gtally[2 * which + 1] += s->gw * distance;
+            }
             for (k = 0; k < 3; ++k) pos[k] = pos[k] + distance * dir[k];
             ct = o_rand(s, 2197);
             ph = o_rand(s, 2198);
@@ -534,6 +548,10 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
             isurface = 0;
             igrain = 0;
         } else {
+            if (s->gtally) {
+                s->gtally[2 * which] += s->gw * (distance * dir[0]);
+                s->gtally[2 * which + 1] += s->gw * distance;
+            }
             for (k = 0; k < 3; ++k) pos[k] = pos[k] + distance * dir[k];
             if (igrain == 0 && isurface == 1) {
                 end_side = 1;
@@ -543,6 +561,12 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
                 /* no surface after a scattering: vgrain(0)%vsurface(0) (undefined) */
                 flags |= PT_PATH_LOST;
                 break;
+            } else if (igrain == 0 && isurface >= 3 && isurface <= 6 &&
+                       g->surfaces[isurface - 1].trans_kind == PT_TRANS_PERIODIC) {
+                /* extension: periodic side wall -> the opposite wall, same direction */
+                int a = (isurface - 1) / 2, plus = (isurface - 1) & 1;
+                pos[a] = plus ? pos[a] - g->box[a] : pos[a] + g->box[a];
+                isurface = ((isurface - 1) ^ 1) + 1;
             } else {
                 const pt_surface *sf = g->surfaces + g->grain_offset[igrain] + (isurface - 1);
                 double spec = o_rand(s, 2230);                    /* determinSpecular RT:2230 */
@@ -909,6 +933,44 @@ int oracle_adjoint_alias(const pt_geom_desc *g, const pt_modes_desc *m, const do
         acc[2] += x * x;
         for (k = 0; k < ngroup; ++k) acc[3 + k] += w * score[k];
         acc[3 + ngroup + ib] += w * ((double)s.nscat / sb - s.len / lam);
+    }
+    return 0;
+}
+
+/*
+ * Alias-mode trace of paths [first, first+n) with per-grain tallies: for every flight of
+ * a path of mode m inside grain g (whichV during the flight), gtally[2g] += dx / mfp_m
+ * and gtally[2g+1] += l / mfp_m.  Also the per-mode tallies of oracle_trace_alias.
+ */
+int oracle_trace_alias_grains(const pt_geom_desc *g, const pt_modes_desc *m,
+                              const double *mfp_nm, const double *prob, const int32_t *alias,
+                              const double *scale, uint64_t seed, int math, int64_t first,
+                              int64_t n, int64_t *mode_paths, int64_t *mode_trans,
+                              double *gtally)
+{
+    ostate s;
+    int64_t i, nm = (int64_t)m->nbranch * m->nkpoint;
+    memset(&s, 0, sizeof s);
+    s.g = g; s.m = m; s.math = math; s.seed = seed;
+    s.gtally = gtally;
+    for (i = 0; i < n; ++i) {
+        pt_path_record r;
+        int64_t mode;
+        int ib;
+        s.path = (uint64_t)(first + i);
+        s.draw = 0;
+        mode = o_sample_mode(&s, nm, prob, alias);
+        ib = (int)(mode % m->nbranch);
+        s.cbranch = ib + 1;
+        s.ckpoint = (int)(mode / m->nbranch) + 1;
+        s.mfp = scale ? mfp_nm[mode] / scale[ib] : mfp_nm[mode];
+        s.gw = 1.0 / mfp_nm[mode];
+        memset(&r, 0, sizeof r);
+        o_trace_particle(&s, 0.0, 0.0, 1, &r);
+        if (!(r.flags & PT_PATH_START_MISMATCH)) {
+            mode_paths[mode]++;
+            if (r.end_side != 1) mode_trans[mode]++;
+        }
     }
     return 0;
 }

@@ -293,8 +293,12 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
             return fail(PT_E_GEOMETRY, "surface %d: radius must be positive", i);
         }
         if (s.internal) {
-            if (s.trans_kind < PT_TRANS_VALUE || s.trans_kind > PT_TRANS_TBC)
+            if (s.trans_kind < PT_TRANS_VALUE || s.trans_kind > PT_TRANS_PERIODIC)
                 return fail(PT_E_GEOMETRY, "surface %d: unknown transmissivity law %d", i, s.trans_kind);
+            if (s.trans_kind == PT_TRANS_LORENTZ && !(s.trans_param > 0.0))
+                return fail(PT_E_GEOMETRY, "surface %d: Lorentzian omega0 must be positive", i);
+            if (s.trans_kind == PT_TRANS_PERIODIC && (i < 2 || i >= 6 || d->grain_offset[1] != 6))
+                return fail(PT_E_GEOMETRY, "surface %d: only box side walls can be periodic", i);
             if (s.grain_outside < 0 || s.grain_outside >= ng)
                 return fail(PT_E_GEOMETRY, "surface %d: grain_outside %d out of range", i, s.grain_outside);
         }
@@ -364,6 +368,38 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     g.ngrain_total = ng;
     g.nsurface = ns;
     g.box_aligned = box_is_axis_aligned(d) ? 1 : 0;
+    g.periodic = 0;
+    for (int a = 1; a < 3 && ns >= 6; ++a) {
+        const bool lo = d->surfaces[2 * a].trans_kind == PT_TRANS_PERIODIC;
+        const bool hi = d->surfaces[2 * a + 1].trans_kind == PT_TRANS_PERIODIC;
+        if (lo != hi) rc = fail(PT_E_GEOMETRY, "periodic walls come in pairs (axis %d)", a);
+        if (!lo) continue;
+        g.periodic = 1;
+        if (!g.box_aligned) rc = fail(PT_E_GEOMETRY, "periodic walls need the axis-aligned box");
+        if (d->kind_grain == PT_GRAIN_VORONOI && d->ngrain > 1)
+            rc = fail(PT_E_GEOMETRY, "periodic walls need a periodic tessellation: not with "
+                                     "several Voronoi grains");
+        for (int i = 6; i < ns && rc == PT_OK; ++i) {
+            const pt_surface &s = d->surfaces[i];
+            if (!s.internal) continue;
+            if (s.kind == PT_SHAPE_SPHERE &&
+                (s.center[a] - s.radius < 0.0 || s.center[a] + s.radius > d->box[a]))
+                rc = fail(PT_E_GEOMETRY, "sphere %d crosses a periodic wall", i);
+            if (s.kind == PT_SHAPE_CYLINDER && a == 1 &&
+                (s.center[1] - s.radius < 0.0 || s.center[1] + s.radius > d->box[1]))
+                rc = fail(PT_E_GEOMETRY, "cylinder %d crosses a periodic wall", i);
+            if (s.kind == PT_SHAPE_POLYGON)
+                for (int k = 0; k < s.nvertex; ++k) {
+                    const double v = d->vertices[3 * (s.vertex_offset + k) + a];
+                    if (v < 0.0 || v > d->box[a])
+                        rc = fail(PT_E_GEOMETRY, "polygon %d crosses a periodic wall", i);
+                }
+        }
+    }
+    if (rc != PT_OK) {
+        pt_geom_destroy(G);
+        return rc;
+    }
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         pt_geom_destroy(G);
@@ -622,6 +658,115 @@ int replay_common(pt_ctx *c, TraceParams &P, int mode, int64_t first, int64_t n,
     return PT_OK;
 }
 
+static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npaths,
+                          int32_t precision, const double *branch_scale,
+                          const int32_t *surface_group, int32_t ngroup, double baseline,
+                          int32_t nbatch, int64_t *mode_paths, int64_t *mode_trans,
+                          double *batch_sums, double *grain_tally, pt_stats *stats, const char *fn)
+{
+    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
+    if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
+    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
+    if (ngroup < 0 || ngroup > kAdjMaxGroups)
+        return fail(PT_E_UNSUPPORTED, "ngroup %d: at most %d specularity groups", ngroup, kAdjMaxGroups);
+    if (ngroup > 0 && !surface_group) return fail(PT_E_INVALID, "surface_group is NULL");
+    if (nbatch < 1 || nbatch > 4096) return fail(PT_E_INVALID, "nbatch must be in 1..4096");
+    if (!std::isfinite(baseline)) return fail(PT_E_INVALID, "baseline is not finite");
+    if (!mode_paths || !mode_trans)
+        return fail(PT_E_INVALID, "tally arrays are NULL");
+    const int ns = G->dg.nsurface;
+    std::vector<int32_t> grp(ns, -1);
+    for (int i = 0; i < ns && ngroup > 0; ++i) {
+        const int gi = surface_group[i];
+        if (gi < -1 || gi >= ngroup)
+            return fail(PT_E_INVALID, "surface_group[%d] = %d outside -1..%d", i, gi, ngroup - 1);
+        if (gi >= 0 && !(G->surf[i].p > 0.0 && G->surf[i].p < 1.0))
+            return fail(PT_E_INVALID, "surface %d of group %d has specularity %g: the score of a "
+                                      "specularity needs 0 < p < 1", i, gi, G->surf[i].p);
+        grp[i] = gi;
+    }
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (npaths == 0) return PT_OK;
+    const size_t nm = (size_t)M->nb * M->nk;
+    PT_HIP(hipSetDevice(c->device));
+    TraceParams P = base_params(c, G, M);
+    P.precision = precision;
+    int rc = set_scale(c, M, branch_scale, P);
+    if (rc) return rc;
+    if (branch_scale)
+        for (int b = 0; b < M->nb; ++b)
+            if (!(branch_scale[b] > 0.0)) return fail(PT_E_INVALID, "branch_scale[%d] must be > 0", b);
+    P.total = npaths;
+    P.ngroup = ngroup;
+    P.nbatch = nbatch;
+    P.baseline = baseline;
+    P.ngt = grain_tally ? G->dg.ngrain_total : 0;
+    const size_t nsum = (size_t)nbatch * adj_stride(P);
+    const size_t ntal = 2 * (size_t)P.ngt;
+    const size_t lds = adj_lds_bytes(P, c->threads);
+    hipDeviceProp_t prop;
+    PT_HIP(hipGetDeviceProperties(&prop, c->device));
+    if (lds > prop.sharedMemPerBlock)
+        return fail(PT_E_UNSUPPORTED, "adjoint needs %zu bytes of LDS per workgroup (limit %zu): "
+                                      "fewer groups or batches", lds, (size_t)prop.sharedMemPerBlock);
+    unsigned long long *d_tal = nullptr;
+    int32_t *d_grp = nullptr;
+    double *d_sum = nullptr, *d_gt = nullptr;
+    PT_HIP(hipMalloc((void **)&d_tal, 2 * nm * sizeof(unsigned long long)));
+    hipError_t e = hipMalloc((void **)&d_grp, (size_t)ns * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_sum, nsum * sizeof(double));
+    if (e == hipSuccess && ntal) e = hipMalloc((void **)&d_gt, ntal * sizeof(double));
+    std::vector<double> h_gt(ntal);
+    std::vector<unsigned long long> h_tal(2 * nm);
+    std::vector<double> h_sum(nsum);
+    unsigned long long host_ctr[16] = {0};
+    float ms = 0.f;
+    int blocks = 0;
+    do {
+        if (e) break;
+        P.mode_paths = d_tal;
+        P.mode_trans = d_tal + nm;
+        P.surf_group = d_grp;
+        P.adj = d_sum;
+        P.grain_tally = d_gt;
+        if ((e = launch_blocks(c, TRACE_ADJOINT, precision, lds, &blocks))) break;
+        if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(d_sum, 0, nsum * sizeof(double), c->stream))) break;
+        if (ntal && (e = hipMemsetAsync(d_gt, 0, ntal * sizeof(double), c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_ADJOINT, blocks, c->threads, c->count_ops, lds, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(h_sum.data(), d_sum, nsum * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
+        if (ntal && (e = hipMemcpyAsync(h_gt.data(), d_gt, ntal * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipStreamSynchronize(c->stream))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    for (void *p : {(void *)d_tal, (void *)d_grp, (void *)d_sum, (void *)d_gt})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(PT_E_HIP, "%s: %s", fn, hipGetErrorString(e));
+    for (size_t i = 0; i < nm; ++i) {
+        mode_paths[i] += (int64_t)h_tal[i];
+        mode_trans[i] += (int64_t)h_tal[nm + i];
+    }
+    if (batch_sums)
+        for (size_t i = 0; i < nsum; ++i) batch_sums[i] += h_sum[i];
+    for (size_t i = 0; i < ntal; ++i) grain_tally[i] += h_gt[i];
+    fill_stats(stats, host_ctr, ms, 0.f);
+    if ((int64_t)host_ctr[1] != npaths)
+        return fail(PT_E_HIP, "%s: %lld of %lld paths completed",
+                    fn, (long long)host_ctr[1], (long long)npaths);
+    return PT_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -814,97 +959,21 @@ int pt_adjoint_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npa
                      int32_t ngroup, double baseline, int32_t nbatch, int64_t *mode_paths,
                      int64_t *mode_trans, double *batch_sums, pt_stats *stats)
 {
-    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
-    if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
-    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
-    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
-        precision != PT_PREC_FP32_FILTER)
-        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
-    if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
-    if (ngroup < 0 || ngroup > kAdjMaxGroups)
-        return fail(PT_E_UNSUPPORTED, "ngroup %d: at most %d specularity groups", ngroup, kAdjMaxGroups);
-    if (ngroup > 0 && !surface_group) return fail(PT_E_INVALID, "surface_group is NULL");
-    if (nbatch < 1 || nbatch > 4096) return fail(PT_E_INVALID, "nbatch must be in 1..4096");
-    if (!std::isfinite(baseline)) return fail(PT_E_INVALID, "baseline is not finite");
-    if (!mode_paths || !mode_trans || !batch_sums)
-        return fail(PT_E_INVALID, "tally or batch arrays are NULL");
-    const int ns = G->dg.nsurface;
-    std::vector<int32_t> grp(ns, -1);
-    for (int i = 0; i < ns && ngroup > 0; ++i) {
-        const int gi = surface_group[i];
-        if (gi < -1 || gi >= ngroup)
-            return fail(PT_E_INVALID, "surface_group[%d] = %d outside -1..%d", i, gi, ngroup - 1);
-        if (gi >= 0 && !(G->surf[i].p > 0.0 && G->surf[i].p < 1.0))
-            return fail(PT_E_INVALID, "surface %d of group %d has specularity %g: the score of a "
-                                      "specularity needs 0 < p < 1", i, gi, G->surf[i].p);
-        grp[i] = gi;
-    }
-    if (stats) memset(stats, 0, sizeof *stats);
-    if (npaths == 0) return PT_OK;
-    const size_t nm = (size_t)M->nb * M->nk;
-    PT_HIP(hipSetDevice(c->device));
-    TraceParams P = base_params(c, G, M);
-    P.precision = precision;
-    int rc = set_scale(c, M, branch_scale, P);
-    if (rc) return rc;
-    if (branch_scale)
-        for (int b = 0; b < M->nb; ++b)
-            if (!(branch_scale[b] > 0.0)) return fail(PT_E_INVALID, "branch_scale[%d] must be > 0", b);
-    P.total = npaths;
-    P.ngroup = ngroup;
-    P.nbatch = nbatch;
-    P.baseline = baseline;
-    const size_t nsum = (size_t)nbatch * adj_stride(P);
-    const size_t lds = adj_lds_bytes(P, c->threads);
-    hipDeviceProp_t prop;
-    PT_HIP(hipGetDeviceProperties(&prop, c->device));
-    if (lds > prop.sharedMemPerBlock)
-        return fail(PT_E_UNSUPPORTED, "adjoint needs %zu bytes of LDS per workgroup (limit %zu): "
-                                      "fewer groups or batches", lds, (size_t)prop.sharedMemPerBlock);
-    unsigned long long *d_tal = nullptr;
-    int32_t *d_grp = nullptr;
-    double *d_sum = nullptr;
-    PT_HIP(hipMalloc((void **)&d_tal, 2 * nm * sizeof(unsigned long long)));
-    hipError_t e = hipMalloc((void **)&d_grp, (size_t)ns * sizeof(int32_t));
-    if (e == hipSuccess) e = hipMalloc((void **)&d_sum, nsum * sizeof(double));
-    std::vector<unsigned long long> h_tal(2 * nm);
-    std::vector<double> h_sum(nsum);
-    unsigned long long host_ctr[16] = {0};
-    float ms = 0.f;
-    int blocks = 0;
-    do {
-        if (e) break;
-        P.mode_paths = d_tal;
-        P.mode_trans = d_tal + nm;
-        P.surf_group = d_grp;
-        P.adj = d_sum;
-        if ((e = launch_blocks(c, TRACE_ADJOINT, precision, lds, &blocks))) break;
-        if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
-        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
-        if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
-        if ((e = hipMemsetAsync(d_sum, 0, nsum * sizeof(double), c->stream))) break;
-        if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_ADJOINT, blocks, c->threads, c->count_ops, lds, c->stream))) break;
-        if ((e = hipEventRecord(c->ev1, c->stream))) break;
-        if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipMemcpyAsync(h_sum.data(), d_sum, nsum * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipStreamSynchronize(c->stream))) break;
-        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    } while (0);
-    for (void *p : {(void *)d_tal, (void *)d_grp, (void *)d_sum})
-        if (p) (void)hipFree(p);
-    if (e != hipSuccess) return fail(PT_E_HIP, "pt_adjoint_alias: %s", hipGetErrorString(e));
-    for (size_t i = 0; i < nm; ++i) {
-        mode_paths[i] += (int64_t)h_tal[i];
-        mode_trans[i] += (int64_t)h_tal[nm + i];
-    }
-    for (size_t i = 0; i < nsum; ++i) batch_sums[i] += h_sum[i];
-    fill_stats(stats, host_ctr, ms, 0.f);
-    if ((int64_t)host_ctr[1] != npaths)
-        return fail(PT_E_HIP, "pt_adjoint_alias: %lld of %lld paths completed",
-                    (long long)host_ctr[1], (long long)npaths);
-    return PT_OK;
+    if (!batch_sums) return fail(PT_E_INVALID, "batch_sums is NULL");
+    return adjoint_common(c, G, M, npaths, precision, branch_scale, surface_group, ngroup,
+                          baseline, nbatch, mode_paths, mode_trans, batch_sums, nullptr, stats,
+                          "pt_adjoint_alias");
+}
+
+int pt_trace_alias_grains(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npaths,
+                          int32_t precision, const double *branch_scale, int64_t *mode_paths,
+                          int64_t *mode_trans, double *grain_tally, pt_stats *stats)
+{
+    if (!grain_tally) return fail(PT_E_INVALID, "grain_tally is NULL");
+    if (G && G->dg.ngrain_total > 4096) return fail(PT_E_UNSUPPORTED, "more than 4096 grains");
+    return adjoint_common(c, G, M, npaths, precision, branch_scale, nullptr, 0, 0.0, 1,
+                          mode_paths, mode_trans, nullptr, grain_tally, stats,
+                          "pt_trace_alias_grains");
 }
 
 }  // extern "C"

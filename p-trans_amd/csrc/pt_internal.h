@@ -47,6 +47,7 @@ struct DevGeom {
     int32_t ngrain_total;
     int32_t nsurface;
     int32_t box_aligned;       // grain 0 is the six axis-aligned walls of setupBoxGrain
+    int32_t periodic;          // some side walls are PT_TRANS_PERIODIC (box_aligned too)
 };
 
 struct DevModes {
@@ -100,12 +101,17 @@ struct TraceParams {
     int32_t nbatch;
     double baseline;
     double *adj;               // [nbatch * (3 + ngroup + nb)]
+    // per-grain tallies (adjoint mode, grain_tally != nullptr): per grain g the sums of
+    // dx / mfp_m and l / mfp_m over the flights inside g, [2 * g], [2 * g + 1]
+    double *grain_tally;
+    int32_t ngt;               // grains tallied (ngrain_total) or 0
 };
 
 __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
 inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 {
-    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p)) * sizeof(double);
+    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt) *
+           sizeof(double);
 }
 
 // kernels' launch wrappers (pt_trace.hip)

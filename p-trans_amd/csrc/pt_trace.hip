@@ -725,6 +725,12 @@ __device__ bool determin_transmit(const DSurf &s, int nmaterial, double omega, d
             rng.oc->flops += 5;
         }
         break;
+    case PT_TRANS_LORENTZ: {              // extension: 1/(1 + (omega/omega0)^2)
+        const double r = omega / s.tparam;
+        threshold = 1.0 / (1.0 + r * r);
+        rng.oc->flops += 4;
+        break;
+    }
     default:
         threshold = 0.0;
     }
@@ -880,6 +886,31 @@ struct AdjLane {
     const int32_t *group;
     int nscat;
     double len;
+    // per-grain tallies: the open grain's sums of dx and l (flushed, times fw = 1/mfp_m,
+    // to the workgroup's accumulators gacc when the path changes grain or ends)
+    double *gacc;
+    int fgrain;
+    double fdx, fl, fw;
+    __device__ void flush()
+    {
+        if (gacc && fgrain >= 0) {
+            atomicAdd(gacc + 2 * fgrain, fw * fdx);
+            atomicAdd(gacc + 2 * fgrain + 1, fw * fl);
+        }
+        fgrain = -1;
+        fdx = 0.0;
+        fl = 0.0;
+    }
+    __device__ void flight(int grain, double l, double dx)
+    {
+        if (!gacc) return;
+        if (grain != fgrain) {
+            flush();
+            fgrain = grain;
+        }
+        fdx += l * dx;
+        fl += l;
+    }
 };
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
@@ -931,7 +962,10 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
         distance = ph.fp;
         scattered = true;
-        if (kAdj) al.nscat++;
+        if (kAdj) {
+            al.nscat++;
+            al.flight(ph.which, distance, ph.dir[0]);
+        }
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
         if (kPrec == kPrecFast) {
@@ -954,6 +988,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         ph.cur = -1;
         oc.flops += 8 + kFlopsSincos;    // + 6 above
     } else {
+        if (kAdj) al.flight(ph.which, distance, ph.dir[0]);
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
         if (ph.cur == 0) {
@@ -964,6 +999,14 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             // no surface to reflect on: vgrain(0)%vsurface(0) in the reference (undefined)
             ph.flags |= PT_PATH_LOST;
             return true;
+        } else if (g.periodic && ph.cur < 6 && g.surf[ph.cur].tkind == PT_TRANS_PERIODIC) {
+            // extension: periodic side wall -> the opposite wall, same direction, no draw
+            const int w = ph.cur;
+            const bool plus = w & 1;
+            if (w < 4) ph.pos[1] = plus ? ph.pos[1] - g.box[1] : ph.pos[1] + g.box[1];
+            else ph.pos[2] = plus ? ph.pos[2] - g.box[2] : ph.pos[2] + g.box[2];
+            ph.cur = w ^ 1;
+            ph.br = -1.0f;
         } else {
             const DSurf s = g.surf[ph.cur];
             const double spec = rng.next();                                 // RT:2230
@@ -1191,8 +1234,11 @@ trace_kernel(TraceParams P)
     al.len = 0.0;
     double *adj_acc = adj_lds + (kAdj ? P.ngroup * blockDim.x : 0);
     const int adj_k = kAdj ? adj_stride(P) : 0;
+    al.gacc = (kAdj && P.grain_tally) ? adj_acc + P.nbatch * adj_k : nullptr;
+    al.fgrain = -1;
+    al.fdx = al.fl = al.fw = 0.0;
     if (kAdj) {
-        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k;
+        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
         for (int i = threadIdx.x; i < nlds; i += blockDim.x) adj_lds[i] = 0.0;
         __syncthreads();
     }
@@ -1230,6 +1276,7 @@ trace_kernel(TraceParams P)
                     al.len = 0.0;
                     if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
                         active = true;
+                        if (kAdj) al.fw = 1.0 / P.m.mfp[mode];
                     } else {
                         if (kMode == TRACE_GRID) P.outcome[pid] = OUT_MISMATCH;
                         st[0] += 1ull;
@@ -1270,6 +1317,7 @@ trace_kernel(TraceParams P)
                         al.score[gi * al.stride] = 0.0;
                     }
                     atomicAdd(acc + 3 + P.ngroup + ib, w * ((double)al.nscat / sb - al.len / lam));
+                    al.flush();
                 }
                 st[0] += 1ull;
                 st[256 * 1] += (unsigned long long)ph.events;
@@ -1286,6 +1334,9 @@ trace_kernel(TraceParams P)
         __syncthreads();
         for (int i = threadIdx.x; i < P.nbatch * adj_k; i += blockDim.x)
             if (adj_acc[i] != 0.0) atomicAdd(P.adj + i, adj_acc[i]);
+        if (al.gacc)
+            for (int i = threadIdx.x; i < 2 * P.ngt; i += blockDim.x)
+                if (al.gacc[i] != 0.0) atomicAdd(P.grain_tally + i, al.gacc[i]);
     }
     // stats slots: paths, events, draws, lost, mismatch, limit -> stats[0..5];
     // retests, fallbacks -> stats[10], stats[11]
@@ -1356,7 +1407,7 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
     rng.oc = &oc;
     rng.seed = P.seed;
     Path ph;
-    AdjLane al = {nullptr, 0, nullptr, 0, 0.0};
+    AdjLane al = {nullptr, 0, nullptr, 0, 0.0, nullptr, -1, 0.0, 0.0, 0.0};
     int mode = 0;
     if (begin_path<kMode, kPrec>(P, first + i, rng, ph, &mode)) {
         while (ph.end_side == 0) {

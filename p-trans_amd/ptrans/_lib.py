@@ -21,7 +21,7 @@ EXPORTS = (
     "pt_geom_create", "pt_geom_destroy", "pt_geom_set_specularity", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
     "pt_trace_grid", "pt_replay", "pt_alias_build", "pt_trace_alias", "pt_replay_alias",
-    "pt_adjoint_alias",
+    "pt_adjoint_alias", "pt_trace_alias_grains",
 )
 
 
@@ -83,6 +83,7 @@ def load():
     L.pt_replay_alias.argtypes = [vp, vp, vp, vp, i64, i64, i32, vp]
     L.pt_geom_set_specularity.argtypes = [vp, vp]
     L.pt_adjoint_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, i32, dbl, i32, vp, vp, vp, vp]
+    L.pt_trace_alias_grains.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

@@ -10,6 +10,7 @@ BALLISTIC_MFP = 1.0e10                 # RT:510, 1686
 
 SHAPE_POLYGON, SHAPE_SPHERE, SHAPE_CYLINDER = 1, 2, 3        # RT:72-74
 TRANS_VALUE, TRANS_GAMMA, TRANS_TBC = 1, 2, 3                # RT:77-79
+TRANS_LORENTZ, TRANS_PERIODIC = 4, 5                         # extensions (include/ptrans.h)
 MFP_MATTHIESSEN, MFP_SAMPLE = 1, 2                           # RT:127-128
 GRAIN_VORONOI, GRAIN_BRICKS = 1, 2                           # RT:131-132
 

@@ -395,3 +395,37 @@ def group_specularity(specularity, group, values):
     sel = group >= 0
     p[sel] = values[group[sel]]
     return p
+
+
+def set_periodic(geo: Geometry, axes=(1,)) -> Geometry:
+    """Make the box side walls of the given axes (1 = y, 2 = z) periodic (an extension:
+    PT_TRANS_PERIODIC, the reference's box walls always reflect).  Returns geo."""
+    from .constants import TRANS_PERIODIC
+    walls = geo.grains[0].surfaces
+    for a in axes:
+        if a not in (1, 2):
+            raise ValueError("only the y (1) and z (2) walls can be periodic")
+        for w in (2 * a, 2 * a + 1):
+            walls[w].trans_kind = TRANS_PERIODIC
+            walls[w].trans_param = 0.0
+    return geo
+
+
+def set_lorentz_boundaries(geo: Geometry, rng, f0_thz=(3.0, 10.0)) -> dict:
+    """Give every internal grain boundary (a facet shared by two grains; both copies) the
+    Lorentzian law t = 1/(1 + (omega/omega0)^2) with omega0 = 2 pi f0, f0 uniform in f0_thz
+    (an extension, PT_TRANS_LORENTZ).  Returns {(grain_a, grain_b): omega0}."""
+    from .constants import TRANS_LORENTZ
+    w0 = {}
+    for gi, g in enumerate(geo.grains):
+        if gi == 0:
+            continue
+        for s in g.surfaces:
+            if not s.internal or s.grain_outside <= 0:
+                continue
+            key = (min(gi, s.grain_outside), max(gi, s.grain_outside))
+            if key not in w0:
+                w0[key] = rng.uniform(*f0_thz) * 1.0e12 * 2.0 * math.pi
+            s.trans_kind = TRANS_LORENTZ
+            s.trans_param = w0[key]
+    return w0

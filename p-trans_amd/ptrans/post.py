@@ -315,3 +315,11 @@ def alias_gradients(batch_sums, length: float, ngroup: int) -> AliasGradient:
     grad, gse = _ratio_of_sums(s[:, 3:], n)
     return AliasGradient(ratio=float(c * ratio), stderr=float(c * se), grad=c * grad,
                          grad_stderr=c * gse, paths=int(n.sum()), ngroup=int(ngroup))
+
+
+def grain_kappa_shares(grain_tally, npaths: int):
+    """Per-grain shares of kappa/kappa_bulk from pt_trace_alias_grains tallies: a path's
+    x displacements add up to L T, so kappa/kappa_bulk = (3L/4) E[T/Lambda] =
+    (3/4) sum_g tally[g, 0] / npaths; share_g = (3/4) tally[g, 0] / npaths."""
+    t = np.asarray(grain_tally, dtype=np.float64)
+    return 0.75 * t[:, 0] / float(npaths)

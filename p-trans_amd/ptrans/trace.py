@@ -203,6 +203,27 @@ def _adjoint_alias(self, geom: Geom, modes: Modes, npaths: int, surface_group=No
 Context.adjoint_alias = _adjoint_alias
 
 
+def _trace_alias_grains(self, geom: Geom, modes: Modes, npaths: int, branch_scale=None,
+                        precision: int = PREC_FP64):
+    """pt_trace_alias_grains: per-mode tallies, per-grain (ngrain_total, 2) sums of
+    [dx / mfp_m, l / mfp_m] over the flights inside each grain, and the stats."""
+    nb, nk = modes.nbranch, modes.nkpoint
+    paths = np.zeros(nb * nk, dtype=np.int64)
+    trans = np.zeros(nb * nk, dtype=np.int64)
+    tally = np.zeros((len(geom.geometry.grains), 2), dtype=np.float64)
+    stats = _lib.PtStats()
+    sc = None if branch_scale is None else np.ascontiguousarray(branch_scale, dtype=np.float64)
+    _lib.check(_lib.load().pt_trace_alias_grains(
+        self.handle, geom.handle, modes.handle, int(npaths), int(precision),
+        sc.ctypes.data if sc is not None else None, paths.ctypes.data, trans.ctypes.data,
+        tally.ctypes.data, ctypes.byref(stats)))
+    return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"), tally,
+            stats.as_dict())
+
+
+Context.trace_alias_grains = _trace_alias_grains
+
+
 def alias_table(weights):
     """pt_alias_build on a flat weight vector: (prob, alias)."""
     w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
