@@ -68,6 +68,11 @@ extern "C" {
 /* grain kinds, src/RTMCphon.f90:131-132 */
 #define PT_GRAIN_VORONOI 1
 #define PT_GRAIN_BRICKS  2
+/* extension: grains that tile the box with triangulated facets (a jittered lattice);
+   the start grain is the grain of the incident facet containing the start point
+   (the rule of findIncidentCollisionGrain, RT:2131-2163, without a fill grain) and
+   there is no isInsideGrain check (RT:1673-1682), which needs convex grains */
+#define PT_GRAIN_MESH    3
 
 /* arithmetic of the trace kernel */
 #define PT_PREC_FP64 0           /* every step in FP64 (the reference's arithmetic)      */
@@ -319,6 +324,42 @@ int pt_trace_alias_grains(pt_ctx *ctx, const pt_geom *geom, const pt_modes *mode
                           int64_t npaths, int32_t precision, const double *branch_scale,
                           int64_t *mode_paths, int64_t *mode_trans, double *grain_tally,
                           pt_stats *stats);
+
+/* ---- batches of mesh structures (an extension) ------------------------------------ */
+/*
+ * Many PT_GRAIN_MESH structures traced in one persistent launch: the structures share
+ * the box (size, scopes, wall laws and materials) and are merged into one geometry with
+ * disjoint grain sets; structure s owns paths [sum_{r<s} npaths[r], ... + npaths[s]) of
+ * the alias-mode numbering (Philox stream = path id), so each structure's results equal
+ * those of pt_trace_alias on its own geometry with those path ids.  Each grain gets a
+ * BVH over its candidate surfaces (leaf_size prims per leaf at most) that replaces the
+ * linear scan of findCollisionSurface (RT:1989-2096) with the same result; each structure
+ * gets a grid x grid locator of its incident facets for the start grain.
+ */
+typedef struct pt_batch pt_batch;
+int pt_batch_create(pt_ctx *ctx, int32_t nstruct, const pt_geom_desc *const *descs,
+                    const int64_t *npaths, int32_t leaf_size, int32_t grid, pt_batch **out);
+int pt_batch_destroy(pt_batch *batch);
+/* info: [surfaces, grains, BVH nodes, BVH prims, BVH depth, paths] of the merged batch */
+int pt_batch_info(const pt_batch *batch, int64_t *info);
+/* every facet of structure s gets specularity mean[s] (0 < mean[s] < 1); box walls keep
+ * theirs.  The gradient of pt_batch_trace is with respect to these values. */
+int pt_batch_set_specularity(pt_batch *batch, const double *mean);
+/*
+ * One launch over the whole batch with the context's seed.  Per structure s, adds to
+ * sums[4 s .. 4 s + 3]: paths, sum x, sum x^2 and sum (x - baseline[s]) * score_s with
+ * x = T / mfp_m (kappa/kappa_bulk = (3L/4) E[x]) and score_s the likelihood-ratio score
+ * of the structure's mean specularity (1/p per specular, -1/(1-p) per diffuse decision on
+ * its facets; baseline may be NULL for 0).  finish_ms[s]: time from the kernel's start to
+ * the end of the structure's last path (ms), or NULL.  precision: PT_PREC_*.
+ */
+int pt_batch_trace(pt_ctx *ctx, pt_batch *batch, const pt_modes *modes, int32_t precision,
+                   const double *baseline, double *sums, double *finish_ms, pt_stats *stats);
+/* As pt_batch_trace, through a hipGraph of (zero fill, trace) built on the first call for
+ * these modes and precision and replayed afterwards with the context's current seed. */
+int pt_batch_trace_graph(pt_ctx *ctx, pt_batch *batch, const pt_modes *modes,
+                         int32_t precision, const double *baseline, double *sums,
+                         double *finish_ms, pt_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -467,12 +467,14 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
     which = 0;
     if (g->ngrain > 0 && g->kind_grain == PT_GRAIN_VORONOI)
         which = o_nearest_incident_grain(g, pos);
-    else if (g->ngrain > 0 && g->kind_grain == PT_GRAIN_BRICKS)
-        which = o_incident_collision_grain(g, pos);
+    else if (g->ngrain > 0 && (g->kind_grain == PT_GRAIN_BRICKS || g->kind_grain == PT_GRAIN_MESH))
+        which = o_incident_collision_grain(g, pos);   /* mesh: igrain_fill = -1 if none */
     rec->start_grain = which;
 
-    /* verbose=1 consistency check (main.f90:14, RT:1673-1682) */
-    tmpwhich = o_find_inside_which_grain(g, pos);
+    /* verbose=1 consistency check (main.f90:14, RT:1673-1682); mesh grains (an
+     * extension) have none: only a start point on no incident facet fails */
+    tmpwhich = g->kind_grain == PT_GRAIN_MESH ? (which >= 0 ? which : -2)
+                                              : o_find_inside_which_grain(g, pos);
     if (tmpwhich != which) {
         rec->end_side = end_side;
         rec->flags = PT_PATH_START_MISMATCH;

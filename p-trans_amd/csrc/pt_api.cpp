@@ -257,8 +257,9 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     if (!c || !d || !out) return fail(PT_E_INVALID, "pt_geom_create: NULL argument");
     *out = nullptr;
     const int ng = d->ngrain_total, ns = d->nsurface;
-    if (d->kind_grain != PT_GRAIN_VORONOI && d->kind_grain != PT_GRAIN_BRICKS)
-        return fail(PT_E_GEOMETRY, "kind_grain %d is neither voronoi nor bricks", d->kind_grain);
+    if (d->kind_grain != PT_GRAIN_VORONOI && d->kind_grain != PT_GRAIN_BRICKS &&
+        d->kind_grain != PT_GRAIN_MESH)
+        return fail(PT_E_GEOMETRY, "kind_grain %d is not voronoi, bricks or mesh", d->kind_grain);
     const int expect = d->kind_grain == PT_GRAIN_BRICKS ? d->ngrain + 2 : d->ngrain + 1;
     if (d->ngrain < 0 || ng != expect)
         return fail(PT_E_GEOMETRY, "ngrain_total %d, expected %d for ngrain %d", ng, expect, d->ngrain);
@@ -311,6 +312,7 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
         o.voff = (int32_t)s.vertex_offset;
         o.gout = s.grain_outside;
         o.tkind = s.trans_kind;
+        o.gin = s.grain_inside;
         memcpy(&tang[9 * (size_t)i], s.tangent, 9 * sizeof(double));
     }
     // box walls x- and x+ must be the first two surfaces (EndSide, RT:1748-1753)
@@ -974,6 +976,513 @@ int pt_trace_alias_grains(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
     return adjoint_common(c, G, M, npaths, precision, branch_scale, nullptr, 0, 0.0, 1,
                           mode_paths, mode_trans, nullptr, grain_tally, stats,
                           "pt_trace_alias_grains");
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Batches of mesh structures (an extension): one merged geometry, one persistent launch.
+// =====================================================================================
+
+struct pt_batch {
+    pt_ctx *ctx = nullptr;
+    pt_geom *geom = nullptr;               // the merged geometry
+    int32_t nstruct = 0, grid = 0, leaf = 0, depth = 0;
+    int64_t npaths = 0;
+    std::vector<int64_t> path_off;
+    std::vector<int32_t> struct_surf_begin; // merged surface range of each structure
+    std::vector<void *> owned;
+    DevBatch bt{};
+    int32_t *d_group = nullptr;            // surface -> 0 (structure facets) or -1 (box)
+    // scratch zeroed by every launch: counters(16) | sums(4n) | finish(n) | start(1) |
+    // done(ceil(n/2)) | mode tallies (2 nm)
+    unsigned long long *scratch = nullptr;
+    size_t scratch_words = 0, nm = 0;
+    int64_t nodes = 0, prims = 0;
+    // graph of (fill, trace) for one (modes, precision)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t trace_node = nullptr;
+    const pt_modes *graph_modes = nullptr;
+    int graph_prec = -1;
+    TraceParams graph_params{};
+    hipKernelNodeParams graph_knp{};
+    void *graph_args[1] = {nullptr};
+};
+
+namespace {
+
+struct BvhPrim {
+    double lo[3], hi[3], c[3];
+    int idx;
+};
+
+float round_down(double v) { float f = (float)v; if ((double)f > v) f = nextafterf(f, -3.0e38f); return f; }
+float round_up(double v) { float f = (float)v; if ((double)f < v) f = nextafterf(f, 3.0e38f); return f; }
+
+// Median-split BVH over prims[b, e); returns the node index.  Leaves hold at most
+// `leaf` prims unless the depth limit forces a larger leaf.
+int bvh_build(std::vector<BvhPrim> &P, int b, int e, int depth, int max_depth, int leaf,
+              std::vector<BvhNode> &nodes, std::vector<int32_t> &prims, int *depth_out)
+{
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+    for (int i = b; i < e; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], P[i].lo[k]);
+            hi[k] = std::max(hi[k], P[i].hi[k]);
+            clo[k] = std::min(clo[k], P[i].c[k]);
+            chi[k] = std::max(chi[k], P[i].c[k]);
+        }
+    const int ni = (int)nodes.size();
+    nodes.push_back(BvhNode{});
+    for (int k = 0; k < 3; ++k) {
+        // outward padding: 1e-3 nm + 1e-6 relative covers FP32 rounding of positions
+        const float l = round_down(lo[k]), h = round_up(hi[k]);
+        nodes[ni].lo[k] = l - (1.0e-3f + 1.0e-6f * fabsf(l));
+        nodes[ni].hi[k] = h + (1.0e-3f + 1.0e-6f * fabsf(h));
+    }
+    *depth_out = std::max(*depth_out, depth);
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+        if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+    if (e - b <= leaf || depth >= max_depth || !(chi[axis] > clo[axis])) {
+        nodes[ni].a = -(int32_t)prims.size() - 1;
+        nodes[ni].b = e - b;
+        for (int i = b; i < e; ++i) prims.push_back(P[i].idx);
+        return ni;
+    }
+    const int m = (b + e) / 2;
+    std::nth_element(P.begin() + b, P.begin() + m, P.begin() + e,
+                     [axis](const BvhPrim &x, const BvhPrim &y) { return x.c[axis] < y.c[axis]; });
+    const int left = bvh_build(P, b, m, depth + 1, max_depth, leaf, nodes, prims, depth_out);
+    const int right = bvh_build(P, m, e, depth + 1, max_depth, leaf, nodes, prims, depth_out);
+    nodes[ni].a = left;
+    nodes[ni].b = right;
+    return ni;
+}
+
+void prim_bounds(const pt_geom_desc &d, const pt_surface &s, BvhPrim &p)
+{
+    if (s.kind == PT_SHAPE_POLYGON) {
+        for (int k = 0; k < 3; ++k) { p.lo[k] = 1e300; p.hi[k] = -1e300; }
+        for (int v = 0; v < s.nvertex; ++v)
+            for (int k = 0; k < 3; ++k) {
+                const double x = d.vertices[3 * (s.vertex_offset + v) + k];
+                p.lo[k] = std::min(p.lo[k], x);
+                p.hi[k] = std::max(p.hi[k], x);
+            }
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            p.lo[k] = s.center[k] - s.radius;
+            p.hi[k] = s.center[k] + s.radius;
+        }
+        if (s.kind == PT_SHAPE_CYLINDER) { p.lo[2] = 0.0; p.hi[2] = d.box[2]; }
+    }
+    for (int k = 0; k < 3; ++k) p.c[k] = 0.5 * (p.lo[k] + p.hi[k]);
+}
+
+bool same_walls(const pt_geom_desc &a, const pt_geom_desc &b)
+{
+    for (int k = 0; k < 3; ++k)
+        if (a.box[k] != b.box[k]) return false;
+    if (a.scope[0] != b.scope[0] || a.scope[1] != b.scope[1] || a.nmaterial != b.nmaterial)
+        return false;
+    for (int w = 0; w < 6; ++w) {
+        const pt_surface &x = a.surfaces[w], &y = b.surfaces[w];
+        if (x.specularity != y.specularity || x.trans_kind != y.trans_kind ||
+            x.trans_param != y.trans_param)
+            return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs,
+                    const int64_t *npaths, int32_t leaf_size, int32_t grid, pt_batch **out)
+{
+    if (!c || !descs || !npaths || !out) return fail(PT_E_INVALID, "pt_batch_create: NULL argument");
+    *out = nullptr;
+    if (nstruct < 1 || nstruct > 4096) return fail(PT_E_INVALID, "nstruct must be in 1..4096");
+    if (leaf_size < 1 || leaf_size > 64) return fail(PT_E_INVALID, "leaf_size must be in 1..64");
+    if (grid < 1 || grid > 1024) return fail(PT_E_INVALID, "grid must be in 1..1024");
+    if (c->threads != 256) return fail(PT_E_UNSUPPORTED, "batch traces run 256-lane workgroups");
+    const pt_geom_desc &d0 = *descs[0];
+    int64_t ntot_paths = 0;
+    size_t ns = 6, nv = 0;
+    int ng = 0;
+    for (int s = 0; s < nstruct; ++s) {
+        const pt_geom_desc &d = *descs[s];
+        if (d.kind_grain != PT_GRAIN_MESH) return fail(PT_E_UNSUPPORTED, "structure %d is not a mesh", s);
+        if (d.ngrain_total != d.ngrain + 1 || d.grain_offset[1] != 6)
+            return fail(PT_E_GEOMETRY, "structure %d: grain 0 must be the six box walls", s);
+        if (!same_walls(d0, d))
+            return fail(PT_E_GEOMETRY, "structure %d: box, scope, materials and walls must be the "
+                                       "batch's", s);
+        if (npaths[s] < 0 || ntot_paths > ((int64_t)1 << 53) - npaths[s])
+            return fail(PT_E_INVALID, "bad path count of structure %d", s);
+        ntot_paths += npaths[s];
+        ns += (size_t)(d.nsurface - 6);
+        nv += (size_t)d.nvertex;
+        ng += d.ngrain;
+    }
+    // merged descriptor: shared box grain, then every structure's grains
+    std::vector<pt_surface> surf(ns);
+    std::vector<double> vtx;
+    vtx.reserve(3 * nv + 3 * 4);
+    std::vector<int32_t> goff(1, 0), ginc(1, d0.grain_incident[0]);
+    std::vector<double> gen(d0.generatrix, d0.generatrix + 3);
+    for (int w = 0; w < 6; ++w) {
+        surf[w] = d0.surfaces[w];
+        if (surf[w].kind == PT_SHAPE_POLYGON) {
+            const int64_t vo = (int64_t)vtx.size() / 3;
+            for (int i = 0; i < 3 * surf[w].nvertex; ++i)
+                vtx.push_back(d0.vertices[3 * surf[w].vertex_offset + i]);
+            surf[w].vertex_offset = vo;
+        }
+    }
+    goff.push_back(6);
+    pt_batch *B = new pt_batch();
+    B->ctx = c;
+    B->nstruct = nstruct;
+    B->grid = grid;
+    B->leaf = leaf_size;
+    B->npaths = ntot_paths;
+    B->path_off.assign(nstruct + 1, 0);
+    B->struct_surf_begin.assign(nstruct + 1, 6);
+    size_t k = 6;
+    int gbase = 1;
+    for (int s = 0; s < nstruct; ++s) {
+        const pt_geom_desc &d = *descs[s];
+        B->path_off[s + 1] = B->path_off[s] + npaths[s];
+        B->struct_surf_begin[s] = (int32_t)k;
+        const int64_t vbase = (int64_t)vtx.size() / 3;
+        vtx.insert(vtx.end(), d.vertices, d.vertices + 3 * d.nvertex);
+        auto remap = [gbase](int x) { return x >= 1 ? gbase + x - 1 : x; };
+        for (int g = 1; g <= d.ngrain; ++g) {
+            for (int i = d.grain_offset[g]; i < d.grain_offset[g + 1]; ++i) {
+                pt_surface t = d.surfaces[i];
+                t.grain_inside = remap(t.grain_inside);
+                t.grain_outside = remap(t.grain_outside);
+                if (t.kind == PT_SHAPE_POLYGON) t.vertex_offset += vbase;
+                surf[k++] = t;
+            }
+            goff.push_back((int32_t)k);
+            ginc.push_back(d.grain_incident[g]);
+            gen.insert(gen.end(), d.generatrix + 3 * g, d.generatrix + 3 * g + 3);
+        }
+        gbase += d.ngrain;
+    }
+    B->struct_surf_begin[nstruct] = (int32_t)k;
+    pt_geom_desc M = d0;
+    M.ngrain = ng;
+    M.ngrain_total = ng + 1;
+    M.igrain_fill = -1;
+    M.nsurface = (int32_t)ns;
+    M.nvertex = (int64_t)vtx.size() / 3;
+    M.grain_offset = goff.data();
+    M.grain_incident = ginc.data();
+    M.generatrix = gen.data();
+    M.surfaces = surf.data();
+    M.vertices = vtx.data();
+    int rc = pt_geom_create(c, &M, &B->geom);
+    if (rc) { delete B; return rc; }
+
+    // BVH per grain over its candidate (internal) surfaces
+    std::vector<BvhNode> nodes;
+    std::vector<int32_t> prims, root(ng + 1, -1);
+    for (int g = 1; g <= ng; ++g) {
+        std::vector<BvhPrim> P;
+        for (int i = goff[g]; i < goff[g + 1]; ++i) {
+            if (!surf[i].internal) continue;
+            BvhPrim p;
+            prim_bounds(M, surf[i], p);
+            p.idx = i;
+            P.push_back(p);
+        }
+        if (P.empty()) continue;
+        int dep = 0;
+        root[g] = bvh_build(P, 0, (int)P.size(), 0, kBvhStack - 2, leaf_size, nodes, prims, &dep);
+        B->depth = std::max(B->depth, dep);
+    }
+    B->nodes = (int64_t)nodes.size();
+    B->prims = (int64_t)prims.size();
+    // start locator: per structure, incident polygons overlapping each (y, z) cell
+    std::vector<int32_t> loc_off(1, 0), loc_idx;
+    for (int s = 0; s < nstruct; ++s) {
+        std::vector<std::vector<int32_t>> cells((size_t)grid * grid);
+        for (int i = B->struct_surf_begin[s]; i < B->struct_surf_begin[s + 1]; ++i) {
+            const pt_surface &t = surf[i];
+            if (!t.incident || t.kind != PT_SHAPE_POLYGON) continue;
+            BvhPrim p;
+            prim_bounds(M, t, p);
+            int y0 = (int)std::floor(p.lo[1] / M.box[1] * grid) - 1, y1 = (int)std::floor(p.hi[1] / M.box[1] * grid) + 1;
+            int z0 = (int)std::floor(p.lo[2] / M.box[2] * grid) - 1, z1 = (int)std::floor(p.hi[2] / M.box[2] * grid) + 1;
+            y0 = std::max(y0, 0); z0 = std::max(z0, 0);
+            y1 = std::min(y1, grid - 1); z1 = std::min(z1, grid - 1);
+            for (int y = y0; y <= y1; ++y)
+                for (int z = z0; z <= z1; ++z) cells[(size_t)y * grid + z].push_back(i);
+        }
+        for (auto &cl : cells) {
+            loc_idx.insert(loc_idx.end(), cl.begin(), cl.end());
+            loc_off.push_back((int32_t)loc_idx.size());
+        }
+    }
+    std::vector<int32_t> group(ns, -1);
+    for (size_t i = 6; i < ns; ++i) group[i] = 0;
+    B->nm = 0;
+    const size_t n = (size_t)nstruct;
+    B->scratch_words = 16 + 4 * n + n + 1 + (n + 1) / 2;
+    hipStream_t st = c->stream;
+    int64_t *d_poff = nullptr;
+    int32_t *d_root = nullptr, *d_prims = nullptr, *d_loff = nullptr, *d_lidx = nullptr;
+    BvhNode *d_nodes = nullptr;
+    double *d_base = nullptr;
+    std::vector<double> zeros(n, 0.0);
+    if ((rc = upload(B->path_off, &d_poff, st)) || (rc = upload(root, &d_root, st)) ||
+        (rc = upload(nodes, &d_nodes, st)) || (rc = upload(prims, &d_prims, st)) ||
+        (rc = upload(loc_off, &d_loff, st)) || (rc = upload(loc_idx, &d_lidx, st)) ||
+        (rc = upload(group, &B->d_group, st)) || (rc = upload(zeros, &d_base, st))) {
+        B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base};
+        pt_batch_destroy(B);
+        return rc;
+    }
+    B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base};
+    DevBatch &bt = B->bt;
+    bt.nstruct = nstruct;
+    bt.grid = grid;
+    bt.path_off = d_poff;
+    bt.grain_root = d_root;
+    bt.nodes = d_nodes;
+    bt.prims = d_prims;
+    bt.loc_off = d_loff;
+    bt.loc_idx = d_lidx;
+    bt.baseline = d_base;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        pt_batch_destroy(B);
+        return fail(PT_E_HIP, "pt_batch_create: %s", hipGetErrorString(e));
+    }
+    *out = B;
+    return PT_OK;
+}
+
+int pt_batch_destroy(pt_batch *B)
+{
+    if (!B) return PT_OK;
+    (void)hipSetDevice(B->ctx->device);
+    (void)hipStreamSynchronize(B->ctx->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    for (void *p : B->owned)
+        if (p) (void)hipFree(p);
+    if (B->scratch) (void)hipFree(B->scratch);
+    if (B->geom) pt_geom_destroy(B->geom);
+    delete B;
+    return PT_OK;
+}
+
+int pt_batch_info(const pt_batch *B, int64_t *info)
+{
+    if (!B || !info) return fail(PT_E_INVALID, "pt_batch_info: NULL argument");
+    info[0] = B->geom->dg.nsurface;
+    info[1] = B->geom->dg.ngrain_total;
+    info[2] = B->nodes;
+    info[3] = B->prims;
+    info[4] = B->depth;
+    info[5] = B->npaths;
+    return PT_OK;
+}
+
+int pt_batch_set_specularity(pt_batch *B, const double *p)
+{
+    if (!B || !p) return fail(PT_E_INVALID, "pt_batch_set_specularity: NULL argument");
+    std::vector<double> all(B->geom->dg.nsurface);
+    for (int i = 0; i < 6; ++i) all[i] = B->geom->surf[i].p;
+    for (int s = 0; s < B->nstruct; ++s) {
+        if (!(p[s] > 0.0 && p[s] < 1.0))
+            return fail(PT_E_INVALID, "mean specularity of structure %d is %g: need 0 < p < 1", s, p[s]);
+        for (int i = B->struct_surf_begin[s]; i < B->struct_surf_begin[s + 1]; ++i) all[i] = p[s];
+    }
+    return pt_geom_set_specularity(B->geom, all.data());
+}
+
+}  // extern "C"
+
+namespace {
+
+int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P, size_t *lds,
+                  int *blocks)
+{
+    pt_ctx *c = B->ctx;
+    if (!M || M->ctx != c) return fail(PT_E_INVALID, "modes belong to another context");
+    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    const size_t nm = (size_t)M->nb * M->nk;
+    const size_t need = 16 + 4 * (size_t)B->nstruct + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm;
+    if (B->scratch_words < need || !B->scratch) {
+        if (B->scratch) PT_HIP(hipFree(B->scratch));
+        B->scratch = nullptr;
+        PT_HIP(hipMalloc((void **)&B->scratch, need * sizeof(unsigned long long)));
+        B->scratch_words = need;
+        if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+        if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
+    }
+    B->nm = nm;
+    unsigned long long *w = B->scratch;
+    P = base_params(c, B->geom, M);
+    P.precision = precision;
+    P.queue = w;
+    P.stats = w + 1;
+    P.total = B->npaths;
+    P.ngroup = 1;
+    P.nbatch = 1;
+    P.surf_group = B->d_group;
+    P.baseline = 0.0;
+    P.m.scale = nullptr;
+    DevBatch bt = B->bt;
+    bt.sums = (double *)(w + 16);
+    bt.finish = w + 16 + 4 * B->nstruct;
+    bt.start = w + 16 + 5 * B->nstruct;
+    bt.done = (unsigned int *)(w + 16 + 5 * B->nstruct + 1);
+    P.bt = bt;
+    P.mode_paths = w + 16 + 5 * B->nstruct + 1 + (B->nstruct + 1) / 2;
+    P.mode_trans = P.mode_paths + nm;
+    P.adj = nullptr;
+    *lds = adj_lds_bytes(P, c->threads);
+    hipDeviceProp_t prop;
+    PT_HIP(hipGetDeviceProperties(&prop, c->device));
+    if (*lds > prop.sharedMemPerBlock)
+        return fail(PT_E_UNSUPPORTED, "batch of %d structures needs %zu bytes of LDS per workgroup",
+                    B->nstruct, *lds);
+    PT_HIP(launch_blocks(c, TRACE_BATCH, precision, *lds, blocks));
+    return PT_OK;
+}
+
+int batch_finish(pt_batch *B, double *sums, double *finish_ms, pt_stats *stats, float ms)
+{
+    pt_ctx *c = B->ctx;
+    const size_t n = (size_t)B->nstruct;
+    std::vector<unsigned long long> h(16 + 5 * n + 1);
+    PT_HIP(hipMemcpyAsync(h.data(), B->scratch, h.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
+    PT_HIP(hipStreamSynchronize(c->stream));
+    const double *hs = (const double *)(h.data() + 16);
+    if (sums)
+        for (size_t i = 0; i < 4 * n; ++i) sums[i] += hs[i];
+    const unsigned long long t0 = h[16 + 5 * n];
+    if (finish_ms)
+        for (size_t s = 0; s < n; ++s) {
+            const unsigned long long t = h[16 + 4 * n + s];
+            // wall_clock64 ticks at 100 MHz on gfx9
+            finish_ms[s] = t >= t0 && t ? (double)(t - t0) * 1.0e-5 : -1.0;
+        }
+    fill_stats(stats, h.data(), ms, 0.f);
+    if ((int64_t)h[1] != B->npaths)
+        return fail(PT_E_HIP, "batch trace: %lld of %lld paths completed", (long long)h[1],
+                    (long long)B->npaths);
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_batch_trace(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t precision,
+                   const double *baseline, double *sums, double *finish_ms, pt_stats *stats)
+{
+    if (!c || !B || B->ctx != c) return fail(PT_E_INVALID, "pt_batch_trace: bad context or batch");
+    PT_HIP(hipSetDevice(c->device));
+    TraceParams P;
+    size_t lds = 0;
+    int blocks = 0;
+    int rc = batch_prepare(B, M, precision, P, &lds, &blocks);
+    if (rc) return rc;
+    if (stats) memset(stats, 0, sizeof *stats);
+    std::vector<double> base(B->nstruct, 0.0);
+    if (baseline) base.assign(baseline, baseline + B->nstruct);
+    float ms = 0.f;
+    hipError_t e;
+    do {
+        if ((e = hipMemcpyAsync((void *)B->bt.baseline, base.data(), base.size() * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = launch_fill(B->scratch, B->scratch_words, c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_BATCH, blocks, c->threads, false, lds, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipEventSynchronize(c->ev1))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_batch_trace: %s", hipGetErrorString(e));
+    return batch_finish(B, sums, finish_ms, stats, ms);
+}
+
+// The (zero-fill, trace) pair as a hipGraph: built once per (modes, precision),
+// replayed with the context's current seed (the trace node's parameters are updated).
+int pt_batch_trace_graph(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t precision,
+                         const double *baseline, double *sums, double *finish_ms, pt_stats *stats)
+{
+    if (!c || !B || B->ctx != c) return fail(PT_E_INVALID, "pt_batch_trace_graph: bad context or batch");
+    PT_HIP(hipSetDevice(c->device));
+    TraceParams P;
+    size_t lds = 0;
+    int blocks = 0;
+    int rc = batch_prepare(B, M, precision, P, &lds, &blocks);
+    if (rc) return rc;
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (!B->exec || B->graph_modes != M || B->graph_prec != precision) {
+        if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+        if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
+        PT_HIP(hipGraphCreate(&B->graph, 0));
+        hipKernelNodeParams fk{};
+        unsigned long long *fill_ptr = B->scratch;
+        size_t fill_n = B->scratch_words;
+        void *fargs[2] = {(void *)&fill_ptr, (void *)&fill_n};      // copied by the node
+        fk.func = (void *)fill_kernel_fn();
+        fk.gridDim = dim3(1024);
+        fk.blockDim = dim3(256);
+        fk.sharedMemBytes = 0;
+        fk.kernelParams = fargs;
+        fk.extra = nullptr;
+        hipGraphNode_t fnode = nullptr;
+        PT_HIP(hipGraphAddKernelNode(&fnode, B->graph, nullptr, 0, &fk));
+        B->graph_params = P;
+        B->graph_args[0] = (void *)&B->graph_params;
+        hipKernelNodeParams &k = B->graph_knp;
+        k.func = (void *)trace_kernel_fn(TRACE_BATCH, precision);
+        k.gridDim = dim3(blocks);
+        k.blockDim = dim3(c->threads);
+        k.sharedMemBytes = (unsigned)lds;
+        k.kernelParams = B->graph_args;
+        k.extra = nullptr;
+        PT_HIP(hipGraphAddKernelNode(&B->trace_node, B->graph, &fnode, 1, &k));
+        PT_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        B->graph_modes = M;
+        B->graph_prec = precision;
+    }
+    B->graph_params = P;                       // the current seed and tuning
+    B->graph_args[0] = (void *)&B->graph_params;
+    PT_HIP(hipGraphExecKernelNodeSetParams(B->exec, B->trace_node, &B->graph_knp));
+    std::vector<double> base(B->nstruct, 0.0);
+    if (baseline) base.assign(baseline, baseline + B->nstruct);
+    float ms = 0.f;
+    hipError_t e;
+    do {
+        if ((e = hipMemcpyAsync((void *)B->bt.baseline, base.data(), base.size() * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = hipGraphLaunch(B->exec, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipEventSynchronize(c->ev1))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_batch_trace_graph: %s", hipGetErrorString(e));
+    return batch_finish(B, sums, finish_ms, stats, ms);
 }
 
 }  // extern "C"

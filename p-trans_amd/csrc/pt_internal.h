@@ -21,7 +21,8 @@ struct alignas(16) DSurf {
     int32_t voff;       // first vertex in the vertex pool
     int32_t gout;       // grain_outside
     int32_t tkind;      // PT_TRANS_*
-    int32_t pad[3];
+    int32_t gin;        // grain_inside
+    int32_t pad[2];
 };
 static_assert(sizeof(DSurf) == 112, "DSurf layout");
 
@@ -66,8 +67,39 @@ enum : uint8_t {
     OUT_MISMATCH = 2,          // start-grain check failed (RT:1673-1682)
 };
 
-// TRACE_ADJOINT: the alias-mode trace plus likelihood-ratio scores of each path
-enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2 };
+// TRACE_ADJOINT: the alias-mode trace plus likelihood-ratio scores of each path;
+// TRACE_BATCH: the adjoint-mode trace over a batch of mesh structures (BVH scans)
+enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3 };
+
+// Bounding volume hierarchy node: FP32 bounds padded outward on the host (so FP32 slab
+// tests never drop a true hit); a >= 0: children a and b; a < 0: leaf of prims
+// [-a - 1, -a - 1 + b).  32 bytes.
+struct BvhNode {
+    float lo[3];
+    float hi[3];
+    int32_t a, b;
+};
+static_assert(sizeof(BvhNode) == 32, "BvhNode layout");
+constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
+
+// A batch of structures merged into one mesh geometry (shared box grain, disjoint grain
+// sets): per structure its path range, start locator and tallies; per grain the BVH of
+// its candidate surfaces.
+struct DevBatch {
+    int32_t nstruct;
+    int32_t grid;                      // start locator: grid x grid cells over y, z
+    const int64_t *path_off;           // [nstruct + 1]
+    const int32_t *grain_root;         // [ngrain_total]: BVH root of the grain, -1 none
+    const BvhNode *nodes;
+    const int32_t *prims;              // global surface indices, leaf order
+    const int32_t *loc_off;            // [nstruct * grid * grid + 1]
+    const int32_t *loc_idx;            // incident facets overlapping each cell
+    const double *baseline;            // [nstruct]
+    double *sums;                      // [nstruct * 4]: n, sum x, sum x^2, sum (x-b) score
+    unsigned int *done;                // [nstruct] paths finished
+    unsigned long long *finish;        // [nstruct] wall clock when the last path ended
+    unsigned long long *start;         // wall clock at kernel start (one word)
+};
 
 // Largest number of specularity groups of one adjoint launch (each lane keeps one FP64
 // score per group in LDS).
@@ -105,13 +137,15 @@ struct TraceParams {
     // dx / mfp_m and l / mfp_m over the flights inside g, [2 * g], [2 * g + 1]
     double *grain_tally;
     int32_t ngt;               // grains tallied (ngrain_total) or 0
+    DevBatch bt;               // TRACE_BATCH
 };
 
 __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
 inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 {
-    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt) *
-           sizeof(double);
+    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt +
+            4 * (size_t)p.bt.nstruct) * sizeof(double) +
+           (p.bt.nstruct ? (size_t)kBvhStack * threads * sizeof(int32_t) : 0);
 }
 
 // kernels' launch wrappers (pt_trace.hip)
@@ -125,5 +159,9 @@ hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t 
 // workgroups per CU the trace kernel of (mode, precision) can hold with `lds` bytes of
 // dynamic LDS
 hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int *blocks_per_cu);
+// the trace kernel of (mode, precision) and the zero-fill kernel, for graph nodes
+const void *trace_kernel_fn(int mode, int precision);
+const void *fill_kernel_fn();
+hipError_t launch_fill(unsigned long long *p, size_t n, hipStream_t s);
 
 }  // namespace pt

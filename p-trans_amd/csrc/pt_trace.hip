@@ -326,6 +326,40 @@ __device__ int incident_collision_grain(const DevGeom &g, const double *p, OpCou
     return found >= 0 ? found : g.igrain_fill;
 }
 
+// Mesh grains (extension): the last grain with an incident facet containing the start
+// point, -1 if none.
+__device__ int mesh_start_grain(const DevGeom &g, const double *p, OpCount &oc)
+{
+    int found = -1;
+    for (int ig = 1; ig <= g.ngrain; ++ig) {
+        if (!g.grain_incident[ig]) continue;
+        for (int j = g.inc_off[ig]; j < g.inc_off[ig + 1]; ++j) {
+            const DSurf &s = g.surf[g.inc_idx[j]];
+            if (inside_polygon(g.vtx + 3 * (int64_t)s.voff, s.nv, p, oc)) { found = ig; break; }
+        }
+    }
+    return found;
+}
+
+// Start grain of a path of batch structure sid: the mesh rule (last grain with an
+// incident facet containing the point) over the facets the structure's start locator
+// lists for the point's cell; every incident facet overlapping the cell is listed.
+__device__ int batch_start_grain(const DevGeom &g, const DevBatch &bt, int sid, const double *p,
+                                 OpCount &oc)
+{
+    const int G = bt.grid;
+    int iy = (int)(p[1] / g.box[1] * G), iz = (int)(p[2] / g.box[2] * G);
+    iy = iy < 0 ? 0 : (iy >= G ? G - 1 : iy);
+    iz = iz < 0 ? 0 : (iz >= G ? G - 1 : iz);
+    const int64_t c = ((int64_t)sid * G + iy) * G + iz;
+    int found = -1;
+    for (int j = bt.loc_off[c]; j < bt.loc_off[c + 1]; ++j) {
+        const DSurf &s = g.surf[bt.loc_idx[j]];
+        if (s.gin > found && inside_polygon(g.vtx + 3 * (int64_t)s.voff, s.nv, p, oc)) found = s.gin;
+    }
+    return found;
+}
+
 // State of one phonon path.
 struct Path;
 __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *best, OpCount &oc);
@@ -345,6 +379,7 @@ struct Path {
     unsigned fallbacks;     // fast path: events decided by the full FP64 scan
     float bc[3];            // safety ball: centre and radius (br <= 0: none); no surface
     float br;               //   the scan can hit lies within it
+    int sid;                // batch structure of the path
 };
 
 // ---------------------------------------------------------------------------------
@@ -639,6 +674,79 @@ __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *b
     }
 }
 
+// findCollisionSurface (RT:1989-2096) for batch structures: the three facing box walls,
+// then the current grain's candidates through its BVH.  Node tests are FP32 slab tests
+// against host-padded bounds (they only prune); surfaces are tested by the reference's
+// FP64 routines, and the winner is the smallest (distance, surface index): the first
+// strictly nearest surface in the reference's scan order (box grain first, then the grain
+// in order).  With no hit the previous surface is kept.
+__device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, double *dmin,
+                         int *best, OpCount &oc, int *stk)
+{
+    double bl = 1.0e10;
+    int bi = ph.cur;
+    bool hit = false;
+    for (int a = 0; a < 3; ++a) {
+        const double d = a == 0 ? ph.dir[0] : (a == 1 ? ph.dir[1] : ph.dir[2]);
+        if (d == 0.0) continue;
+        const int w = 2 * a + (d < 0.0 ? 0 : 1);
+        double l;
+        if (surface_distance(g, g.surf[w], ph.pos, ph.dir, &l, oc) && (l < bl || (hit && l == bl && w < bi))) {
+            bl = l;
+            bi = w;
+            hit = true;
+        }
+    }
+    const int root = ph.which >= 0 ? bt.grain_root[ph.which] : -1;
+    if (root >= 0) {
+        const float ox = (float)ph.pos[0], oy = (float)ph.pos[1], oz = (float)ph.pos[2];
+        const float dx = (float)ph.dir[0], dy = (float)ph.dir[1], dz = (float)ph.dir[2];
+        const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+        int sp = 0;
+        stk[0] = root;
+        sp = 1;
+        while (sp > 0) {
+            const int ni = stk[--sp * 256];
+            const BvhNode nd = bt.nodes[ni];
+            // slab test; an axis the ray does not move along passes if the origin is in
+            float t0 = -3.0e38f, t1 = 3.0e38f;
+            if (dx != 0.0f) {
+                const float a0 = (nd.lo[0] - ox) * ix, a1 = (nd.hi[0] - ox) * ix;
+                t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
+            } else if (ox < nd.lo[0] || ox > nd.hi[0]) continue;
+            if (dy != 0.0f) {
+                const float a0 = (nd.lo[1] - oy) * iy, a1 = (nd.hi[1] - oy) * iy;
+                t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
+            } else if (oy < nd.lo[1] || oy > nd.hi[1]) continue;
+            if (dz != 0.0f) {
+                const float a0 = (nd.lo[2] - oz) * iz, a1 = (nd.hi[2] - oz) * iz;
+                t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
+            } else if (oz < nd.lo[2] || oz > nd.hi[2]) continue;
+            // margins: 1e-5 relative on the parameters (the padded bounds cover the rest)
+            if (t1 < -1.0e-5f * fabsf(t1) || t0 > t1 + 1.0e-5f * fabsf(t1)) continue;
+            if (hit && (double)t0 * (1.0 - 1.0e-5) > bl) continue;
+            if (nd.a < 0) {
+                const int f = -nd.a - 1;
+                for (int k = f; k < f + nd.b; ++k) {
+                    const int idx = bt.prims[k];
+                    double l;
+                    if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) &&
+                        (l < bl || (hit && l == bl && idx < bi))) {
+                        bl = l;
+                        bi = idx;
+                        hit = true;
+                    }
+                }
+            } else if (sp + 2 <= kBvhStack) {
+                stk[sp++ * 256] = nd.b;
+                stk[sp++ * 256] = nd.a;
+            }
+        }
+    }
+    *dmin = bl;
+    *best = bi;
+}
+
 // Safety ball of a diffusive path (an acceleration with no counterpart in the
 // reference).  At the current position, R = the smallest distance to any surface the
 // collision scan tests (box walls and the current grain's candidates): the plane of a
@@ -817,7 +925,7 @@ __device__ void surface_frame32(const DevGeom &g, int idx, const DSurf &s, const
 // after the position instead of taking the cell's fixed angle.
 template <int kPrec>
 __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool lambert, Rng &rng,
-                           Path &ph)
+                           Path &ph, const DevBatch *bt = nullptr)
 {
     OpCount &oc = *rng.oc;
     oc.flops += 7 + kFlopsSincos;
@@ -863,9 +971,12 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     int which = 0;
     if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
     else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos, oc);
+    else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_MESH)
+        which = bt ? batch_start_grain(g, *bt, ph.sid, ph.pos, oc) : mesh_start_grain(g, ph.pos, oc);
     ph.which = which;
     ph.start_grain = which;
-    if (find_inside_which_grain(g, ph.pos, oc) != which) {                      // RT:1673-1682
+    if (g.kind_grain == PT_GRAIN_MESH ? which < 0
+                                      : find_inside_which_grain(g, ph.pos, oc) != which) {  // RT:1673-1682
         ph.flags = PT_PATH_START_MISMATCH;
         return false;
     }
@@ -914,9 +1025,9 @@ struct AdjLane {
 };
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
-template <int kPrec, bool kAdj>
+template <int kPrec, bool kAdj, bool kBatch = false>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
-                          Path &ph, AdjLane &al)
+                          Path &ph, AdjLane &al, const DevBatch *bt = nullptr, int *stk = nullptr)
 {
     OpCount &oc = *rng.oc;
     oc.flops += 3;                       // box + SMALL_VAL
@@ -940,9 +1051,13 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     bool decided = false;
     // accelerated precisions: inside its safety ball the free flight ends first (dmin
     // stays beyond it and the event is an internal scattering, as the scan would find)
-    const bool in_safe = kPrec != kPrecFP64 && in_ball(ph);
+    const bool in_safe = !kBatch && kPrec != kPrecFP64 && in_ball(ph);
     if (in_safe) decided = true;
-    if (kPrec != kPrecFP64 && !in_safe) {
+    if (kBatch) {
+        scan_bvh(g, *bt, ph, &dmin, &best, oc, stk);
+        decided = true;
+    }
+    if (!kBatch && kPrec != kPrecFP64 && !in_safe) {
         decided = scan_fast(g, ph, &dmin, &best, oc);
         if (!decided) {
             dmin = 1.0e10;
@@ -1112,7 +1227,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     }
     // a diffusive path (mean free path under 5% of the smallest box side) that scattered
     // outside its ball takes a new ball around the scattering point
-    if (kPrec != kPrecFP64) {
+    if (!kBatch && kPrec != kPrecFP64) {
         const double lim = 0.05 * fmin(g.box[0], fmin(g.box[1], g.box[2]));
         if (scattered && !in_safe && ph.mfp < lim) ball_setup(g, ph);
     }
@@ -1187,12 +1302,23 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
         *mode = (ib - 1) + P.m.nb * (ik - 1);
         return start_path<kPrec>(P.g, P.cos_t[ia - 1], P.sin_t[ia - 1], false, rng, ph);
     } else {
+        if (kMode == TRACE_BATCH) {
+            // structure of the path: path_off[sid] <= pid < path_off[sid + 1]
+            int lo = 0, hi = P.bt.nstruct - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (P.bt.path_off[mid] <= pid) lo = mid;
+                else hi = mid - 1;
+            }
+            ph.sid = lo;
+        }
         const int m = sample_mode(P.m, rng);
         const int ib = m % P.m.nb;
         ph.omega = P.m.omega[m];
         ph.mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[ib] : P.m.mfp[m];
         *mode = m;
-        return start_path<kPrec>(P.g, 0.0, 0.0, true, rng, ph);
+        return start_path<kPrec>(P.g, 0.0, 0.0, true, rng, ph,
+                                 kMode == TRACE_BATCH ? &P.bt : nullptr);
     }
 }
 
@@ -1219,7 +1345,8 @@ template <int kMode, bool kCount, int kPrec>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 trace_kernel(TraceParams P)
 {
-    constexpr bool kAdj = kMode == TRACE_ADJOINT;
+    constexpr bool kBatch = kMode == TRACE_BATCH;
+    constexpr bool kAdj = kMode == TRACE_ADJOINT || kBatch;
     extern __shared__ double adj_lds[];
     // per-lane statistics live in LDS, not in registers held through the whole trace:
     // paths, events, draws, lost, start mismatch, event limit, FP64 retests, fallbacks
@@ -1237,9 +1364,14 @@ trace_kernel(TraceParams P)
     al.gacc = (kAdj && P.grain_tally) ? adj_acc + P.nbatch * adj_k : nullptr;
     al.fgrain = -1;
     al.fdx = al.fl = al.fw = 0.0;
+    // batch: per-structure sums [n, x, x^2, (x - b) score] and the lane's BVH stack
+    double *st_acc = adj_acc + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
+    int *stk = (int *)(st_acc + (kBatch ? 4 * P.bt.nstruct : 0)) + threadIdx.x;
     if (kAdj) {
-        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
+        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0) +
+                         (kBatch ? 4 * P.bt.nstruct : 0);
         for (int i = threadIdx.x; i < nlds; i += blockDim.x) adj_lds[i] = 0.0;
+        if (kBatch && blockIdx.x == 0 && threadIdx.x == 0) *P.bt.start = wall_clock64();
         __syncthreads();
     }
     OpCount oc = {0ull, 0ull};
@@ -1293,7 +1425,8 @@ trace_kernel(TraceParams P)
         }
         if (active) {
             if (ph.end_side != 0 ||
-                step_path<kPrec, kAdj>(P.g, P.m.max_omega, P.event_limit, rng, ph, al)) {
+                step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
+                                               &P.bt, stk)) {
                 const bool transmitted = ph.end_side != 1;          // RT:1833
                 if (kMode == TRACE_GRID) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
@@ -1301,7 +1434,19 @@ trace_kernel(TraceParams P)
                     atomicAdd(P.mode_paths + mode, 1ull);
                     if (transmitted) atomicAdd(P.mode_trans + mode, 1ull);
                 }
-                if (kAdj) {
+                if (kBatch) {
+                    const double x = transmitted ? 1.0 / P.m.mfp[mode] : 0.0;
+                    const int sid = ph.sid;
+                    double *acc = st_acc + 4 * sid;
+                    atomicAdd(acc + 0, 1.0);
+                    atomicAdd(acc + 1, x);
+                    atomicAdd(acc + 2, x * x);
+                    const double v = al.score[0];
+                    if (v != 0.0) atomicAdd(acc + 3, (x - P.bt.baseline[sid]) * v);
+                    al.score[0] = 0.0;
+                    const unsigned n_s = (unsigned)(P.bt.path_off[sid + 1] - P.bt.path_off[sid]);
+                    if (atomicAdd(P.bt.done + sid, 1u) + 1u == n_s) P.bt.finish[sid] = wall_clock64();
+                } else if (kAdj) {
                     const double lam = P.m.mfp[mode];
                     const int ib = mode % P.m.nb;
                     const double sb = P.m.scale ? P.m.scale[ib] : 1.0;
@@ -1337,6 +1482,9 @@ trace_kernel(TraceParams P)
         if (al.gacc)
             for (int i = threadIdx.x; i < 2 * P.ngt; i += blockDim.x)
                 if (al.gacc[i] != 0.0) atomicAdd(P.grain_tally + i, al.gacc[i]);
+        if (kBatch)
+            for (int i = threadIdx.x; i < 4 * P.bt.nstruct; i += blockDim.x)
+                if (st_acc[i] != 0.0) atomicAdd(P.bt.sums + i, st_acc[i]);
     }
     // stats slots: paths, events, draws, lost, mismatch, limit -> stats[0..5];
     // retests, fallbacks -> stats[10], stats[11]
@@ -1439,7 +1587,8 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
     } while (0)
     if (mode == TRACE_GRID) PT_LAUNCH_R(TRACE_GRID);
     else if (mode == TRACE_ALIAS) PT_LAUNCH_R(TRACE_ALIAS);
-    else PT_LAUNCH_R(TRACE_ADJOINT);
+    else if (mode == TRACE_ADJOINT) PT_LAUNCH_R(TRACE_ADJOINT);
+    else PT_LAUNCH_R(TRACE_BATCH);
 #undef PT_LAUNCH_R
 #undef PT_LAUNCH_C
 #undef PT_LAUNCH
@@ -1454,9 +1603,40 @@ hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int
                             : precision == kPrecFilter ? PT_OCC(M, kPrecFilter) : PT_OCC(M, kPrecFP64))
     if (mode == TRACE_GRID) return PT_OCC_R(TRACE_GRID);
     if (mode == TRACE_ALIAS) return PT_OCC_R(TRACE_ALIAS);
-    return PT_OCC_R(TRACE_ADJOINT);
+    if (mode == TRACE_ADJOINT) return PT_OCC_R(TRACE_ADJOINT);
+    return PT_OCC_R(TRACE_BATCH);
 #undef PT_OCC_R
 #undef PT_OCC
+}
+
+const void *trace_kernel_fn(int mode, int precision)
+{
+#define PT_FN(M) (precision == kPrecFast ? (const void *)trace_kernel<M, false, kPrecFast>            \
+                  : precision == kPrecFilter ? (const void *)trace_kernel<M, false, kPrecFilter>      \
+                                             : (const void *)trace_kernel<M, false, kPrecFP64>)
+    if (mode == TRACE_GRID) return PT_FN(TRACE_GRID);
+    if (mode == TRACE_ALIAS) return PT_FN(TRACE_ALIAS);
+    if (mode == TRACE_ADJOINT) return PT_FN(TRACE_ADJOINT);
+    return PT_FN(TRACE_BATCH);
+#undef PT_FN
+}
+
+// Zero fill of n 64-bit words (used instead of memset nodes in captured graphs).
+__global__ void fill_zero_kernel(unsigned long long *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+const void *fill_kernel_fn() { return (const void *)fill_zero_kernel; }
+
+hipError_t launch_fill(unsigned long long *p, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np, int64_t nb,

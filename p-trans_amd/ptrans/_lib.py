@@ -22,6 +22,8 @@ EXPORTS = (
     "pt_modes_create", "pt_modes_destroy",
     "pt_trace_grid", "pt_replay", "pt_alias_build", "pt_trace_alias", "pt_replay_alias",
     "pt_adjoint_alias", "pt_trace_alias_grains",
+    "pt_batch_create", "pt_batch_destroy", "pt_batch_info", "pt_batch_set_specularity",
+    "pt_batch_trace", "pt_batch_trace_graph",
 )
 
 
@@ -84,6 +86,12 @@ def load():
     L.pt_geom_set_specularity.argtypes = [vp, vp]
     L.pt_adjoint_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, i32, dbl, i32, vp, vp, vp, vp]
     L.pt_trace_alias_grains.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp, vp]
+    L.pt_batch_create.argtypes = [vp, i32, vp, vp, i32, i32, ctypes.POINTER(vp)]
+    L.pt_batch_destroy.argtypes = [vp]
+    L.pt_batch_info.argtypes = [vp, vp]
+    L.pt_batch_set_specularity.argtypes = [vp, vp]
+    L.pt_batch_trace.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
+    L.pt_batch_trace_graph.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

@@ -13,6 +13,7 @@ TRANS_VALUE, TRANS_GAMMA, TRANS_TBC = 1, 2, 3                # RT:77-79
 TRANS_LORENTZ, TRANS_PERIODIC = 4, 5                         # extensions (include/ptrans.h)
 MFP_MATTHIESSEN, MFP_SAMPLE = 1, 2                           # RT:127-128
 GRAIN_VORONOI, GRAIN_BRICKS = 1, 2                           # RT:131-132
+GRAIN_MESH = 3                                               # extension (include/ptrans.h)
 
 # C-ABI status codes (include/ptrans.h)
 PT_OK = 0

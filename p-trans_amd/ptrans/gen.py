@@ -249,3 +249,62 @@ def tri_brick(L=(1000.0, 500.0, 500.0), lo=(300.0, 100.0, 100.0), hi=(700.0, 400
     return McpModel(ncell=(1, 1, 1), lcell=tuple(L), cellspec=(p_box,) * 6, nphonon=nphonon,
                     nangle=nangle, mfp=mfp, kind_grain="bricks", grains=[g],
                     **_one_material(kind, value, p))
+
+
+def sc_lattice(seed=6, L=1000.0, k=4, m=2, jitter=5.0, kind="Gamma", value=2.0, p=0.3,
+               p_box=0.0, nphonon=1, nangle=3, mfp=None) -> McpModel:
+    """C5-type structure: the L^3 box split into a k x k x k simple-cubic lattice of
+    grains whose faces are triangulated (each face m x m squares, two triangles each,
+    12 k^3 m^2 triangles over all grains), on one shared vertex lattice jittered by up to
+    `jitter` nm (box-face vertices only within their face, so the box stays closed; the
+    triangles of neighbouring grains share vertices).  Triangles are counter-clockwise
+    seen from outside their grain; faces on the box are 'Boundary', shared faces name the
+    neighbour grain.  kind_grain 'mesh' (start grain from the incident facet containing
+    the start point, no inside test; an extension)."""
+    rng = np.random.default_rng(seed)
+    n = k * m + 1
+    h = L / (k * m)
+    base = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1) * h
+    jit = rng.uniform(-jitter, jitter, size=base.shape)
+    idx = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1)
+    jit[(idx == 0) | (idx == n - 1)] = 0.0
+    V = base + jit
+
+    def vert(a, b, c):
+        return tuple(float(x) for x in V[a, b, c])
+
+    def gid(i, j, l):
+        return 1 + i + k * (j + k * l)
+
+    grains = []
+    for l in range(k):
+        for j in range(k):
+            for i in range(k):
+                lo = (i * m, j * m, l * m)
+                facets = []
+                for axis in range(3):
+                    u_ax, v_ax = [(1, 2), (2, 0), (0, 1)][axis]
+                    for side in (0, 1):
+                        cell = [i, j, l]
+                        cell[axis] += 1 if side else -1
+                        if 0 <= cell[axis] < k:
+                            to, mat = gid(*cell), "Si"
+                        else:
+                            to, mat = 0, "Boundary"
+                        for a in range(m):
+                            for b in range(m):
+                                def q(da, db):
+                                    g = list(lo)
+                                    g[axis] += m if side else 0
+                                    g[u_ax] += a + da
+                                    g[v_ax] += b + db
+                                    return vert(*g)
+                                q00, q10, q11, q01 = q(0, 0), q(1, 0), q(1, 1), q(0, 1)
+                                tris = ([[q00, q10, q11], [q00, q11, q01]] if side else
+                                        [[q00, q11, q10], [q00, q01, q11]])
+                                facets.extend(FacetSpec(t, mat, to) for t in tris)
+                c = tuple(float(x) for x in (np.array(lo) + m / 2.0) * h)
+                grains.append(GrainSpec(c, "polyhedron", (k * h,) * 3, "Si", facets))
+    return McpModel(ncell=(1, 1, 1), lcell=(L, L, L), cellspec=(p_box,) * 6, nphonon=nphonon,
+                    nangle=nangle, mfp=mfp, kind_grain="mesh", grains=grains,
+                    **_one_material(kind, value, p))

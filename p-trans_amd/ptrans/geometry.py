@@ -15,7 +15,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .constants import (GRAIN_BRICKS, GRAIN_VORONOI, SHAPE_CYLINDER, SHAPE_POLYGON,
+from .constants import (GRAIN_BRICKS, GRAIN_MESH, GRAIN_VORONOI, SHAPE_CYLINDER, SHAPE_POLYGON,
                         SHAPE_SPHERE, TRANS_GAMMA, TRANS_TBC, TRANS_VALUE)
 from .mcp import McpModel, trans_kind_id
 
@@ -286,14 +286,14 @@ def build_geometry(m: McpModel) -> Geometry:
     if m.specfluc > 0.0:
         raise NotImplementedError("FluctuationSpecularity > 0 draws per-surface jitter (RT:1151-1157)")
     box = m.box
-    kind_grain = GRAIN_VORONOI if m.kind_grain == "voronoi" else GRAIN_BRICKS
+    kind_grain = {"voronoi": GRAIN_VORONOI, "bricks": GRAIN_BRICKS, "mesh": GRAIN_MESH}[m.kind_grain]
     ngrain = len(m.grains)
     igrain_fill = ngrain + 1 if kind_grain == GRAIN_BRICKS else -1
     kind_t, trans_t, spec_t = _material_tables(m)
     grains = [box_grain(box, m.cellspec)]
     for ig, gs in enumerate(m.grains, start=1):
         g = Grain(incident=False, generatrix=tuple(float(x) for x in gs.generatrix))
-        if kind_grain == GRAIN_VORONOI or gs.shape in ("box", "length", "polyhedron"):
+        if kind_grain != GRAIN_BRICKS or gs.shape in ("box", "length", "polyhedron"):
             kshape = SHAPE_POLYGON
             diameter = 0.0
         elif gs.shape == "sphere":

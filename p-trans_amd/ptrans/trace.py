@@ -224,6 +224,55 @@ def _trace_alias_grains(self, geom: Geom, modes: Modes, npaths: int, branch_scal
 Context.trace_alias_grains = _trace_alias_grains
 
 
+class Batch:
+    """pt_batch: mesh structures (same box) traced together in one persistent launch, each
+    structure owning a contiguous range of alias-mode path ids."""
+
+    def __init__(self, ctx, geometries, npaths, leaf_size: int = 4, grid: int = 64):
+        self.ctx = ctx
+        self.geometries = list(geometries)
+        self._descs = [g.desc() for g in self.geometries]
+        n = len(self._descs)
+        ptrs = (ctypes.c_void_p * n)(*[ctypes.addressof(d) for d, _ in self._descs])
+        self.npaths = np.ascontiguousarray(np.broadcast_to(np.asarray(npaths, dtype=np.int64), (n,)))
+        self.path_off = np.concatenate([[0], np.cumsum(self.npaths)])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().pt_batch_create(ctx.handle, n, ctypes.addressof(ptrs),
+                                                self.npaths.ctypes.data, int(leaf_size), int(grid),
+                                                ctypes.byref(h)))
+        self.handle = h
+        self.nstruct = n
+        self.leaf_size, self.grid = int(leaf_size), int(grid)
+        ctx._owned.insert(0, self)          # closed before the geometries and the context
+
+    def info(self):
+        v = np.zeros(6, dtype=np.int64)
+        _lib.check(_lib.load().pt_batch_info(self.handle, v.ctypes.data))
+        return dict(zip(("surfaces", "grains", "nodes", "prims", "depth", "paths"), v.tolist()))
+
+    def set_specularity(self, mean):
+        m = np.ascontiguousarray(mean, dtype=np.float64)
+        _lib.check(_lib.load().pt_batch_set_specularity(self.handle, m.ctypes.data))
+
+    def trace(self, modes: Modes, precision: int = PREC_FP64, baseline=None, graph: bool = False):
+        """One launch: per-structure sums (nstruct, 4) of [paths, sum x, sum x^2,
+        sum (x - baseline) score], completion times (ms) and the stats."""
+        sums = np.zeros((self.nstruct, 4), dtype=np.float64)
+        fin = np.zeros(self.nstruct, dtype=np.float64)
+        st = _lib.PtStats()
+        b = None if baseline is None else np.ascontiguousarray(baseline, dtype=np.float64)
+        fn = _lib.load().pt_batch_trace_graph if graph else _lib.load().pt_batch_trace
+        _lib.check(fn(self.ctx.handle, self.handle, modes.handle, int(precision),
+                      b.ctypes.data if b is not None else None, sums.ctypes.data, fin.ctypes.data,
+                      ctypes.byref(st)))
+        return sums, fin, st.as_dict()
+
+    def close(self):
+        if self.handle:
+            _lib.load().pt_batch_destroy(self.handle)
+            self.handle = None
+
+
 def alias_table(weights):
     """pt_alias_build on a flat weight vector: (prob, alias)."""
     w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)

@@ -1,0 +1,127 @@
+"""Batches of triangulated structures (C5): jittered simple-cubic grain lattices traced
+together in one persistent launch, each grain's candidates behind a BVH.
+
+The batch has no reference counterpart; it is pinned by the oracle: a batch structure's
+per-structure tallies equal the oracle's alias-mode trace of that structure alone over
+the same path ids (the brute-force scan of findCollisionSurface, RT:1989-2096), and the
+GPU's non-batched mesh paths equal the oracle's bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ptrans import gen, geometry, post
+
+M_TWOPI = 3.14159265358979323846 * 2.0
+
+
+def _modes(nb=2, nk=4, seed=3):
+    from test_alias import _table, _weights
+    omega, vel, mfp, weight = _table(nb, nk, seed)
+    mfp = np.clip(mfp, 80.0, None)
+    prob, alias = O.build_alias(_weights(omega, vel, mfp, weight))
+    return omega, vel, mfp, weight, prob, alias
+
+
+def _lattices():
+    return [geometry.build_geometry(gen.sc_lattice(seed=11 + j, k=2 + (j % 2), m=1 + j, p=0.3,
+                                                    kind="Gamma", value=2.0))
+            for j in range(3)]
+
+
+def test_lattice_closed_outward_and_tiles_the_start_face():
+    geo = geometry.build_geometry(gen.sc_lattice(seed=6, k=3, m=2))
+    from collections import Counter
+    for gi, g in enumerate(geo.grains[1:], 1):
+        E = Counter()
+        for f in g.surfaces:
+            v = f.vertices
+            for t in range(3):
+                E[(v[t], v[(t + 1) % 3])] += 1
+            assert np.dot(np.array(f.center) - np.array(g.generatrix), f.normal) > 0
+        assert all(E[(b, a)] == 1 for (a, b) in E)
+    inc = [f for g in geo.grains for f in g.surfaces if f.incident]
+    area = sum(0.5 * np.linalg.norm(np.cross(np.subtract(f.vertices[1], f.vertices[0]),
+                                             np.subtract(f.vertices[2], f.vertices[0]))) for f in inc)
+    assert abs(area - geo.box[1] * geo.box[2]) < 1e-6 * area
+
+
+def test_oracle_mesh_start_grains_are_the_first_layer():
+    geo = geometry.build_geometry(gen.sc_lattice(seed=6, k=3, m=2))
+    gd, _g = geo.desc()
+    omega, vel, mfp, weight, prob, alias = _modes()
+    md, _m = geometry.modes_desc(omega, vel, mfp, weight)
+    _, _, rec = O.trace_alias(gd, md, mfp, prob, alias, 4, 0, 3000, records=True)
+    assert not np.any(rec["flags"] & 2)
+    first_layer = {1 + 0 + 3 * (j + 3 * l) for j in range(3) for l in range(3)}
+    assert set(np.unique(rec["start_grain"]).tolist()) == first_layer
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 2])
+def test_gpu_mesh_paths_bitexact_vs_oracle(gpu_ctx, precision):
+    geo = geometry.build_geometry(gen.sc_lattice(seed=6, k=2, m=2))
+    omega, vel, mfp, weight, prob, alias = _modes()
+    gd, _g = geo.desc()
+    md, _m = geometry.modes_desc(omega, vel, mfp, weight)
+    _, _, orec = O.trace_alias(gd, md, mfp, prob, alias, 9, 0, 1500, records=True)
+    gpu_ctx.set_seed(9)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=omega, vel=vel, mfp=mfp, weight=weight)
+    rec = gpu_ctx.replay_alias(g, m, 0, 1500, precision=precision)
+    for f in ("end_side", "flags", "events", "draws", "which_grain", "start_grain"):
+        np.testing.assert_array_equal(rec[f], orec[f], err_msg=f)
+    np.testing.assert_array_equal(rec["pos"].view(np.uint64), orec["pos"].view(np.uint64))
+
+
+def _oracle_struct_sums(geo, mods, seed, first, n):
+    omega, vel, mfp, weight, prob, alias = mods
+    gd, _g = geo.desc()
+    md, _m = geometry.modes_desc(omega, vel, mfp, weight)
+    mp, mt = O.trace_alias(gd, md, mfp, prob, alias, seed, first, n)
+    return int(mp.sum()), float((mt / mfp).sum()), int(mt.sum())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_gpu_batch_equals_each_structure_alone(gpu_ctx, leaf):
+    from ptrans.trace import Batch
+    geos = _lattices()
+    mods = _modes()
+    omega, vel, mfp, weight, prob, alias = mods
+    npaths = [3000, 5000, 2000]
+    gpu_ctx.set_seed(21)
+    m = gpu_ctx.modes(omega=omega, vel=vel, mfp=mfp, weight=weight)
+    b = Batch(gpu_ctx, geos, npaths, leaf_size=leaf, grid=16)
+    print("batch", b.info(), flush=True)
+    for prec in (0, 2):
+        sums, fin, st = b.trace(m, precision=prec)
+        assert st["paths"] == sum(npaths)
+        assert np.all(fin > 0) and np.all(fin <= st["kernel_ms"] * 1.5 + 1.0)
+        for s, geo in enumerate(geos):
+            n, sx, nt = _oracle_struct_sums(geo, mods, 21, b.path_off[s], npaths[s])
+            assert sums[s, 0] == n, (s, sums[s], n)
+            assert abs(sums[s, 1] - sx) <= 1e-12 * max(sx, 1e-300), (s, sums[s, 1], sx)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batch_graph_replays_equal_direct_launch(gpu_ctx):
+    """The (fill, trace) graph replayed several times, with the program's own eager work
+    between replays: every replay equals a direct launch with the same seed."""
+    from ptrans.trace import Batch
+    geos = _lattices()
+    omega, vel, mfp, weight, prob, alias = _modes()
+    m = gpu_ctx.modes(omega=omega, vel=vel, mfp=mfp, weight=weight)
+    other = gpu_ctx.geometry(geometry.build_geometry(gen.film_holes(seed=2, nphonon=1, nangle=3)))
+    b = Batch(gpu_ctx, geos, [4000, 4000, 4000], leaf_size=2, grid=16)
+    for rep, seed in enumerate((5, 6, 5, 7)):
+        gpu_ctx.set_seed(seed)
+        direct, _, _ = b.trace(m, precision=1)
+        gpu_ctx.trace_alias(other, m, 50000 * (rep + 1))          # unrelated eager work
+        gpu_ctx.set_seed(seed)
+        graph, fin, st = b.trace(m, precision=1, graph=True)
+        np.testing.assert_array_equal(graph[:, 0], direct[:, 0])
+        np.testing.assert_allclose(graph, direct, rtol=1e-12, atol=1e-14)
+        assert st["paths"] == 12000
+    b.close()
