@@ -19,6 +19,12 @@ specularity; same bulk table): the alias-mode forward trace and the likelihood-r
 adjoint (12 specularity + 6 branch-scale derivatives of kappa) over 1e8 paths, and
 kappa-gradient evaluations/s = adjoint paths/s / the paths that make the gradient
 2-sigma significant (|stderr| = |grad| / 2).
+
+The "batch" object measures the screening configuration: 256 jittered simple-cubic grain
+structures (4^3 grains, 2e3 - 1e5 triangles each, log-uniform) traced together in one
+graph-replayed persistent launch with BVH scans, the leaf size and launch tile picked by
+the project's autotuner, per-structure likelihood-ratio scores of the mean specularity,
+p50/p99 per-structure completion times and the per-structure kappa error against FP64.
 """
 from __future__ import annotations
 
@@ -158,6 +164,68 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
     }
 
 
+def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
+    """C5 screening batch (see the module docstring)."""
+    from ptrans import autotune
+    from ptrans.trace import Batch
+    t0 = time.perf_counter()
+    geos, tris = gen.c5_structures(n=nstruct)
+    t_gen = time.perf_counter() - t0
+    t = synthetic_table(3, 1000, seed=table_seed)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    m = ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    tune = autotune.tune_batch(ctx, geos, m, probe_paths=max(1000, paths_per_struct // 20),
+                               leaf_sizes=(2, 4, 8), chunks=(256, 1024), refills=(16,),
+                               repeats=1, precision=PREC_FP32_RETEST)
+    t0 = time.perf_counter()
+    b = Batch(ctx, geos, paths_per_struct, leaf_size=tune["leaf_size"], grid=64)
+    t_build = time.perf_counter() - t0
+    info = b.info()
+    b.set_specularity(np.full(nstruct, 0.3))
+    ctx.set_seed(4242)
+    b.trace(m, precision=PREC_FP32_RETEST, graph=True)                     # graph build
+    ctx.set_seed(4243)
+    pilot, _, _ = b.trace(m, precision=PREC_FP32_RETEST, graph=True)
+    base = pilot[:, 1] / np.maximum(pilot[:, 0], 1.0)
+    ctx.set_seed(4244)
+    t0 = time.perf_counter()
+    sums, fin, st = b.trace(m, precision=PREC_FP32_RETEST, baseline=base, graph=True)
+    wall = time.perf_counter() - t0
+    ctx.set_seed(4244)
+    s64, fin64, st64 = b.trace(m, precision=PREC_FP64, graph=True)
+    b.close()
+    c = 0.75 * geos[0].box[0]
+    n = sums[:, 0]
+    k32 = c * sums[:, 1] / n
+    k64 = c * s64[:, 1] / s64[:, 0]
+    var32 = np.maximum(sums[:, 2] / n - (sums[:, 1] / n) ** 2, 0.0)
+    var64 = np.maximum(s64[:, 2] / s64[:, 0] - (s64[:, 1] / s64[:, 0]) ** 2, 0.0)
+    sig = c * np.sqrt(var32 / n + var64 / s64[:, 0])
+    rel = np.abs(k32 - k64) / k64
+    grad = c * sums[:, 3] / n
+    return {
+        "workload": "%d jittered 4x4x4 simple-cubic grain structures (1000 nm cube, +-5 nm "
+                    "jitter, Gamma law, p = 0.3), %d - %d triangles each, 3x1000 modes, "
+                    "%d paths per structure, one graph-replayed persistent launch, FP32 trace"
+                    % (nstruct, min(tris), max(tris), paths_per_struct),
+        "structures": nstruct, "triangles": int(sum(tris)), "surfaces": info["surfaces"],
+        "bvh_nodes": info["nodes"], "bvh_depth": info["depth"],
+        "paths": int(st["paths"]), "kernel_ms": st["kernel_ms"], "wall_s": wall,
+        "paths_per_s": st["paths"] / (st["kernel_ms"] * 1e-3),
+        "events_per_path": st["events"] / max(1, st["paths"]),
+        "completion_ms_p50": float(np.percentile(fin, 50)),
+        "completion_ms_p99": float(np.percentile(fin, 99)),
+        "fp64_paths_per_s": st64["paths"] / (st64["kernel_ms"] * 1e-3),
+        "kappa_rel_err_vs_fp64_median": float(np.median(rel)),
+        "kappa_rel_err_vs_fp64_max": float(np.max(rel)),
+        "kappa_diff_sigma_max": float(np.max(np.abs(k32 - k64) / sig)),
+        "mean_specularity_grad_median": float(np.median(grad)),
+        "autotune": {k: v for k, v in tune.items() if k != "table"},
+        "autotune_configs": len(tune["table"]),
+        "host_generate_s": t_gen, "host_build_s": t_build,
+    }
+
+
 def roofline(cstats, kernel_ms, kernel_key):
     """VALU roofline of the trace kernel: algorithmic FP64 and FP32 operations of one
     launch (instrumented twin kernel, same paths) over the timed kernel's average duration
@@ -193,6 +261,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-paths", type=float, default=1e8,
                     help="paths of the inclusions adjoint leg (0: skip it)")
+    ap.add_argument("--batch-structures", type=int, default=256,
+                    help="structures of the screening batch leg (0: skip it)")
+    ap.add_argument("--batch-paths", type=float, default=1e5,
+                    help="paths per structure of the screening batch leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
     args = ap.parse_args()
@@ -240,6 +312,8 @@ def main():
             res[prec] = dict(dt=dt, kernel_ms=float(np.mean(kern_ms)), counts=counts, stats=stats,
                              cstats=cstats)
         adj = adjoint_leg(ctx, int(args.adjoint_paths)) if args.adjoint_paths > 0 else None
+        bat = (batch_leg(ctx, args.batch_structures, int(args.batch_paths))
+               if args.batch_structures > 0 else None)
     fast, plain, filt = res[PREC_FP32_RETEST], res[PREC_FP64], res[PREC_FP32_FILTER]
     dt = fast["dt"]
     value = paths * args.steps / dt
@@ -300,6 +374,7 @@ def main():
         "roofline": roofline(cstats, k_ms, "fp32_trace"),
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "batch": bat,
     }
     print(json.dumps(out))
 

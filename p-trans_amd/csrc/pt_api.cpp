@@ -1005,6 +1005,8 @@ struct pt_batch {
     hipGraphNode_t trace_node = nullptr;
     const pt_modes *graph_modes = nullptr;
     int graph_prec = -1;
+    int graph_blocks = 0;                  // launch shape the graph was built with
+    size_t graph_lds = 0;
     TraceParams graph_params{};
     hipKernelNodeParams graph_knp{};
     void *graph_args[1] = {nullptr};
@@ -1435,7 +1437,8 @@ int pt_batch_trace_graph(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t prec
     int rc = batch_prepare(B, M, precision, P, &lds, &blocks);
     if (rc) return rc;
     if (stats) memset(stats, 0, sizeof *stats);
-    if (!B->exec || B->graph_modes != M || B->graph_prec != precision) {
+    if (!B->exec || B->graph_modes != M || B->graph_prec != precision ||
+        B->graph_blocks != blocks || B->graph_lds != lds) {
         if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
         if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
         PT_HIP(hipGraphCreate(&B->graph, 0));
@@ -1464,6 +1467,8 @@ int pt_batch_trace_graph(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t prec
         PT_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
         B->graph_modes = M;
         B->graph_prec = precision;
+        B->graph_blocks = blocks;
+        B->graph_lds = lds;
     }
     B->graph_params = P;                       // the current seed and tuning
     B->graph_args[0] = (void *)&B->graph_params;

@@ -797,6 +797,55 @@ __device__ void ball_setup(const DevGeom &g, Path &ph)
     ph.br = r * (1.0f - 1.0e-5f) - 1.0e-2f;
 }
 
+// ball_setup for batch structures: the box walls, then the current grain's candidates
+// through its BVH, pruning nodes whose (outward-padded) box is farther than the radius
+// found so far.  Leaf surfaces are bounded as in ball_setup.
+__device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, int *stk)
+{
+    const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
+    float r = fminf(fminf(fminf(px, (float)g.box[0] - px), fminf(py, (float)g.box[1] - py)),
+                    fminf(pz, (float)g.box[2] - pz));
+    r = fmaxf(r, 0.0f);
+    const int root = ph.which >= 0 ? bt.grain_root[ph.which] : -1;
+    if (root >= 0) {
+        int sp = 1;
+        stk[0] = root;
+        while (sp > 0) {
+            const BvhNode nd = bt.nodes[stk[--sp * 256]];
+            const float ex = fmaxf(fmaxf(nd.lo[0] - px, px - nd.hi[0]), 0.0f);
+            const float ey = fmaxf(fmaxf(nd.lo[1] - py, py - nd.hi[1]), 0.0f);
+            const float ez = fmaxf(fmaxf(nd.lo[2] - pz, pz - nd.hi[2]), 0.0f);
+            if (ex * ex + ey * ey + ez * ez >= r * r) continue;
+            if (nd.a < 0) {
+                const int f = -nd.a - 1;
+                for (int k = f; k < f + nd.b; ++k) {
+                    const DSurf s = g.surf[bt.prims[k]];
+                    float d;
+                    if (s.kind == PT_SHAPE_POLYGON) {
+                        const double *v = g.vtx + 3 * (int64_t)s.voff;
+                        const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
+                        const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny +
+                                        (pz - (float)v[2]) * nz;
+                        d = fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+                    } else {
+                        const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
+                        const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
+                        d = fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
+                    }
+                    r = fminf(r, d);
+                }
+            } else if (sp + 2 <= kBvhStack) {
+                stk[sp++ * 256] = nd.b;
+                stk[sp++ * 256] = nd.a;
+            }
+        }
+    }
+    ph.bc[0] = px;
+    ph.bc[1] = py;
+    ph.bc[2] = pz;
+    ph.br = r * (1.0f - 1.0e-5f) - 1.0e-2f;
+}
+
 // True when the free flight from the current position stays inside the safety ball.
 __device__ inline bool in_ball(const Path &ph)
 {
@@ -1051,9 +1100,9 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     bool decided = false;
     // accelerated precisions: inside its safety ball the free flight ends first (dmin
     // stays beyond it and the event is an internal scattering, as the scan would find)
-    const bool in_safe = !kBatch && kPrec != kPrecFP64 && in_ball(ph);
+    const bool in_safe = kPrec != kPrecFP64 && in_ball(ph);
     if (in_safe) decided = true;
-    if (kBatch) {
+    if (kBatch && !in_safe) {
         scan_bvh(g, *bt, ph, &dmin, &best, oc, stk);
         decided = true;
     }
@@ -1227,9 +1276,12 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     }
     // a diffusive path (mean free path under 5% of the smallest box side) that scattered
     // outside its ball takes a new ball around the scattering point
-    if (!kBatch && kPrec != kPrecFP64) {
+    if (kPrec != kPrecFP64) {
         const double lim = 0.05 * fmin(g.box[0], fmin(g.box[1], g.box[2]));
-        if (scattered && !in_safe && ph.mfp < lim) ball_setup(g, ph);
+        if (scattered && !in_safe && ph.mfp < lim) {
+            if (kBatch) ball_setup_bvh(g, *bt, ph, stk);
+            else ball_setup(g, ph);
+        }
     }
     // whichV on transmission, RT:1793-1796
     if (jreflect == 0) {

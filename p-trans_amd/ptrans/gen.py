@@ -308,3 +308,112 @@ def sc_lattice(seed=6, L=1000.0, k=4, m=2, jitter=5.0, kind="Gamma", value=2.0, 
     return McpModel(ncell=(1, 1, 1), lcell=(L, L, L), cellspec=(p_box,) * 6, nphonon=nphonon,
                     nangle=nangle, mfp=mfp, kind_grain="mesh", grains=grains,
                     **_one_material(kind, value, p))
+
+
+def sc_lattice_flat(seed=6, L=1000.0, k=4, m=2, jitter=5.0, trans_kind=2, trans_param=2.0,
+                    p=0.3, p_box=0.0):
+    """sc_lattice built directly as flattened arrays (geometry.FlatGeometry), vectorised,
+    for structures of up to ~1e5 triangles: the same vertices, facet order, flags, laws
+    and polygon set-up (bit-identical to building sc_lattice surface by surface)."""
+    from . import geometry as G
+    from .constants import GRAIN_MESH, SHAPE_POLYGON, TRANS_VALUE
+    rng = np.random.default_rng(seed)
+    n = k * m + 1
+    h = L / (k * m)
+    idx = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1)
+    base = idx * h
+    jit = rng.uniform(-jitter, jitter, size=base.shape)
+    jit[(idx == 0) | (idx == n - 1)] = 0.0
+    V = base + jit
+    # facets of one grain, in sc_lattice's order: axis, side, a, b, two triangles
+    tri_idx, sides, axes = [], [], []
+    for axis in range(3):
+        u_ax, v_ax = [(1, 2), (2, 0), (0, 1)][axis]
+        for side in (0, 1):
+            for a in range(m):
+                for b in range(m):
+                    def q(da, db):
+                        g = [0, 0, 0]
+                        g[axis] += m if side else 0
+                        g[u_ax] += a + da
+                        g[v_ax] += b + db
+                        return g
+                    q00, q10, q11, q01 = q(0, 0), q(1, 0), q(1, 1), q(0, 1)
+                    tris = ([[q00, q10, q11], [q00, q11, q01]] if side else
+                            [[q00, q11, q10], [q00, q01, q11]])
+                    tri_idx.extend(tris)
+                    sides.extend([side, side])
+                    axes.extend([axis, axis])
+    tri_idx = np.array(tri_idx)                  # (F, 3, 3) offsets within the grain
+    sides, axes = np.array(sides), np.array(axes)
+    F = len(tri_idx)
+    grains = [(i, j, l) for l in range(k) for j in range(k) for i in range(k)]
+    ng = len(grains)
+    lo = np.array([[i * m, j * m, l * m] for (i, j, l) in grains])          # (ng, 3)
+    vi = lo[:, None, None, :] + tri_idx[None]                               # (ng, F, 3, 3)
+    verts = V[vi[..., 0], vi[..., 1], vi[..., 2]]                           # (ng, F, 3, 3)
+    gcell = np.array(grains)                                                # (ng, 3)
+    nb = np.repeat(gcell[:, None, :], F, axis=1).copy()
+    step = np.where(sides == 1, 1, -1)
+    nb[:, np.arange(F), axes] += step[None, :]
+    inside = np.all((nb >= 0) & (nb < k), axis=2)
+    to = 1 + nb[..., 0] + k * (nb[..., 1] + k * nb[..., 2])
+    to = np.where(inside, to, 0)
+    # box walls (the reference's setupBoxGrain) and the grains' triangles
+    box = G.box_grain((L, L, L), (p_box,) * 6)
+    nf = ng * F
+    surf = np.zeros(6 + nf, dtype=G.SURFACE_DTYPE)
+    vpool = [np.asarray(w.vertices, dtype=np.float64) for w in box.surfaces]
+    off = 0
+    for i, w in enumerate(box.surfaces):
+        r = surf[i]
+        r["kind"], r["nvertex"], r["vertex_offset"] = SHAPE_POLYGON, 4, off
+        r["internal"], r["incident"] = 1, 0
+        r["trans_kind"], r["trans_param"], r["specularity"] = TRANS_VALUE, 0.0, w.specularity
+        r["center"], r["normal"] = w.center, w.normal
+        r["tangent"] = [w.tangent[j][q] for j in range(3) for q in range(3)]
+        off += 4
+    tv = verts.reshape(nf, 3, 3)
+    c, nrm, tang = G.setup_triangles(tv[:, 0], tv[:, 1], tv[:, 2])
+    s = surf[6:]
+    s["kind"], s["nvertex"] = SHAPE_POLYGON, 3
+    s["vertex_offset"] = off + 3 * np.arange(nf)
+    xs = tv[:, :, 0]
+    incident = np.all(xs <= 0.0, axis=1)
+    on_face = np.zeros(nf, dtype=bool)
+    for ax in range(3):
+        on_face |= np.all(tv[:, :, ax] <= 0.0, axis=1) | np.all(tv[:, :, ax] >= L, axis=1)
+    internal = ~incident & ~on_face
+    s["internal"], s["incident"] = internal, incident
+    s["grain_inside"] = np.repeat(np.arange(1, ng + 1), F)
+    s["grain_outside"] = to.reshape(nf)
+    s["trans_kind"] = np.where(internal, trans_kind, TRANS_VALUE)
+    s["trans_param"] = np.where(internal, trans_param, 0.0)
+    s["specularity"] = np.where(internal, p, 0.0)
+    s["center"], s["normal"], s["tangent"] = c, nrm, tang
+    s["radius"] = 0.0
+    vertices = np.concatenate([np.concatenate(vpool).reshape(-1), tv.reshape(-1)])
+    goff = np.concatenate([[0, 6], 6 + F * np.arange(1, ng + 1)]).astype(np.int32)
+    gcen = (lo + m / 2.0) * h
+    ginc = np.concatenate([[0], (gcell[:, 0] == 0).astype(np.int32)]).astype(np.int32)
+    gen_ = np.concatenate([[0.0, 0.0, 0.0], gcen.reshape(-1)]).astype(np.float64)
+    arrays = dict(grain_offset=goff, grain_incident=ginc, generatrix=gen_, surfaces=surf,
+                  vertices=vertices, nvertex=len(vertices) // 3)
+    return G.FlatGeometry((L, L, L), (1.0, 1.0), GRAIN_MESH, ng, -1, 1, arrays)
+
+
+def c5_structures(n=256, seed=5, triangles=(2.0e3, 1.0e5), k=4, L=1000.0, jitter=5.0,
+                  trans_kind=2, trans_param=2.0, p=0.3):
+    """C5 batch: n jittered k x k x k simple-cubic grain structures whose triangle counts
+    (unique facets, 480 m^2 for k = 4) are log-uniform in `triangles`, each with its own
+    jitter seed.  Returns (geometries, triangle counts)."""
+    rng = np.random.default_rng(seed)
+    faces = 3 * k * k * (k - 1) + 6 * k * k
+    geos, tris = [], []
+    for j in range(n):
+        t = math.exp(rng.uniform(math.log(triangles[0]), math.log(triangles[1])))
+        m = max(1, int(round(math.sqrt(t / (2.0 * faces)))))
+        geos.append(sc_lattice_flat(seed=int(rng.integers(1 << 30)), L=L, k=k, m=m, jitter=jitter,
+                                    trans_kind=trans_kind, trans_param=trans_param, p=p))
+        tris.append(2 * faces * m * m)
+    return geos, tris

@@ -429,3 +429,71 @@ def set_lorentz_boundaries(geo: Geometry, rng, f0_thz=(3.0, 10.0)) -> dict:
             s.trans_kind = TRANS_LORENTZ
             s.trans_param = w0[key]
     return w0
+
+
+class FlatGeometry:
+    """A geometry held directly as the flattened arrays of pt_geom_desc (for meshes too
+    large to build surface by surface); desc() and flatten() as Geometry's."""
+
+    def __init__(self, box, scope, kind_grain, ngrain, igrain_fill, nmaterial, arrays):
+        self.box, self.scope = tuple(box), tuple(scope)
+        self.kind_grain, self.ngrain, self.igrain_fill = kind_grain, ngrain, igrain_fill
+        self.nmaterial = nmaterial
+        self._arr = arrays
+        self.grains = [None] * (len(arrays["grain_offset"]) - 1)
+
+    def flatten(self):
+        return self._arr
+
+    def desc(self):
+        arr = self._arr
+        d = PtGeomDesc()
+        d.box = (ctypes.c_double * 3)(*self.box)
+        d.scope = (ctypes.c_double * 2)(*self.scope)
+        d.kind_grain = self.kind_grain
+        d.ngrain = self.ngrain
+        d.igrain_fill = self.igrain_fill
+        d.nmaterial = self.nmaterial
+        d.ngrain_total = len(arr["grain_offset"]) - 1
+        d.nsurface = len(arr["surfaces"])
+        d.nvertex = arr["nvertex"]
+        d.grain_offset = arr["grain_offset"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        d.grain_incident = arr["grain_incident"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        d.generatrix = arr["generatrix"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        d.surfaces = ctypes.c_void_p(arr["surfaces"].ctypes.data)
+        d.vertices = arr["vertices"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return d, arr
+
+
+def setup_triangles(v0, v1, v2):
+    """setup_polygon for many triangles at once (arrays (n, 3)), with the same operations
+    in the same order, so the results are bit-identical: centroid, averaged fan normal and
+    the tangent frame (n, 9), columns t1, t2, n."""
+    def cr(a, b):
+        return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1],
+                         a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                         a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], axis=1)
+
+    def unit(c):
+        ln = np.sqrt(c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1] + c[:, 2] * c[:, 2])
+        return c / ln[:, None]
+
+    inv = 1.0 / 3
+    z = np.zeros_like(v0)
+    center = inv * (((z + v0) + v1) + v2)
+    n = z + unit(cr(v1 - v0, v2 - v0))
+    n = n + unit(cr(v2 - v1, v0 - v1))
+    n = n + unit(cr(v0 - v2, v1 - v2))
+    normal = inv * n
+    tang = np.zeros((len(v0), 9))
+    tang[:, 6:9] = normal
+    axis = (np.abs(normal[:, 2]) == 1.0) | ((normal[:, 0] ** 2 == 0.0) & (normal[:, 1] ** 2 == 0.0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.sqrt(normal[:, 0] * normal[:, 0] + normal[:, 1] * normal[:, 1])
+        c2 = np.stack([-normal[:, 1] / r, normal[:, 0] / r, np.zeros(len(v0))], axis=1)
+    c1 = cr(c2, normal)
+    c2[axis] = (0.0, 1.0, 0.0)
+    c1[axis] = (1.0, 0.0, 0.0)
+    tang[:, 0:3] = c1
+    tang[:, 3:6] = c2
+    return center, normal, tang
