@@ -171,16 +171,20 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
     t0 = time.perf_counter()
     geos, tris = gen.c5_structures(n=nstruct)
     t_gen = time.perf_counter() - t0
+    say = lambda *a: print("batch:", *a, file=sys.stderr, flush=True)   # noqa: E731
+    say("%d structures, %d triangles, generated in %.1f s" % (nstruct, sum(tris), t_gen))
     t = synthetic_table(3, 1000, seed=table_seed)
     omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
     m = ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
     tune = autotune.tune_batch(ctx, geos, m, probe_paths=max(1000, paths_per_struct // 20),
                                leaf_sizes=(2, 4, 8), chunks=(256, 1024), refills=(16,),
-                               repeats=1, precision=PREC_FP32_RETEST)
+                               repeats=1, precision=PREC_FP32_RETEST,
+                               log=lambda r: say("autotune", json.dumps(r)))
     t0 = time.perf_counter()
     b = Batch(ctx, geos, paths_per_struct, leaf_size=tune["leaf_size"], grid=64)
     t_build = time.perf_counter() - t0
     info = b.info()
+    say("built", json.dumps(info), "in %.1f s" % t_build)
     b.set_specularity(np.full(nstruct, 0.3))
     ctx.set_seed(4242)
     b.trace(m, precision=PREC_FP32_RETEST, graph=True)                     # graph build
@@ -191,6 +195,7 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
     t0 = time.perf_counter()
     sums, fin, st = b.trace(m, precision=PREC_FP32_RETEST, baseline=base, graph=True)
     wall = time.perf_counter() - t0
+    say("fp32 trace %.1f ms" % st["kernel_ms"])
     ctx.set_seed(4244)
     s64, fin64, st64 = b.trace(m, precision=PREC_FP64, graph=True)
     b.close()

@@ -162,6 +162,8 @@ typedef struct pt_stats {
     int64_t philox_blocks;   /* Philox4x32-10 blocks evaluated (op counting on only) */
     int64_t retests;         /* PT_PREC_FP32_RETEST: candidates settled by FP64       */
     int64_t fallbacks;       /* PT_PREC_FP32_RETEST: events decided by the FP64 scan  */
+    int64_t bvh_nodes;       /* batch trace: BVH nodes visited (op counting on only)  */
+    int64_t bvh_tests;       /* batch trace: leaf surfaces tested (op counting on)    */
 } pt_stats;
 
 /* Per-path record of pt_replay(). */

@@ -642,6 +642,8 @@ void fill_stats(pt_stats *stats, const unsigned long long *h, float ms, float ms
     stats->philox_blocks = (int64_t)h[9];
     stats->retests = (int64_t)h[11];
     stats->fallbacks = (int64_t)h[12];
+    stats->bvh_nodes = (int64_t)h[13];
+    stats->bvh_tests = (int64_t)h[14];
 }
 
 int replay_common(pt_ctx *c, TraceParams &P, int mode, int64_t first, int64_t n,
@@ -1022,10 +1024,17 @@ struct BvhPrim {
 float round_down(double v) { float f = (float)v; if ((double)f > v) f = nextafterf(f, -3.0e38f); return f; }
 float round_up(double v) { float f = (float)v; if ((double)f < v) f = nextafterf(f, 3.0e38f); return f; }
 
-// Median-split BVH over prims[b, e); returns the node index.  Leaves hold at most
-// `leaf` prims unless the depth limit forces a larger leaf.
-int bvh_build(std::vector<BvhPrim> &P, int b, int e, int depth, int max_depth, int leaf,
-              std::vector<BvhNode> &nodes, std::vector<int32_t> &prims, int *depth_out)
+// A child slot of a BvhNode: its reference and padded bounds.
+struct BvhChild {
+    int32_t ref, count;
+    float lo[3], hi[3];
+};
+
+// Median-split BVH over prims[b, e); returns the child slot of the subtree (a leaf or an
+// internal node).  Leaves hold at most `leaf` prims unless the depth limit forces a
+// larger leaf.
+BvhChild bvh_build(std::vector<BvhPrim> &P, int b, int e, int depth, int max_depth, int leaf,
+                   std::vector<BvhNode> &nodes, std::vector<int32_t> &prims, int *depth_out)
 {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
@@ -1036,32 +1045,63 @@ int bvh_build(std::vector<BvhPrim> &P, int b, int e, int depth, int max_depth, i
             clo[k] = std::min(clo[k], P[i].c[k]);
             chi[k] = std::max(chi[k], P[i].c[k]);
         }
-    const int ni = (int)nodes.size();
-    nodes.push_back(BvhNode{});
+    BvhChild me{};
     for (int k = 0; k < 3; ++k) {
         // outward padding: 1e-3 nm + 1e-6 relative covers FP32 rounding of positions
         const float l = round_down(lo[k]), h = round_up(hi[k]);
-        nodes[ni].lo[k] = l - (1.0e-3f + 1.0e-6f * fabsf(l));
-        nodes[ni].hi[k] = h + (1.0e-3f + 1.0e-6f * fabsf(h));
+        me.lo[k] = l - (1.0e-3f + 1.0e-6f * fabsf(l));
+        me.hi[k] = h + (1.0e-3f + 1.0e-6f * fabsf(h));
     }
     *depth_out = std::max(*depth_out, depth);
     int axis = 0;
     for (int k = 1; k < 3; ++k)
         if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
     if (e - b <= leaf || depth >= max_depth || !(chi[axis] > clo[axis])) {
-        nodes[ni].a = -(int32_t)prims.size() - 1;
-        nodes[ni].b = e - b;
+        me.ref = -(int32_t)prims.size() - 1;
+        me.count = e - b;
         for (int i = b; i < e; ++i) prims.push_back(P[i].idx);
-        return ni;
+        return me;
     }
+    const int ni = (int)nodes.size();
+    nodes.push_back(BvhNode{});
     const int m = (b + e) / 2;
     std::nth_element(P.begin() + b, P.begin() + m, P.begin() + e,
                      [axis](const BvhPrim &x, const BvhPrim &y) { return x.c[axis] < y.c[axis]; });
-    const int left = bvh_build(P, b, m, depth + 1, max_depth, leaf, nodes, prims, depth_out);
-    const int right = bvh_build(P, m, e, depth + 1, max_depth, leaf, nodes, prims, depth_out);
-    nodes[ni].a = left;
-    nodes[ni].b = right;
-    return ni;
+    const BvhChild kid[2] = {bvh_build(P, b, m, depth + 1, max_depth, leaf, nodes, prims, depth_out),
+                             bvh_build(P, m, e, depth + 1, max_depth, leaf, nodes, prims, depth_out)};
+    for (int c = 0; c < 2; ++c) {
+        nodes[ni].child[c] = kid[c].ref;
+        nodes[ni].count[c] = kid[c].count;
+        for (int k = 0; k < 3; ++k) {
+            nodes[ni].lo[c][k] = kid[c].lo[k];
+            nodes[ni].hi[c][k] = kid[c].hi[k];
+        }
+    }
+    me.ref = ni;
+    me.count = 0;
+    return me;
+}
+
+// Root node of a BVH: the top split, or for a single leaf a node whose second child is
+// an empty leaf (no prims, inverted bounds).
+int bvh_root(std::vector<BvhPrim> &P, int max_depth, int leaf, std::vector<BvhNode> &nodes,
+             std::vector<int32_t> &prims, int *depth_out)
+{
+    const BvhChild top = bvh_build(P, 0, (int)P.size(), 0, max_depth, leaf, nodes, prims, depth_out);
+    if (top.ref >= 0) return top.ref;
+    BvhNode nd{};
+    nd.child[0] = top.ref;
+    nd.count[0] = top.count;
+    nd.child[1] = -1;
+    nd.count[1] = 0;
+    for (int k = 0; k < 3; ++k) {
+        nd.lo[0][k] = top.lo[k];
+        nd.hi[0][k] = top.hi[k];
+        nd.lo[1][k] = 3.0e38f;
+        nd.hi[1][k] = -3.0e38f;
+    }
+    nodes.push_back(nd);
+    return (int)nodes.size() - 1;
 }
 
 void prim_bounds(const pt_geom_desc &d, const pt_surface &s, BvhPrim &p)
@@ -1207,7 +1247,7 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
         }
         if (P.empty()) continue;
         int dep = 0;
-        root[g] = bvh_build(P, 0, (int)P.size(), 0, kBvhStack - 2, leaf_size, nodes, prims, &dep);
+        root[g] = bvh_root(P, kBvhStack - 2, leaf_size, nodes, prims, &dep);
         B->depth = std::max(B->depth, dep);
     }
     B->nodes = (int64_t)nodes.size();
@@ -1415,7 +1455,7 @@ int pt_batch_trace(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t precision,
                                 hipMemcpyHostToDevice, c->stream))) break;
         if ((e = launch_fill(B->scratch, B->scratch_words, c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_BATCH, blocks, c->threads, false, lds, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_BATCH, blocks, c->threads, c->count_ops, lds, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipEventSynchronize(c->ev1))) break;
         e = hipEventElapsedTime(&ms, c->ev0, c->ev1);

@@ -74,12 +74,15 @@ enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3 };
 // Bounding volume hierarchy node: FP32 bounds padded outward on the host (so FP32 slab
 // tests never drop a true hit); a >= 0: children a and b; a < 0: leaf of prims
 // [-a - 1, -a - 1 + b).  32 bytes.
+// Binary BVH node holding both children's bounds (outward padded, FP32), so one 64-byte
+// load decides which children a ray or a ball query enters and in which order.
 struct BvhNode {
-    float lo[3];
-    float hi[3];
-    int32_t a, b;
+    float lo[2][3];
+    float hi[2][3];
+    int32_t child[2];   // >= 0: internal node index; < 0: leaf, prims from -(child + 1)
+    int32_t count[2];   // leaf prim count (0 for an internal child)
 };
-static_assert(sizeof(BvhNode) == 32, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
 constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
 
 // A batch of structures merged into one mesh geometry (shared box grain, disjoint grain

@@ -30,6 +30,8 @@ constexpr double kBallistic = 1.0e10;                        // RT:1686
 struct OpCount {
     unsigned long long flops;
     unsigned long long philox;     // Philox4x32-10 blocks (integer VALU work)
+    unsigned long long nodes;      // BVH nodes visited (scan and ball)
+    unsigned long long tests;      // BVH leaf surfaces tested in FP64
 };
 
 constexpr unsigned kFlopsLog = 33;     // pt_log: 4 + Horner 20 + 5 + 4
@@ -674,6 +676,74 @@ __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *b
     }
 }
 
+// FP32 ray for BVH slab tests.  An axis the direction barely moves along (|d| <= 1e-20,
+// i.e. less than 1e-10 nm over 1e10 nm) is taken as parallel: the origin must lie in
+// the slab, which the 1e-3 nm box padding makes safe.
+struct RayF32 {
+    float o[3], inv[3];
+    bool mv[3];
+};
+
+__device__ inline RayF32 ray_f32(const double *pos, const double *dir)
+{
+    RayF32 r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.o[k] = (float)pos[k];
+        const float d = (float)dir[k];
+        r.mv[k] = fabsf(d) > 1.0e-20f;
+        r.inv[k] = r.mv[k] ? 1.0f / d : 0.0f;
+    }
+    return r;
+}
+
+// Entry parameter of the ray into a padded box; false when it misses (margins 1e-5
+// relative on the parameters).
+__device__ inline bool slab_enter(const RayF32 &r, const float *lo, const float *hi, float *tin)
+{
+    float t0 = -3.0e38f, t1 = 3.0e38f;
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (r.mv[k]) {
+            const float a0 = (lo[k] - r.o[k]) * r.inv[k], a1 = (hi[k] - r.o[k]) * r.inv[k];
+            t0 = fmaxf(t0, fminf(a0, a1));
+            t1 = fminf(t1, fmaxf(a0, a1));
+        } else {
+            in = in && !(r.o[k] < lo[k] || r.o[k] > hi[k]);
+        }
+    }
+    *tin = t0;
+    return in && !(t1 < -1.0e-5f * fabsf(t1) || t0 > t1 + 1.0e-5f * fabsf(t1));
+}
+
+// The FP64 surface tests of one BVH leaf; keeps the smallest (distance, surface index).
+__device__ inline void bvh_leaf_hits(const DevGeom &g, const DevBatch &bt, const Path &ph,
+                                     int first, int count, double *bl, int *bi, bool *hit,
+                                     OpCount &oc)
+{
+    for (int k = first; k < first + count; ++k) {
+        const int idx = bt.prims[k];
+        double l;
+        ++oc.tests;
+        if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) &&
+            (l < *bl || (*hit && l == *bl && idx < *bi))) {
+            *bl = l;
+            *bi = idx;
+            *hit = true;
+        }
+    }
+}
+
+// Squared distance from a point to a padded box (0 inside).
+__device__ inline float box_dist2(const float *lo, const float *hi, float px, float py, float pz)
+{
+    const float ex = fmaxf(fmaxf(lo[0] - px, px - hi[0]), 0.0f);
+    const float ey = fmaxf(fmaxf(lo[1] - py, py - hi[1]), 0.0f);
+    const float ez = fmaxf(fmaxf(lo[2] - pz, pz - hi[2]), 0.0f);
+    return ex * ex + ey * ey + ez * ez;
+}
+
 // findCollisionSurface (RT:1989-2096) for batch structures: the three facing box walls,
 // then the current grain's candidates through its BVH.  Node tests are FP32 slab tests
 // against host-padded bounds (they only prune); surfaces are tested by the reference's
@@ -699,48 +769,30 @@ __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, d
     }
     const int root = ph.which >= 0 ? bt.grain_root[ph.which] : -1;
     if (root >= 0) {
-        const float ox = (float)ph.pos[0], oy = (float)ph.pos[1], oz = (float)ph.pos[2];
-        const float dx = (float)ph.dir[0], dy = (float)ph.dir[1], dz = (float)ph.dir[2];
-        const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-        int sp = 0;
+        const RayF32 rf = ray_f32(ph.pos, ph.dir);
+        int sp = 1;
         stk[0] = root;
-        sp = 1;
         while (sp > 0) {
-            const int ni = stk[--sp * 256];
-            const BvhNode nd = bt.nodes[ni];
-            // slab test; an axis the ray does not move along passes if the origin is in
-            float t0 = -3.0e38f, t1 = 3.0e38f;
-            if (dx != 0.0f) {
-                const float a0 = (nd.lo[0] - ox) * ix, a1 = (nd.hi[0] - ox) * ix;
-                t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
-            } else if (ox < nd.lo[0] || ox > nd.hi[0]) continue;
-            if (dy != 0.0f) {
-                const float a0 = (nd.lo[1] - oy) * iy, a1 = (nd.hi[1] - oy) * iy;
-                t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
-            } else if (oy < nd.lo[1] || oy > nd.hi[1]) continue;
-            if (dz != 0.0f) {
-                const float a0 = (nd.lo[2] - oz) * iz, a1 = (nd.hi[2] - oz) * iz;
-                t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
-            } else if (oz < nd.lo[2] || oz > nd.hi[2]) continue;
-            // margins: 1e-5 relative on the parameters (the padded bounds cover the rest)
-            if (t1 < -1.0e-5f * fabsf(t1) || t0 > t1 + 1.0e-5f * fabsf(t1)) continue;
-            if (hit && (double)t0 * (1.0 - 1.0e-5) > bl) continue;
-            if (nd.a < 0) {
-                const int f = -nd.a - 1;
-                for (int k = f; k < f + nd.b; ++k) {
-                    const int idx = bt.prims[k];
-                    double l;
-                    if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) &&
-                        (l < bl || (hit && l == bl && idx < bi))) {
-                        bl = l;
-                        bi = idx;
-                        hit = true;
-                    }
-                }
-            } else if (sp + 2 <= kBvhStack) {
-                stk[sp++ * 256] = nd.b;
-                stk[sp++ * 256] = nd.a;
-            }
+            const BvhNode nd = bt.nodes[stk[--sp * 256]];
+            ++oc.nodes;
+            float t0, t1;
+            bool g0 = slab_enter(rf, nd.lo[0], nd.hi[0], &t0);
+            bool g1 = slab_enter(rf, nd.lo[1], nd.hi[1], &t1);
+            // margin 1e-5 relative on the entry parameter (the padded bounds cover the rest)
+            g0 = g0 && !(hit && (double)t0 * (1.0 - 1.0e-5) > bl);
+            g1 = g1 && !(hit && (double)t1 * (1.0 - 1.0e-5) > bl);
+            // nearer child first: its hits can prune the farther one
+            const bool sw = g1 && (!g0 || t1 < t0);
+            const int nc = sw ? nd.child[1] : nd.child[0], fc = sw ? nd.child[0] : nd.child[1];
+            const int nn = sw ? nd.count[1] : nd.count[0], fn = sw ? nd.count[0] : nd.count[1];
+            const bool ng = sw ? g1 : g0;
+            bool fg = sw ? g0 : g1;
+            const float ft = sw ? t0 : t1;
+            if (ng && nc < 0) bvh_leaf_hits(g, bt, ph, -nc - 1, nn, &bl, &bi, &hit, oc);
+            fg = fg && !(hit && (double)ft * (1.0 - 1.0e-5) > bl);
+            if (fg && fc < 0) bvh_leaf_hits(g, bt, ph, -fc - 1, fn, &bl, &bi, &hit, oc);
+            if (fg && fc >= 0 && sp < kBvhStack) stk[sp++ * 256] = fc;
+            if (ng && nc >= 0 && sp < kBvhStack) stk[sp++ * 256] = nc;
         }
     }
     *dmin = bl;
@@ -797,10 +849,35 @@ __device__ void ball_setup(const DevGeom &g, Path &ph)
     ph.br = r * (1.0f - 1.0e-5f) - 1.0e-2f;
 }
 
+// Lower bound on the distance from a point to the surfaces of one BVH leaf, as in
+// ball_setup.
+__device__ inline float bvh_leaf_bound(const DevGeom &g, const DevBatch &bt, int first, int count,
+                                       float px, float py, float pz)
+{
+    float r = 3.0e38f;
+    for (int k = first; k < first + count; ++k) {
+        const DSurf s = g.surf[bt.prims[k]];
+        float d;
+        if (s.kind == PT_SHAPE_POLYGON) {
+            const double *v = g.vtx + 3 * (int64_t)s.voff;
+            const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
+            const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny + (pz - (float)v[2]) * nz;
+            d = fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+        } else {
+            const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
+            const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
+            d = fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
+        }
+        r = fminf(r, d);
+    }
+    return r;
+}
+
 // ball_setup for batch structures: the box walls, then the current grain's candidates
 // through its BVH, pruning nodes whose (outward-padded) box is farther than the radius
 // found so far.  Leaf surfaces are bounded as in ball_setup.
-__device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, int *stk)
+__device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, int *stk,
+                               OpCount &oc)
 {
     const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
     float r = fminf(fminf(fminf(px, (float)g.box[0] - px), fminf(py, (float)g.box[1] - py)),
@@ -812,32 +889,17 @@ __device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, i
         stk[0] = root;
         while (sp > 0) {
             const BvhNode nd = bt.nodes[stk[--sp * 256]];
-            const float ex = fmaxf(fmaxf(nd.lo[0] - px, px - nd.hi[0]), 0.0f);
-            const float ey = fmaxf(fmaxf(nd.lo[1] - py, py - nd.hi[1]), 0.0f);
-            const float ez = fmaxf(fmaxf(nd.lo[2] - pz, pz - nd.hi[2]), 0.0f);
-            if (ex * ex + ey * ey + ez * ez >= r * r) continue;
-            if (nd.a < 0) {
-                const int f = -nd.a - 1;
-                for (int k = f; k < f + nd.b; ++k) {
-                    const DSurf s = g.surf[bt.prims[k]];
-                    float d;
-                    if (s.kind == PT_SHAPE_POLYGON) {
-                        const double *v = g.vtx + 3 * (int64_t)s.voff;
-                        const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
-                        const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny +
-                                        (pz - (float)v[2]) * nz;
-                        d = fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
-                    } else {
-                        const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
-                        const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
-                        d = fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
-                    }
-                    r = fminf(r, d);
-                }
-            } else if (sp + 2 <= kBvhStack) {
-                stk[sp++ * 256] = nd.b;
-                stk[sp++ * 256] = nd.a;
-            }
+            ++oc.nodes;
+            const float d0 = box_dist2(nd.lo[0], nd.hi[0], px, py, pz);
+            const float d1 = box_dist2(nd.lo[1], nd.hi[1], px, py, pz);
+            const bool sw = d1 < d0;                 // nearer child first
+            const int nc = sw ? nd.child[1] : nd.child[0], fc = sw ? nd.child[0] : nd.child[1];
+            const int nn = sw ? nd.count[1] : nd.count[0], fn = sw ? nd.count[0] : nd.count[1];
+            const float dn = sw ? d1 : d0, df = sw ? d0 : d1;
+            if (dn < r * r && nc < 0) r = fminf(r, bvh_leaf_bound(g, bt, -nc - 1, nn, px, py, pz));
+            if (df < r * r && fc < 0) r = fminf(r, bvh_leaf_bound(g, bt, -fc - 1, fn, px, py, pz));
+            if (df < r * r && fc >= 0 && sp < kBvhStack) stk[sp++ * 256] = fc;
+            if (dn < r * r && nc >= 0 && sp < kBvhStack) stk[sp++ * 256] = nc;
         }
     }
     ph.bc[0] = px;
@@ -1279,7 +1341,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     if (kPrec != kPrecFP64) {
         const double lim = 0.05 * fmin(g.box[0], fmin(g.box[1], g.box[2]));
         if (scattered && !in_safe && ph.mfp < lim) {
-            if (kBatch) ball_setup_bvh(g, *bt, ph, stk);
+            if (kBatch) ball_setup_bvh(g, *bt, ph, stk, oc);
             else ball_setup(g, ph);
         }
     }
@@ -1426,7 +1488,7 @@ trace_kernel(TraceParams P)
         if (kBatch && blockIdx.x == 0 && threadIdx.x == 0) *P.bt.start = wall_clock64();
         __syncthreads();
     }
-    OpCount oc = {0ull, 0ull};
+    OpCount oc = {0ull, 0ull, 0ull, 0ull};
     Rng rng;
     rng.oc = &oc;
     rng.seed = P.seed;
@@ -1546,9 +1608,12 @@ trace_kernel(TraceParams P)
     }
     if (kCount) {
         const unsigned long long f = wave_sum(oc.flops), b = wave_sum(oc.philox);
+        const unsigned long long bn = wave_sum(oc.nodes), bs = wave_sum(oc.tests);
         if (lane == 0) {
             atomicAdd(P.stats + 7, f);
             atomicAdd(P.stats + 8, b);
+            atomicAdd(P.stats + 12, bn);
+            atomicAdd(P.stats + 13, bs);
         }
     }
 }
@@ -1602,7 +1667,7 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    OpCount oc = {0ull, 0ull};
+    OpCount oc = {0ull, 0ull, 0ull, 0ull};
     Rng rng;
     rng.oc = &oc;
     rng.seed = P.seed;
