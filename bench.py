@@ -2,11 +2,12 @@
 
 Workload (the thin-film configuration): a 600 x 600 x 200 nm film, transport along x,
 three z-cylindrical holes of radius 50 nm (seeded, non-overlapping; specularity 0.3,
-vacuum t = 0), diffuse film faces (p = 0), mirror in-plane side walls (p = 1, the
-reference has no periodic boundary); synthetic bulk table 6 branches x 2000 frequency
+vacuum t = 0), diffuse film faces (p = 0), periodic in-plane side walls (y; the
+reference's walls always reflect, so periodic walls are this library's extension and
+the oracle's restatement of them); synthetic bulk table 6 branches x 2000 frequency
 bins; 11 polar angles (10 traced); Np particles per (k, b, angle) cell chosen so one
 step traces ~1e8 phonon paths; ballistic (Matthiessen) trace in FP64 exactly as
-calcTransmit (src/RTMCphon.f90:1608-2414).  One step = one pt_trace_grid call, i.e.
+calcTransmit (src/RTMCphon.f90:1608-2414) apart from the periodic wrap.  One step = one pt_trace_grid call, i.e.
 TransmissionFunction (RT:1544-1604) for the whole table; inputs are resident in HBM
 before timing starts.
 
@@ -19,6 +20,12 @@ specularity; same bulk table): the alias-mode forward trace and the likelihood-r
 adjoint (12 specularity + 6 branch-scale derivatives of kappa) over 1e8 paths, and
 kappa-gradient evaluations/s = adjoint paths/s / the paths that make the gradient
 2-sigma significant (|stderr| = |grad| / 2).
+
+The "grains" object measures the polycrystal configuration: a 1000 nm cube cut into 5
+Voronoi grains whose shared boundaries transmit with the Lorentzian law
+t = 1/(1 + (omega/omega0)^2), omega0 = 2 pi f0, f0 ~ U(3, 10) THz per boundary; alias
+mode over 1e9 paths with per-grain heat-flux tallies (each grain's share of kappa), FP32
+trace against FP64.
 
 The "batch" object measures the screening configuration: 256 jittered simple-cubic grain
 structures (4^3 grains, 2e3 - 1e5 triangles each, log-uniform) traced together in one
@@ -54,7 +61,7 @@ METRIC = "phonon paths/s and κ-gradient evals/s on 1 MI355X at fixed κ std-err
 def workload(np_per_cell: int, nangle: int = 11, nb: int = 6, nk: int = 2000):
     model = gen.film_holes(seed=2, lx=600.0, ly=600.0, h=200.0, radius=50.0, nholes=3,
                            p_hole=0.3, p_film=0.0, p_side=1.0, nphonon=np_per_cell, nangle=nangle)
-    geo = geometry.build_geometry(model)
+    geo = geometry.set_periodic(geometry.build_geometry(model), axes=(1,))
     t = synthetic_table(nb, nk, seed=20)
     omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12          # RT:727
     table = Anphon(material="synthetic", rtemp=t.rtemp, vol=t.vol, omega=omega, tau=t.tau_ps,
@@ -130,7 +137,8 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
     ctx.set_seed(778)
     _, _, pilot, _ = ctx.adjoint_alias(g, m, 1_000_000, nbatch=1, precision=FAST)
     base = pilot[0, 1] / pilot[0, 0]
-    t0 = time.perf_counter()
+    ctx.set_seed(777)              # the forward run's paths: both launches end with the same
+    t0 = time.perf_counter()       # longest path, so the ratio is the adjoint's own cost
     _, _, sums, ast = ctx.adjoint_alias(g, m, npaths, surface_group=group, ngroup=ngroup,
                                         baseline=base, nbatch=256, precision=FAST)
     t_adj = time.perf_counter() - t0
@@ -161,6 +169,46 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
         "paths_for_2sigma_gradient": n2s,
         "kappa_gradient_evals_per_s": adj_rate / n2s if n2s else None,
         "events_per_path": ast["events"] / max(1, ast["paths"]),
+    }
+
+
+def grains_leg(ctx, npaths: int, table_seed: int = 20):
+    """C4 polycrystal with Lorentzian boundaries and per-grain flux tallies (see the
+    module docstring)."""
+    model = gen.voronoi_cube(seed=4, L=1000.0, ngrain=5, kind="Gamma", value=2.0, p=0.2,
+                             nphonon=1, nangle=3)
+    geo = geometry.build_geometry(model)
+    w0 = geometry.set_lorentz_boundaries(geo, np.random.default_rng(41), f0_thz=(3.0, 10.0))
+    t = synthetic_table(6, 2000, seed=table_seed)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    g = ctx.geometry(geo)
+    m = ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    out = {}
+    for name, prec in (("fp32", PREC_FP32_RETEST), ("fp64", PREC_FP64)):
+        ctx.set_seed(2024)
+        mp, mt, gt, st = ctx.trace_alias_grains(g, m, npaths, precision=prec)
+        k = post.alias_kappa(mp, mt, t.mfp_nm, geo.box[0])
+        out[name] = dict(k=k, shares=post.grain_kappa_shares(gt, k.paths), st=st)
+        print("grains: %s %.1f ms" % (name, st["kernel_ms"]), file=sys.stderr, flush=True)
+    g.close()
+    a, b = out["fp32"], out["fp64"]
+    sig = float(np.hypot(a["k"].stderr, b["k"].stderr))
+    return {
+        "workload": "1000 nm cube, 5 Voronoi grains, Lorentzian grain-boundary transmission "
+                    "(f0 ~ U(3, 10) THz, %d boundaries), p = 0.2, 6x2000 modes, alias mode, "
+                    "per-grain heat-flux tallies, FP32 trace" % len(w0),
+        "paths": int(a["k"].paths),
+        "paths_per_s": a["k"].paths / (a["st"]["kernel_ms"] * 1e-3),
+        "kernel_ms": a["st"]["kernel_ms"],
+        "events_per_path": a["st"]["events"] / max(1, a["st"]["paths"]),
+        "fp64_paths_per_s": b["k"].paths / (b["st"]["kernel_ms"] * 1e-3),
+        "speedup_vs_fp64": b["st"]["kernel_ms"] / a["st"]["kernel_ms"],
+        "kappa_ratio": a["k"].ratio, "kappa_ratio_stderr": a["k"].stderr,
+        "kappa_ratio_fp64": b["k"].ratio,
+        "kappa_rel_err_vs_fp64": abs(a["k"].ratio - b["k"].ratio) / b["k"].ratio,
+        "kappa_diff_in_sigma": abs(a["k"].ratio - b["k"].ratio) / sig,
+        "grain_shares": [round(float(v), 6) for v in a["shares"][1:]],
+        "grain_shares_sum_vs_kappa": float(a["shares"].sum() / a["k"].ratio),
     }
 
 
@@ -266,6 +314,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-paths", type=float, default=1e8,
                     help="paths of the inclusions adjoint leg (0: skip it)")
+    ap.add_argument("--grain-paths", type=float, default=1e9,
+                    help="paths of the Voronoi/Lorentzian grains leg (0: skip it)")
     ap.add_argument("--batch-structures", type=int, default=256,
                     help="structures of the screening batch leg (0: skip it)")
     ap.add_argument("--batch-paths", type=float, default=1e5,
@@ -316,7 +366,12 @@ def main():
             ctx.set_op_counting(False)
             res[prec] = dict(dt=dt, kernel_ms=float(np.mean(kern_ms)), counts=counts, stats=stats,
                              cstats=cstats)
+            print("film: precision %d, %.2f ms per step" % (prec, dt / args.steps * 1e3),
+                  file=sys.stderr, flush=True)
         adj = adjoint_leg(ctx, int(args.adjoint_paths)) if args.adjoint_paths > 0 else None
+        if adj:
+            print("adjoint: %.1f ms" % adj["adjoint_kernel_ms"], file=sys.stderr, flush=True)
+        grn = grains_leg(ctx, int(args.grain_paths)) if args.grain_paths > 0 else None
         bat = (batch_leg(ctx, args.batch_structures, int(args.batch_paths))
                if args.batch_structures > 0 else None)
     fast, plain, filt = res[PREC_FP32_RETEST], res[PREC_FP64], res[PREC_FP32_FILTER]
@@ -355,6 +410,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded film geometry and bulk table)",
         "config": {"workload": "thin film 600x600x200 nm, 3 z-cylinder holes r=50 nm p=0.3, "
+                               "periodic in-plane (y) side walls, "
                                "6x2000 modes x 10 angles x %d particles, ballistic grid "
                                "trace (TransmissionFunction)" % args.np,
                    "paths_per_step": paths, "events_per_path": events_per_path,
@@ -379,6 +435,7 @@ def main():
         "roofline": roofline(cstats, k_ms, "fp32_trace"),
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "grains": grn,
         "batch": bat,
     }
     print(json.dumps(out))

@@ -164,6 +164,7 @@ typedef struct pt_stats {
     int64_t fallbacks;       /* PT_PREC_FP32_RETEST: events decided by the FP64 scan  */
     int64_t bvh_nodes;       /* batch trace: BVH nodes visited (op counting on only)  */
     int64_t bvh_tests;       /* batch trace: leaf surfaces tested (op counting on)    */
+    int64_t event_limited;   /* paths stopped by pt_ctx_set_event_limit's limit       */
 } pt_stats;
 
 /* Per-path record of pt_replay(). */

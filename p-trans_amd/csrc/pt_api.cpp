@@ -644,6 +644,7 @@ void fill_stats(pt_stats *stats, const unsigned long long *h, float ms, float ms
     stats->fallbacks = (int64_t)h[12];
     stats->bvh_nodes = (int64_t)h[13];
     stats->bvh_tests = (int64_t)h[14];
+    stats->event_limited = (int64_t)h[6];
 }
 
 int replay_common(pt_ctx *c, TraceParams &P, int mode, int64_t first, int64_t n,

@@ -1266,8 +1266,13 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
                 surface_frame32(g, ph.cur, s, ph.pos, true, n, rot);
                 if (n[0] * d[0] + n[1] * d[1] + n[2] * d[2] < 0.0f)
                     for (int k = 0; k < 3; ++k) n[k] = -n[k];
-                const float st = __builtin_amdgcn_sqrtf((float)rng.next());  // RT:2213
-                const float ct = __builtin_amdgcn_sqrtf(1.0f - st * st);
+                // u = sin^2(theta), RT:2213.  (float)u is 1.0f for u > 1 - 2^-25; cos(theta)
+                // = 0 would leave the surface exactly tangent, and the FP32 frame's rounding
+                // could then put the path on the wall's other side (inside a hole, say, where
+                // it stays).  Below 1 - 2^-24, cos(theta) >= 2^-12, far above that rounding.
+                const float u2 = fminf((float)rng.next(), 0.99999994f);
+                const float st = __builtin_amdgcn_sqrtf(u2);
+                const float ct = __builtin_amdgcn_sqrtf(1.0f - u2);
                 float sph, cph;
                 fsincos_rev((float)rng.next(), &sph, &cph);                 // RT:2218
                 const float loc[3] = {st * cph, st * sph, ct};

@@ -71,7 +71,7 @@ module ptrans
          skipped = 0
     real(c_double) :: kernel_ms = 0.0d0, reduce_ms = 0.0d0
     integer(c_int64_t) :: fp64_ops = 0, philox_blocks = 0, retests = 0, fallbacks = 0
-    integer(c_int64_t) :: bvh_nodes = 0, bvh_tests = 0
+    integer(c_int64_t) :: bvh_nodes = 0, bvh_tests = 0, event_limited = 0
   end type pt_stats
 
   type, bind(C) :: pt_path_record

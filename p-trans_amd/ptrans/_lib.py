@@ -34,7 +34,7 @@ class PtStats(ctypes.Structure):
                 ("reduce_ms", ctypes.c_double), ("fp64_ops", ctypes.c_int64),
                 ("philox_blocks", ctypes.c_int64), ("retests", ctypes.c_int64),
                 ("fallbacks", ctypes.c_int64), ("bvh_nodes", ctypes.c_int64),
-                ("bvh_tests", ctypes.c_int64)]
+                ("bvh_tests", ctypes.c_int64), ("event_limited", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

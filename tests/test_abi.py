@@ -35,7 +35,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(PtGeomDesc) == 112
     assert ctypes.sizeof(PtModesDesc) == 56
     assert _lib.RECORD_DTYPE.itemsize == 80
-    assert ctypes.sizeof(_lib.PtStats) == 112
+    assert ctypes.sizeof(_lib.PtStats) == 120
 
 
 def test_setup_polygon_helper_matches_python_host():

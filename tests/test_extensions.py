@@ -186,3 +186,35 @@ def test_gpu_grain_tallies_vs_oracle(gpu_ctx, precision):
     np.testing.assert_array_equal(gp, mp)
     np.testing.assert_array_equal(gtr, mt)
     np.testing.assert_allclose(gg, gt, rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("periodic", [True, False])
+def test_gpu_fp32_film_paths_all_end(gpu_ctx, periodic):
+    """FP32 trace of the 600 x 600 x 200 nm film with 3 holes, 6 x 2000 modes x 10 angles x
+    834 particles (1e8 paths): no path reaches a 1e6-event limit.  A diffuse reflection
+    whose FP32 draw rounds to 1.0f leaves the surface exactly tangent; without the clamp
+    of that draw, FP32 rounding of the frame sent such paths into a vacuum hole, where they
+    bounced until the event limit (about one path in 1e8 on this film)."""
+    from ptrans.anphon import synthetic_table
+    from ptrans.constants import M_TWOPI
+    m = gen.film_holes(seed=2, lx=600.0, ly=600.0, h=200.0, radius=50.0, nholes=3, p_hole=0.3,
+                       p_film=0.0, p_side=1.0, nphonon=834, nangle=11)
+    geo = geometry.build_geometry(m)
+    if periodic:
+        geometry.set_periodic(geo, axes=(1,))
+    t = synthetic_table(6, 2000, seed=20)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    g = gpu_ctx.geometry(geo)
+    mm = gpu_ctx.modes(omega=omega)
+    gpu_ctx.set_seed(12345)
+    gpu_ctx.set_event_limit(1_000_000)
+    try:
+        print("film periodic=%s, FP32 trace of 1e8 paths" % periodic, flush=True)
+        counts, st = gpu_ctx.trace_grid(g, mm, 1.0e10, 11, 834, precision=1)
+    finally:
+        gpu_ctx.set_event_limit(100_000_000)
+    print("events/path %.3f, kernel %.1f ms" % (st["events"] / st["paths"], st["kernel_ms"]), flush=True)
+    assert st["paths"] == 6 * 2000 * 10 * 834
+    assert st["event_limited"] == 0
+    assert st["lost"] == 0

@@ -35,7 +35,7 @@ def test_fortran_types_have_c_layout():
     import ctypes
     assert sizes == [SURFACE_DTYPE.itemsize, ctypes.sizeof(PtGeomDesc),
                      ctypes.sizeof(PtModesDesc), ctypes.sizeof(_lib.PtStats),
-                     _lib.RECORD_DTYPE.itemsize] == [184, 112, 56, 112, 80]
+                     _lib.RECORD_DTYPE.itemsize] == [184, 112, 56, 120, 80]
 
 
 def test_rtmc_adapter_compiles_against_reference_module():
