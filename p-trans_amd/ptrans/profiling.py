@@ -2,9 +2,15 @@
 
     python -m ptrans.profiling kernels  <run_results.db>
     python -m ptrans.profiling counters <run_results.db> [kernel-substring]
+    python -m ptrans.profiling traffic  <kt.db> <fetch.db> <write.db> <copy_fetch.db>
+                                        <copy_write.db> <sq.db> <copy_mib> <paths> <workload>
 
 'kernels' prints per-kernel call count and average/total duration; 'counters' prints,
-per kernel and counter, the mean value per dispatch.
+per kernel and counter, the mean value per dispatch.  'traffic' prints the JSON summary
+the bench's roofline reads: per trace-kernel precision its average duration (kernel
+trace), HBM bytes per launch (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the
+same counter reads for a device copy of copy_mib MiB) and VALU wave-instructions per
+launch and per path with the issue fraction over 256 CUs x 2.4 GHz.
 """
 from __future__ import annotations
 
@@ -34,7 +40,50 @@ def counter_means(db: str, kernel_filter: str = ""):
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
+PEAK_VALU_WAVE_INSTR_PER_S = 256 * 2.4e9      # one VALU wave-instruction per CU per clock
+
+# trace_kernel<mode, counting, precision> names of the uncounted grid kernels
+TRACE_KERNELS = {"fp64": "trace_kernel<0, false, 0>", "fp32_filter": "trace_kernel<0, false, 2>",
+                 "fp32_trace": "trace_kernel<0, false, 1>"}
+
+
+def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy_mib: float,
+                    paths: int, workload: str):
+    copy_bytes = copy_mib * 2 ** 20
+
+    def one(db, counter, sub):
+        for (name, cname), (mean, _n) in counter_means(db, sub).items():
+            if cname == counter:
+                return name, mean
+        raise KeyError("%s of %s not in %s" % (counter, sub, db))
+
+    _, cf = one(cfetch_db, "FETCH_SIZE", "copyBuffer")
+    _, cw = one(cwrite_db, "WRITE_SIZE", "copyBuffer")
+    fscale, wscale = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    kt = {k["name"]: k for k in kernel_stats(kt_db)}
+    out = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale,
+           "peak_valu_wave_instr_per_s": PEAK_VALU_WAVE_INSTR_PER_S, "kernels": {}}
+    for key, sub in TRACE_KERNELS.items():
+        name = next(n for n in kt if sub in n)
+        avg_us = kt[name]["average_ns"] / 1e3
+        _, f = one(fetch_db, "FETCH_SIZE", sub)
+        _, w = one(write_db, "WRITE_SIZE", sub)
+        _, v = one(sq_db, "SQ_INSTS_VALU", sub)
+        fb, wb = f * 1024.0 * fscale, w * 1024.0 * wscale
+        out["kernels"][key] = {
+            "kernel": name, "average_us": avg_us,
+            "hbm_fetch_bytes": fb, "hbm_write_bytes": wb, "hbm_bytes_per_launch": fb + wb,
+            "valu_wave_instr_per_launch": v, "valu_wave_instr_per_path": v / paths,
+            "valu_issue_fraction": v / (avg_us * 1e-6) / PEAK_VALU_WAVE_INSTR_PER_S}
+    return out
+
+
 def main(argv):
+    if len(argv) >= 11 and argv[1] == "traffic":
+        import json
+        print(json.dumps(traffic_summary(*argv[2:8], float(argv[8]), int(argv[9]), argv[10]),
+                         indent=1))
+        return 0
     if len(argv) < 3 or argv[1] not in ("kernels", "counters"):
         print(__doc__)
         return 2
