@@ -168,6 +168,8 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
         "grad_norm": gnorm, "grad_stderr_norm": snorm,
         "paths_for_2sigma_gradient": n2s,
         "kappa_gradient_evals_per_s": adj_rate / n2s if n2s else None,
+        "paths_for_1e-3_rel_stderr": npaths * (k32.stderr / (1e-3 * k32.ratio)) ** 2,
+        "seconds_for_1e-3_rel_stderr": npaths * (k32.stderr / (1e-3 * k32.ratio)) ** 2 * t_fwd / npaths,
         "events_per_path": ast["events"] / max(1, ast["paths"]),
     }
 
@@ -207,6 +209,9 @@ def grains_leg(ctx, npaths: int, table_seed: int = 20):
         "kappa_ratio_fp64": b["k"].ratio,
         "kappa_rel_err_vs_fp64": abs(a["k"].ratio - b["k"].ratio) / b["k"].ratio,
         "kappa_diff_in_sigma": abs(a["k"].ratio - b["k"].ratio) / sig,
+        "paths_for_1e-3_rel_stderr": a["k"].paths * (a["k"].stderr / (1e-3 * a["k"].ratio)) ** 2,
+        "seconds_for_1e-3_rel_stderr": (a["k"].paths * (a["k"].stderr / (1e-3 * a["k"].ratio)) ** 2
+                                        * a["st"]["kernel_ms"] * 1e-3 / a["k"].paths),
         "grain_shares": [round(float(v), 6) for v in a["shares"][1:]],
         "grain_shares_sum_vs_kappa": float(a["shares"].sum() / a["k"].ratio),
     }
@@ -255,7 +260,17 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
     var64 = np.maximum(s64[:, 2] / s64[:, 0] - (s64[:, 1] / s64[:, 0]) ** 2, 0.0)
     sig = c * np.sqrt(var32 / n + var64 / s64[:, 0])
     rel = np.abs(k32 - k64) / k64
+    # pooled over structures: the mean relative difference and its standard error
+    d = (k32 - k64) / k64
+    pooled, pooled_se = float(d.mean()), float(d.std(ddof=1) / np.sqrt(len(d)))
     grad = c * sums[:, 3] / n
+    gvar = np.maximum(sums[:, 4] / n - (sums[:, 3] / n) ** 2, 0.0)
+    gse = c * np.sqrt(gvar / n)
+    gnorm, snorm = float(np.linalg.norm(grad)), float(np.linalg.norm(gse))
+    n2s = float(n.sum()) * (snorm / (0.5 * gnorm)) ** 2 if gnorm > 0 else None
+    rate = st["paths"] / (st["kernel_ms"] * 1e-3)
+    ks = c * np.sqrt(var32 / n)
+    need = n * (ks / (1e-3 * k32)) ** 2
     return {
         "workload": "%d jittered 4x4x4 simple-cubic grain structures (1000 nm cube, +-5 nm "
                     "jitter, Gamma law, p = 0.3), %d - %d triangles each, 3x1000 modes, "
@@ -272,7 +287,14 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
         "kappa_rel_err_vs_fp64_median": float(np.median(rel)),
         "kappa_rel_err_vs_fp64_max": float(np.max(rel)),
         "kappa_diff_sigma_max": float(np.max(np.abs(k32 - k64) / sig)),
+        "kappa_rel_diff_vs_fp64_pooled": pooled, "kappa_rel_diff_vs_fp64_pooled_stderr": pooled_se,
+        "paths_for_1e-3_rel_stderr_per_structure_median": float(np.median(need)),
+        "seconds_for_1e-3_rel_stderr_all_structures": float(need.sum() / rate),
         "mean_specularity_grad_median": float(np.median(grad)),
+        "mean_specularity_grad_stderr_median": float(np.median(gse)),
+        "gradient_components": nstruct,
+        "paths_for_2sigma_gradient": n2s,
+        "kappa_gradient_evals_per_s": rate / n2s if n2s else None,
         "autotune": {k: v for k, v in tune.items() if k != "table"},
         "autotune_configs": len(tune["table"]),
         "host_generate_s": t_gen, "host_build_s": t_build,

@@ -350,7 +350,8 @@ int pt_batch_info(const pt_batch *batch, int64_t *info);
 int pt_batch_set_specularity(pt_batch *batch, const double *mean);
 /*
  * One launch over the whole batch with the context's seed.  Per structure s, adds to
- * sums[4 s .. 4 s + 3]: paths, sum x, sum x^2 and sum (x - baseline[s]) * score_s with
+ * sums[5 s .. 5 s + 4]: paths, sum x, sum x^2, sum (x - baseline[s]) * score_s and the sum
+ * of its squares (for the gradient's standard error), with
  * x = T / mfp_m (kappa/kappa_bulk = (3L/4) E[x]) and score_s the likelihood-ratio score
  * of the structure's mean specularity (1/p per specular, -1/(1-p) per diffuse decision on
  * its facets; baseline may be NULL for 0).  finish_ms[s]: time from the kernel's start to

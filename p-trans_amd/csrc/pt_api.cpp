@@ -997,7 +997,7 @@ struct pt_batch {
     std::vector<void *> owned;
     DevBatch bt{};
     int32_t *d_group = nullptr;            // surface -> 0 (structure facets) or -1 (box)
-    // scratch zeroed by every launch: counters(16) | sums(4n) | finish(n) | start(1) |
+    // scratch zeroed by every launch: counters(16) | sums(kBatchSums n) | finish(n) | start(1) |
     // done(ceil(n/2)) | mode tallies (2 nm)
     unsigned long long *scratch = nullptr;
     size_t scratch_words = 0, nm = 0;
@@ -1278,7 +1278,7 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     for (size_t i = 6; i < ns; ++i) group[i] = 0;
     B->nm = 0;
     const size_t n = (size_t)nstruct;
-    B->scratch_words = 16 + 4 * n + n + 1 + (n + 1) / 2;
+    B->scratch_words = 16 + kBatchSums * n + n + 1 + (n + 1) / 2;
     hipStream_t st = c->stream;
     int64_t *d_poff = nullptr;
     int32_t *d_root = nullptr, *d_prims = nullptr, *d_loff = nullptr, *d_lidx = nullptr;
@@ -1367,7 +1367,8 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
         precision != PT_PREC_FP32_FILTER)
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     const size_t nm = (size_t)M->nb * M->nk;
-    const size_t need = 16 + 4 * (size_t)B->nstruct + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm;
+    const size_t S = (size_t)kBatchSums * B->nstruct;
+    const size_t need = 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm;
     if (B->scratch_words < need || !B->scratch) {
         if (B->scratch) PT_HIP(hipFree(B->scratch));
         B->scratch = nullptr;
@@ -1390,11 +1391,11 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
     P.m.scale = nullptr;
     DevBatch bt = B->bt;
     bt.sums = (double *)(w + 16);
-    bt.finish = w + 16 + 4 * B->nstruct;
-    bt.start = w + 16 + 5 * B->nstruct;
-    bt.done = (unsigned int *)(w + 16 + 5 * B->nstruct + 1);
+    bt.finish = w + 16 + S;
+    bt.start = w + 16 + S + B->nstruct;
+    bt.done = (unsigned int *)(w + 16 + S + B->nstruct + 1);
     P.bt = bt;
-    P.mode_paths = w + 16 + 5 * B->nstruct + 1 + (B->nstruct + 1) / 2;
+    P.mode_paths = w + 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2;
     P.mode_trans = P.mode_paths + nm;
     P.adj = nullptr;
     *lds = adj_lds_bytes(P, c->threads);
@@ -1411,17 +1412,18 @@ int batch_finish(pt_batch *B, double *sums, double *finish_ms, pt_stats *stats, 
 {
     pt_ctx *c = B->ctx;
     const size_t n = (size_t)B->nstruct;
-    std::vector<unsigned long long> h(16 + 5 * n + 1);
+    const size_t S = (size_t)kBatchSums * n;
+    std::vector<unsigned long long> h(16 + S + n + 1);
     PT_HIP(hipMemcpyAsync(h.data(), B->scratch, h.size() * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, c->stream));
     PT_HIP(hipStreamSynchronize(c->stream));
     const double *hs = (const double *)(h.data() + 16);
     if (sums)
-        for (size_t i = 0; i < 4 * n; ++i) sums[i] += hs[i];
-    const unsigned long long t0 = h[16 + 5 * n];
+        for (size_t i = 0; i < S; ++i) sums[i] += hs[i];
+    const unsigned long long t0 = h[16 + S + n];
     if (finish_ms)
         for (size_t s = 0; s < n; ++s) {
-            const unsigned long long t = h[16 + 4 * n + s];
+            const unsigned long long t = h[16 + S + s];
             // wall_clock64 ticks at 100 MHz on gfx9
             finish_ms[s] = t >= t0 && t ? (double)(t - t0) * 1.0e-5 : -1.0;
         }

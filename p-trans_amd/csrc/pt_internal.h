@@ -84,6 +84,8 @@ struct BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
 constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
+// per-structure sums: n, sum x, sum x^2, sum (x - b) score, sum ((x - b) score)^2
+constexpr int kBatchSums = 5;
 
 // A batch of structures merged into one mesh geometry (shared box grain, disjoint grain
 // sets): per structure its path range, start locator and tallies; per grain the BVH of
@@ -98,7 +100,7 @@ struct DevBatch {
     const int32_t *loc_off;            // [nstruct * grid * grid + 1]
     const int32_t *loc_idx;            // incident facets overlapping each cell
     const double *baseline;            // [nstruct]
-    double *sums;                      // [nstruct * 4]: n, sum x, sum x^2, sum (x-b) score
+    double *sums;                      // [nstruct * kBatchSums]
     unsigned int *done;                // [nstruct] paths finished
     unsigned long long *finish;        // [nstruct] wall clock when the last path ended
     unsigned long long *start;         // wall clock at kernel start (one word)
@@ -147,7 +149,7 @@ __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.n
 inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 {
     return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt +
-            4 * (size_t)p.bt.nstruct) * sizeof(double) +
+            (size_t)kBatchSums * p.bt.nstruct) * sizeof(double) +
            (p.bt.nstruct ? (size_t)kBvhStack * threads * sizeof(int32_t) : 0);
 }
 

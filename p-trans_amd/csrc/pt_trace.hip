@@ -1483,12 +1483,13 @@ trace_kernel(TraceParams P)
     al.gacc = (kAdj && P.grain_tally) ? adj_acc + P.nbatch * adj_k : nullptr;
     al.fgrain = -1;
     al.fdx = al.fl = al.fw = 0.0;
-    // batch: per-structure sums [n, x, x^2, (x - b) score] and the lane's BVH stack
+    // batch: per-structure sums [n, x, x^2, (x - b) score, ((x - b) score)^2] and the
+    // lane's BVH stack
     double *st_acc = adj_acc + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
-    int *stk = (int *)(st_acc + (kBatch ? 4 * P.bt.nstruct : 0)) + threadIdx.x;
+    int *stk = (int *)(st_acc + (kBatch ? kBatchSums * P.bt.nstruct : 0)) + threadIdx.x;
     if (kAdj) {
         const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0) +
-                         (kBatch ? 4 * P.bt.nstruct : 0);
+                         (kBatch ? kBatchSums * P.bt.nstruct : 0);
         for (int i = threadIdx.x; i < nlds; i += blockDim.x) adj_lds[i] = 0.0;
         if (kBatch && blockIdx.x == 0 && threadIdx.x == 0) *P.bt.start = wall_clock64();
         __syncthreads();
@@ -1556,12 +1557,16 @@ trace_kernel(TraceParams P)
                 if (kBatch) {
                     const double x = transmitted ? 1.0 / P.m.mfp[mode] : 0.0;
                     const int sid = ph.sid;
-                    double *acc = st_acc + 4 * sid;
+                    double *acc = st_acc + kBatchSums * sid;
                     atomicAdd(acc + 0, 1.0);
                     atomicAdd(acc + 1, x);
                     atomicAdd(acc + 2, x * x);
                     const double v = al.score[0];
-                    if (v != 0.0) atomicAdd(acc + 3, (x - P.bt.baseline[sid]) * v);
+                    if (v != 0.0) {
+                        const double gs = (x - P.bt.baseline[sid]) * v;
+                        atomicAdd(acc + 3, gs);
+                        atomicAdd(acc + 4, gs * gs);
+                    }
                     al.score[0] = 0.0;
                     const unsigned n_s = (unsigned)(P.bt.path_off[sid + 1] - P.bt.path_off[sid]);
                     if (atomicAdd(P.bt.done + sid, 1u) + 1u == n_s) P.bt.finish[sid] = wall_clock64();
@@ -1602,7 +1607,7 @@ trace_kernel(TraceParams P)
             for (int i = threadIdx.x; i < 2 * P.ngt; i += blockDim.x)
                 if (al.gacc[i] != 0.0) atomicAdd(P.grain_tally + i, al.gacc[i]);
         if (kBatch)
-            for (int i = threadIdx.x; i < 4 * P.bt.nstruct; i += blockDim.x)
+            for (int i = threadIdx.x; i < kBatchSums * P.bt.nstruct; i += blockDim.x)
                 if (st_acc[i] != 0.0) atomicAdd(P.bt.sums + i, st_acc[i]);
     }
     // stats slots: paths, events, draws, lost, mismatch, limit -> stats[0..5];

@@ -255,9 +255,10 @@ class Batch:
         _lib.check(_lib.load().pt_batch_set_specularity(self.handle, m.ctypes.data))
 
     def trace(self, modes: Modes, precision: int = PREC_FP64, baseline=None, graph: bool = False):
-        """One launch: per-structure sums (nstruct, 4) of [paths, sum x, sum x^2,
-        sum (x - baseline) score], completion times (ms) and the stats."""
-        sums = np.zeros((self.nstruct, 4), dtype=np.float64)
+        """One launch: per-structure sums (nstruct, 5) of [paths, sum x, sum x^2,
+        sum (x - baseline) score, sum ((x - baseline) score)^2], completion times (ms) and
+        the stats."""
+        sums = np.zeros((self.nstruct, 5), dtype=np.float64)
         fin = np.zeros(self.nstruct, dtype=np.float64)
         st = _lib.PtStats()
         b = None if baseline is None else np.ascontiguousarray(baseline, dtype=np.float64)

@@ -144,19 +144,26 @@ def test_gpu_batch_mean_specularity_gradient_matches_fd(gpu_ctx):
     gpu_ctx.set_seed(100)
     pilot, _, _ = b.trace(m, precision=1)
     base = pilot[:, 1] / pilot[:, 0]
-    lr, fd = [], []
+    lr, fd, se = [], [], []
     for seed in range(8):
         gpu_ctx.set_seed(200 + seed)
         b.set_specularity(p0)
         s0, _, _ = b.trace(m, precision=1, baseline=base)
         lr.append(c * s0[:, 3] / s0[:, 0])
+        # standard error of one launch's LR estimate from its sum of squared terms
+        se.append(c * np.sqrt(np.maximum(s0[:, 4] / s0[:, 0] - (s0[:, 3] / s0[:, 0]) ** 2, 0.0)
+                              / s0[:, 0]))
         b.set_specularity(p0 + 1e-2)
         sp, _, _ = b.trace(m, precision=1)
         b.set_specularity(p0 - 1e-2)
         sm, _, _ = b.trace(m, precision=1)
         fd.append(c * (sp[:, 1] / sp[:, 0] - sm[:, 1] / sm[:, 0]) / 2e-2)
-    lr, fd = np.array(lr), np.array(fd)
+    lr, fd, se = np.array(lr), np.array(fd), np.array(se)
     for s in range(3):
+        # the spread over seeds agrees with the per-launch standard error (8 samples)
+        ratio = lr[:, s].std(ddof=1) / se[:, s].mean()
+        print("structure %d  seed spread / launch stderr %.2f" % (s, ratio), flush=True)
+        assert 0.3 < ratio < 2.5, ratio
         a, ea = lr[:, s].mean(), lr[:, s].std(ddof=1) / np.sqrt(8)
         f, ef = fd[:, s].mean(), fd[:, s].std(ddof=1) / np.sqrt(8)
         print("structure %d  LR %.5f +- %.5f  FD %.5f +- %.5f" % (s, a, ea, f, ef), flush=True)
