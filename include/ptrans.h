@@ -197,6 +197,16 @@ const char *pt_build_info(void);
  * Returns the message length. */
 size_t pt_last_error(char *buf, size_t len);
 
+/* Philox4x32-10 stream of the set-up draws (FluctuationSpecularity's per-surface jitter,
+ * RT:1151-1157, re-keyed from the runtime's RANDOM_NUMBER): path id 2^62, outside every
+ * trace's path range. */
+#define PT_SETUP_STREAM 4611686018427387904ull
+
+/* Host-side draws first_draw .. first_draw + n - 1 of stream `path` under `seed`: the same
+ * 53-bit uniforms in [0, 1) the kernels draw for path `path` (no GPU needed).  Hosts use it
+ * for set-up randomness that must follow the library's streams. */
+int pt_philox_fill(uint64_t seed, uint64_t path, uint64_t first_draw, int64_t n, double *out);
+
 /* ---- context: one GPU, one HIP stream, one RNG seed ---------------------------- */
 /* Replaces MPI_Init/init_random_seed (main.f90:7-16, RT:3314-3330). */
 int pt_ctx_create(int device, uint64_t seed, pt_ctx **out);

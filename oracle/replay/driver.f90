@@ -8,6 +8,8 @@
 ! Usage (run in a directory holding Materials/<name>.anphon, RT:671):
 !   replay grid  <file.mcp> <mfp_nm> <seed>
 !       TransmissionFunction (RT:1544) once; prints "C ib ik ia count" per cell.
+!   In every mode, the draws of loadMCP (FluctuationSpecularity, RT:1151-1157) come from
+!   the Philox stream (seed, 2^62), the host's set-up stream.
 !   replay paths <file.mcp> <mfp_nm> <seed> <first_path> <n>
 !       calcTransmit (RT:1608) one particle at a time for paths first..first+n-1 of the
 !       numbering path = cell*Np + p, cell = ((ik-1)*nb + (ib-1))*(nangle-1) + (ia-1);
@@ -19,6 +21,11 @@
 !   replay geom  <cases.txt>
 !       runs setupSurfaceGeometry, isInsidePolygon, calcSurfaceDistance*,
 !       calcSurfaceNormal* and calcSurfaceRotation* on the cases in the file.
+!   replay laws  <file.mcp> <seed>
+!       loadMCP with the set-up draws (FluctuationSpecularity, RT:1151-1157) served from
+!       the Philox stream (seed, 2^62); prints "T i2 i1 kind value" for every material
+!       pair (GammaDeterm's gamma for TBC rows, RT:858-870) and "S ig is p" for every
+!       surface of every grain.
 program replay_driver
   use RTMC
   use iso_c_binding
@@ -60,6 +67,8 @@ program replay_driver
     call run_geom()
   case ('post')
     call run_post()
+  case ('laws')
+    call run_laws()
   case default
     write(*,*) 'usage: replay grid|paths|geom ...'
     stop 2
@@ -82,9 +91,10 @@ contains
     integer :: ib, ik, ia
     real(8) :: mfp
     integer(c_int64_t) :: seed
-    call load(status)
     call getarg(3, arg); read(arg,*) mfp
     call getarg(4, arg); read(arg,*) seed
+    call replay_configure(seed, 4611686018427387904_c_int64_t)   ! set-up draws
+    call load(status)
     param%MeanFreePath_current = mfp
     interval_trace = huge(1)
     open(fd_trace, file='trace.out')
@@ -104,9 +114,10 @@ contains
     integer :: ib, ik, ia, np, na, nb
     integer(c_int64_t) :: seed, first, n, path, cell, draws
     real(8) :: mfp, trans, dummy
-    call load(status)
     call getarg(3, arg); read(arg,*) mfp
     call getarg(4, arg); read(arg,*) seed
+    call replay_configure(seed, 4611686018427387904_c_int64_t)   ! set-up draws
+    call load(status)
     call getarg(5, arg); read(arg,*) first
     call getarg(6, arg); read(arg,*) n
     param%MeanFreePath_current = mfp
@@ -140,6 +151,25 @@ contains
     call replay_log_close()
     close(fd_trace)
   end subroutine run_paths
+
+  subroutine run_laws()
+    integer(c_int64_t) :: seed
+    integer :: i1, i2, ig, is
+    call getarg(3, arg); read(arg,*) seed
+    call replay_configure(seed, 4611686018427387904_c_int64_t)
+    call load(status)
+    do i1 = 1, size(param%kindTrans, 2)
+      do i2 = 1, size(param%kindTrans, 1)
+        write(*,'(a,2i6,i4,es27.17e3)') 'T ', i2, i1, param%kindTrans(i2,i1), &
+          param%transmissivity(i2,i1)
+      end do
+    end do
+    do ig = lbound(vgrain, 1), ubound(vgrain, 1)
+      do is = 1, vgrain(ig)%nsurface
+        write(*,'(a,2i8,es27.17e3)') 'S ', ig, is, vgrain(ig)%vsurface(is)%specularity
+      end do
+    end do
+  end subroutine run_laws
 
   subroutine run_post()
     character(len=256) :: tfile, mcp

@@ -12,6 +12,7 @@
 
 #include "../../include/ptrans.h"
 #include "pt_internal.h"
+#include "pt_philox.h"
 #include "pt_math.h"
 
 using namespace pt;
@@ -91,6 +92,13 @@ struct pt_modes {
 extern "C" {
 
 int32_t pt_version(void) { return PT_ABI_VERSION; }
+
+int pt_philox_fill(uint64_t seed, uint64_t path, uint64_t first_draw, int64_t n, double *out)
+{
+    if (n < 0 || (n > 0 && !out)) return fail(PT_E_INVALID, "pt_philox_fill: bad arguments");
+    for (int64_t i = 0; i < n; ++i) out[i] = pt_philox_draw(seed, path, first_draw + (uint64_t)i);
+    return PT_OK;
+}
 
 const char *pt_build_info(void)
 {

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libptrans.so")
 
 # every entry point declared in include/ptrans.h
 EXPORTS = (
-    "pt_version", "pt_build_info", "pt_last_error",
+    "pt_version", "pt_build_info", "pt_last_error", "pt_philox_fill",
     "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_set_seed", "pt_ctx_set_event_limit",
     "pt_ctx_set_op_counting", "pt_ctx_set_tuning",
     "pt_geom_create", "pt_geom_destroy", "pt_geom_set_specularity", "pt_surface_setup_polygon",
@@ -93,6 +93,7 @@ def load():
     L.pt_batch_set_specularity.argtypes = [vp, vp]
     L.pt_batch_trace.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     L.pt_batch_trace_graph.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
+    L.pt_philox_fill.argtypes = [u64, u64, u64, i64, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

@@ -252,9 +252,11 @@ def box_grain(box, cellspec) -> Grain:
     return g
 
 
-def _material_tables(m: McpModel):
+def _material_tables(m: McpModel, tables=None):
     """param%kindTrans / transmissivity / specularity (RT:770-875), indexed
-    [imaterial2][imaterial1] (1-based); row Nt2+1 is the boundary row (t = p = 0)."""
+    [imaterial2][imaterial1] (1-based); row Nt2+1 is the boundary row (t = p = 0).  A TBC
+    row (MW/m^2/K) becomes GammaDeterm's gamma (RT:858-870) from the bulk table of its
+    material1, tables[material1]."""
     nt1, nt2 = m.nt
     kind = {}
     trans = {}
@@ -270,7 +272,13 @@ def _material_tables(m: McpModel):
         kind[(i2, i1)] = k
         spec[(i2, i1)] = float(r.specularity)
         if k == TRANS_TBC:
-            raise NotImplementedError("TBC transmission rows need GammaDeterm (RT:2499-2599)")
+            from .laws import gamma_determ
+            if not tables or r.material1 not in tables:
+                raise ValueError("the TBC row %s-%s needs the bulk table of %s (tables=...)"
+                                 % (r.material1, r.material2, r.material1))
+            trans[(i2, i1)] = gamma_determ(tables[r.material1], float(r.value) * 1.0e6,
+                                           float(m.temperature))
+            continue
         trans[(i2, i1)] = float(r.value)
     return kind, trans, spec
 
@@ -281,15 +289,18 @@ def _set_trans(s: Surface, kind: int, value: float):
     s.trans_param = value
 
 
-def build_geometry(m: McpModel) -> Geometry:
-    """loadGeometry (RT:894-1251) on a parsed .mcp."""
-    if m.specfluc > 0.0:
-        raise NotImplementedError("FluctuationSpecularity > 0 draws per-surface jitter (RT:1151-1157)")
+def build_geometry(m: McpModel, tables=None, setup_seed=None) -> Geometry:
+    """loadGeometry (RT:894-1251) on a parsed .mcp.  tables: {material: anphon.Anphon},
+    needed for TBC rows.  setup_seed: Philox seed of the FluctuationSpecularity jitter,
+    needed when the .mcp sets it above 0."""
+    if m.specfluc > 0.0 and setup_seed is None:
+        raise ValueError("FluctuationSpecularity > 0 draws per-surface jitter: pass setup_seed")
+    ndraw = 0
     box = m.box
     kind_grain = {"voronoi": GRAIN_VORONOI, "bricks": GRAIN_BRICKS, "mesh": GRAIN_MESH}[m.kind_grain]
     ngrain = len(m.grains)
     igrain_fill = ngrain + 1 if kind_grain == GRAIN_BRICKS else -1
-    kind_t, trans_t, spec_t = _material_tables(m)
+    kind_t, trans_t, spec_t = _material_tables(m, tables)
     grains = [box_grain(box, m.cellspec)]
     for ig, gs in enumerate(m.grains, start=1):
         g = Grain(incident=False, generatrix=tuple(float(x) for x in gs.generatrix))
@@ -327,6 +338,11 @@ def build_geometry(m: McpModel) -> Geometry:
             else:
                 s.imaterial2 = m.materials.index(f.material_to) + 1
             s.specularity = spec_t[(s.imaterial2, s.imaterial1)]
+            if m.specfluc > 0.0:                                   # RT:1151-1157
+                from .laws import fluctuate_specularity, setup_uniforms
+                u = setup_uniforms(int(setup_seed), ndraw, 12)
+                ndraw += 12
+                s.specularity = fluctuate_specularity(s.specularity, float(m.specfluc), u)
             _set_trans(s, kind_t[(s.imaterial2, s.imaterial1)], trans_t[(s.imaterial2, s.imaterial1)])
         grains.append(g)
     if kind_grain == GRAIN_BRICKS:

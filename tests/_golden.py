@@ -29,7 +29,10 @@ class Case:
             self.meta = json.load(f)
         self.model = mcp.read_mcp(os.path.join(d, "input.mcp"))
         self.table = anphon.read_anphon(os.path.join(d, "Si.anphon"), "Si")
-        self.geom = geometry.build_geometry(self.model)
+        # TBC rows need the bulk table; FluctuationSpecularity draws from the set-up stream
+        # of the seed the reference ran with (paths and grid share it where it matters)
+        self.geom = geometry.build_geometry(self.model, tables={"Si": self.table},
+                                            setup_seed=self.meta["paths_seed"])
         self.gdesc, self._garr = self.geom.desc()
         t = self.table
         self.mdesc, self._marr = geometry.modes_desc(t.omega, t.vel, t.mfp, t.weight)

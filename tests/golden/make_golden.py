@@ -18,6 +18,7 @@ Only runs where /root/reference and oracle/_ref exist (the development container
 """
 from __future__ import annotations
 
+import dataclasses
 import json
 import os
 import shutil
@@ -58,6 +59,13 @@ CASES = {
     "c5_tri_brick": (lambda: gen.tri_brick(nsub=3, jitter=0.0, kind="Value", value=0.3, p=0.5,
                                            nphonon=20, nangle=5, mfp=(600.0,)),
                      (2, 3, 18), 600.0, 1008, 240, 2008),
+    # TBC row (GammaDeterm) and FluctuationSpecularity: the set-up draws use the case's
+    # seed, so paths and grid share it
+    "c4_voronoi_tbc_fluct": (lambda: dataclasses.replace(
+                                 gen.voronoi_cube(seed=4, kind="TBC", value=300.0, p=0.4,
+                                                  nphonon=20, nangle=5, mfp=(400.0,)),
+                                 specfluc=0.1),
+                             (2, 3, 19), 400.0, 1009, 240, 1009),
 }
 
 
@@ -252,9 +260,50 @@ def run_post(name, spec):
     return semfp
 
 
+# name -> (model factory, (nbranch, nkpoint, table seed), set-up seeds)
+LAW_CASES = {
+    "voronoi_tbc300_fluct": (lambda: dataclasses.replace(
+        gen.voronoi_cube(seed=4, kind="TBC", value=300.0, p=0.4, nphonon=10, nangle=5),
+        specfluc=0.1), (3, 50, 5), (7, 8)),
+    "inclusions_tbc50_fluct": (lambda: dataclasses.replace(
+        gen.inclusions(seed=3, kind="TBC", value=50.0, p=0.5, nphonon=10, nangle=5),
+        specfluc=0.25), (2, 40, 6), (11,)),
+    "voronoi_tbc1": (lambda: gen.voronoi_cube(seed=5, kind="TBC", value=1.0, p=0.2, nphonon=10,
+                                              nangle=5), (3, 60, 7), (1,)),
+    "voronoi_tbc1e4": (lambda: gen.voronoi_cube(seed=5, kind="TBC", value=1.0e4, p=0.2,
+                                                nphonon=10, nangle=5), (3, 60, 7), (1,)),
+}
+
+
+def run_laws(name, spec):
+    """loadMCP of the reference on .mcp inputs with TBC rows and FluctuationSpecularity:
+    the gamma of each material pair and every surface's specularity."""
+    factory, (nb, nk, tseed), seeds = spec
+    model = factory()
+    outdir = os.path.join(HERE, "laws", name)
+    os.makedirs(outdir, exist_ok=True)
+    mcp.write_mcp(os.path.join(outdir, "input.mcp"), model)
+    anphon.write_anphon(os.path.join(outdir, "Si.anphon"), anphon.synthetic_table(nb, nk, tseed))
+    for seed in seeds:
+        with tempfile.TemporaryDirectory() as tmp:
+            os.makedirs(os.path.join(tmp, "Materials"))
+            shutil.copy(os.path.join(outdir, "input.mcp"), os.path.join(tmp, "input.mcp"))
+            shutil.copy(os.path.join(outdir, "Si.anphon"), os.path.join(tmp, "Materials", "Si.anphon"))
+            out = subprocess.run([os.path.join(REF_DIR, "replay_libm"), "laws", "input.mcp", str(seed)],
+                                 cwd=tmp, capture_output=True, text=True, timeout=120, check=True)
+        lines = [ln for ln in out.stdout.splitlines() if ln.startswith(("T ", "S "))]
+        with open(os.path.join(outdir, "ref_seed%d.txt" % seed), "w") as f:
+            f.write("\n".join(lines) + "\n")
+    return len(lines)
+
+
 def main():
     if not os.path.exists(os.path.join(REF_DIR, "replay_own")):
         sys.exit("oracle/_ref is not built: make -C oracle ref (needs /root/reference)")
+    for name, spec in LAW_CASES.items():
+        print("%-22s %d lines" % (name, run_laws(name, spec)))
+    if len(sys.argv) > 1 and sys.argv[1] == "laws":
+        return
     for name, spec in POST_CASES.items():
         sem = run_post(name, spec)
         print("%-22s SEMFP mean %.6g" % (name, sem.mean()))
