@@ -279,6 +279,22 @@ int pt_replay(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
               int64_t first_path, int64_t n, int32_t precision, pt_path_record *records);
 
 /*
+ * Trajectory recorder (trace.out, RT:1661-1665 and 1818-1831): traces paths first_path +
+ * i * stride (i < n) of the pt_trace_grid numbering, one at a time as pt_replay, and
+ * stores the positions the reference writes to trace.out for a traced particle: the
+ * start position, then the position at the end of every event-loop iteration.  The
+ * first max_points of path i go to points[(i * max_points + k) * 3 + 0..2] (FP64, nm),
+ * their total number (also beyond max_points) to npoints[i]; records may be NULL.  The
+ * reference traces every interval_trace-th particle of its OpenMP master thread; stride
+ * plays that role here on the deterministic path numbering.  Host buffers of
+ * n * max_points * 24 B (at most 4 GiB).
+ */
+int pt_trace_log(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, double mfp_nm,
+                 int32_t nangle, const double *vangle, int64_t np, int64_t first_path,
+                 int64_t stride, int64_t n, int32_t precision, int64_t max_points,
+                 double *points, int64_t *npoints, pt_path_record *records);
+
+/*
  * Alias-mode trace (an extension of the reference's estimator, whose (k, b, angle)
  * enumeration is pt_trace_grid): npaths paths, path i with Philox stream i; each path
  * draws its mode from the alias table, enters through x = 0 with Lambertian polar

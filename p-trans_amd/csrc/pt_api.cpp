@@ -656,17 +656,32 @@ void fill_stats(pt_stats *stats, const unsigned long long *h, float ms, float ms
 }
 
 int replay_common(pt_ctx *c, TraceParams &P, int mode, int64_t first, int64_t n,
-                  pt_path_record *records)
+                  pt_path_record *records, int64_t stride = 1, double *points = nullptr,
+                  int64_t max_points = 0, int64_t *npoints = nullptr)
 {
     pt_path_record *d_rec = nullptr;
+    double *d_pts = nullptr;
+    int64_t *d_npts = nullptr;
     PT_HIP(hipMalloc((void **)&d_rec, n * sizeof(pt_path_record)));
-    hipError_t e;
-    do {
-        if ((e = launch_replay(P, mode, first, n, d_rec, c->stream))) break;
-        if ((e = hipMemcpyAsync(records, d_rec, n * sizeof(pt_path_record), hipMemcpyDeviceToHost, c->stream))) break;
+    hipError_t e = hipSuccess;
+    if (points) {
+        if ((e = hipMalloc((void **)&d_pts, (size_t)n * max_points * 3 * sizeof(double))) == hipSuccess)
+            e = hipMalloc((void **)&d_npts, n * sizeof(int64_t));
+    }
+    while (e == hipSuccess) {
+        if ((e = launch_replay(P, mode, first, n, stride, d_rec, d_pts, max_points, d_npts, c->stream))) break;
+        if (records && (e = hipMemcpyAsync(records, d_rec, n * sizeof(pt_path_record), hipMemcpyDeviceToHost, c->stream))) break;
+        if (points) {
+            if ((e = hipMemcpyAsync(points, d_pts, (size_t)n * max_points * 3 * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream))) break;
+            if ((e = hipMemcpyAsync(npoints, d_npts, n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream))) break;
+        }
         e = hipStreamSynchronize(c->stream);
-    } while (0);
+        break;
+    }
     (void)hipFree(d_rec);
+    if (d_pts) (void)hipFree(d_pts);
+    if (d_npts) (void)hipFree(d_npts);
     if (e != hipSuccess) return fail(PT_E_HIP, "replay: %s", hipGetErrorString(e));
     return PT_OK;
 }
@@ -857,21 +872,22 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     return PT_OK;
 }
 
-int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
-              const double *vangle, int64_t np, int64_t first_path, int64_t n,
-              int32_t precision, pt_path_record *records)
+static int grid_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
+                       const double *vangle, int64_t np, int64_t first_path, int64_t stride,
+                       int64_t n, int32_t precision, pt_path_record *records, double *points,
+                       int64_t max_points, int64_t *npoints, const char *fn)
 {
     int rc = check_trace_args(c, G, M, mfp, nangle, np);
     if (rc) return rc;
-    if (np < 1) return fail(PT_E_INVALID, "np must be >= 1");
+    if (np < 1) return fail(PT_E_INVALID, "%s: np must be >= 1", fn);
     if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
         precision != PT_PREC_FP32_FILTER)
-        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
-    if (n < 0 || first_path < 0) return fail(PT_E_INVALID, "bad path range");
+        return fail(PT_E_UNSUPPORTED, "%s: precision %d", fn, precision);
+    if (n < 0 || first_path < 0 || stride < 1) return fail(PT_E_INVALID, "%s: bad path range", fn);
     if (n == 0) return PT_OK;
-    if (!records) return fail(PT_E_INVALID, "records is NULL");
     const int64_t total = (int64_t)M->nb * M->nk * (nangle - 1) * np;
-    if (first_path + n > total) return fail(PT_E_INVALID, "path range beyond the last cell");
+    if (first_path + (n - 1) * stride >= total)
+        return fail(PT_E_INVALID, "%s: path range beyond the last cell", fn);
     PT_HIP(hipSetDevice(c->device));
     std::vector<double> ct, st;
     angle_tables(nangle, vangle, ct, st);
@@ -888,9 +904,31 @@ int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_
     P.mfp = mfp;
     P.np = np;
     P.nangle = nangle;
-    rc = replay_common(c, P, TRACE_GRID, first_path, n, records);
+    rc = replay_common(c, P, TRACE_GRID, first_path, n, records, stride, points, max_points, npoints);
     (void)hipFree(d_ang);
     return rc;
+}
+
+int pt_replay(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
+              const double *vangle, int64_t np, int64_t first_path, int64_t n,
+              int32_t precision, pt_path_record *records)
+{
+    if (n > 0 && !records) return fail(PT_E_INVALID, "pt_replay: records is NULL");
+    return grid_replay(c, G, M, mfp, nangle, vangle, np, first_path, 1, n, precision, records,
+                       nullptr, 0, nullptr, "pt_replay");
+}
+
+int pt_trace_log(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, int32_t nangle,
+                 const double *vangle, int64_t np, int64_t first_path, int64_t stride, int64_t n,
+                 int32_t precision, int64_t max_points, double *points, int64_t *npoints,
+                 pt_path_record *records)
+{
+    if (n > 0 && (!points || !npoints)) return fail(PT_E_INVALID, "pt_trace_log: points or npoints is NULL");
+    if (max_points < 1) return fail(PT_E_INVALID, "pt_trace_log: max_points must be >= 1");
+    if (n > 0 && (double)n * (double)max_points * 24.0 > 4.0 * 1073741824.0)
+        return fail(PT_E_INVALID, "pt_trace_log: n * max_points * 24 B exceeds 4 GiB");
+    return grid_replay(c, G, M, mfp, nangle, vangle, np, first_path, stride, n, precision, records,
+                       points, max_points, npoints, "pt_trace_log");
 }
 
 int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npaths,

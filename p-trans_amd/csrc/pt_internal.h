@@ -159,8 +159,9 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
                               int64_t nb, int64_t nk, int32_t nangle,
                               long long *counts, unsigned long long *skipped, hipStream_t s);
-hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n,
-                         pt_path_record *rec, hipStream_t s);
+hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n, int64_t stride,
+                         pt_path_record *rec, double *pts, int64_t max_pts, int64_t *npts,
+                         hipStream_t s);
 // workgroups per CU the trace kernel of (mode, precision) can hold with `lds` bytes of
 // dynamic LDS
 hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int *blocks_per_cu);

@@ -1670,10 +1670,17 @@ __global__ void __launch_bounds__(256) reduce_grid_kernel(const uint8_t *__restr
     }
 }
 
-// Single paths with full records (parity checks), one lane per path.
+// Single paths with full records (parity checks, trajectory logs), one lane per path:
+// path first + i * stride for i < n.  With pts, the lane also stores the positions the
+// reference writes to trace.out: the start position (RT:1663, before the start-grain
+// check) and the position at the end of every event-loop iteration (RT:1818-1822; a path
+// stopped at the top of an iteration, as lost or at the event limit, writes nothing
+// more); the first max_pts of them, at pts[(i * max_pts + k) * 3], and their number
+// (all of them) at npts[i].
 template <int kMode, int kPrec>
 __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first, int64_t n,
-                                                    pt_path_record *rec)
+                                                    int64_t stride, pt_path_record *rec,
+                                                    double *pts, int64_t max_pts, int64_t *npts)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1684,11 +1691,28 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
     Path ph;
     AdjLane al = {nullptr, 0, nullptr, 0, 0.0, nullptr, -1, 0.0, 0.0, 0.0};
     int mode = 0;
-    if (begin_path<kMode, kPrec>(P, first + i, rng, ph, &mode)) {
+    int64_t cnt = 0;
+    const bool started = begin_path<kMode, kPrec>(P, first + i * stride, rng, ph, &mode);
+    if (pts) {
+        double *q = pts + (i * max_pts) * 3;
+        q[0] = ph.pos[0]; q[1] = ph.pos[1]; q[2] = ph.pos[2];
+    }
+    cnt = 1;
+    if (started) {
         while (ph.end_side == 0) {
-            if (step_path<kPrec, false>(P.g, P.m.max_omega, P.event_limit, rng, ph, al)) break;
+            const int64_t ev = ph.events;
+            const bool done = step_path<kPrec, false>(P.g, P.m.max_omega, P.event_limit, rng, ph, al);
+            if (ph.events != ev) {
+                if (pts && cnt < max_pts) {
+                    double *q = pts + (i * max_pts + cnt) * 3;
+                    q[0] = ph.pos[0]; q[1] = ph.pos[1]; q[2] = ph.pos[2];
+                }
+                ++cnt;
+            }
+            if (done) break;
         }
     }
+    if (npts) npts[i] = cnt;
     pt_path_record r;
     r.end_side = ph.end_side;
     r.flags = ph.flags;
@@ -1776,12 +1800,14 @@ hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
     return hipGetLastError();
 }
 
-hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n,
-                         pt_path_record *rec, hipStream_t s)
+hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t n, int64_t stride,
+                         pt_path_record *rec, double *pts, int64_t max_pts, int64_t *npts,
+                         hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     const dim3 gr((unsigned)((n + 63) / 64)), bl(64);
-#define PT_REPLAY(M, R) hipLaunchKernelGGL((replay_kernel<M, R>), gr, bl, 0, s, p, first, n, rec)
+#define PT_REPLAY(M, R) hipLaunchKernelGGL((replay_kernel<M, R>), gr, bl, 0, s, p, first, n, stride, \
+                                           rec, pts, max_pts, npts)
 #define PT_REPLAY_R(M)                                                   \
     do {                                                                 \
         if (p.precision == kPrecFast) PT_REPLAY(M, kPrecFast);           \

@@ -20,7 +20,8 @@ EXPORTS = (
     "pt_ctx_set_op_counting", "pt_ctx_set_tuning",
     "pt_geom_create", "pt_geom_destroy", "pt_geom_set_specularity", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
-    "pt_trace_grid", "pt_replay", "pt_alias_build", "pt_trace_alias", "pt_replay_alias",
+    "pt_trace_grid", "pt_replay", "pt_trace_log", "pt_alias_build", "pt_trace_alias",
+    "pt_replay_alias",
     "pt_adjoint_alias", "pt_trace_alias_grains",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_info", "pt_batch_set_specularity",
     "pt_batch_trace", "pt_batch_trace_graph",
@@ -94,6 +95,7 @@ def load():
     L.pt_batch_trace.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     L.pt_batch_trace_graph.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     L.pt_philox_fill.argtypes = [u64, u64, u64, i64, vp]
+    L.pt_trace_log.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i64, i64, i64, i32, i64, vp, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is ctypes.c_int:   # default

@@ -27,8 +27,8 @@ import math
 
 import numpy as np
 
-HBAR = 1.054571e-34          # RT:67
-KB = 1.3806503e-23           # RT:68
+from .constants import HBAR, KB   # RT:67-68
+
 SETUP_STREAM = 1 << 62       # PT_SETUP_STREAM in ptrans.h
 
 

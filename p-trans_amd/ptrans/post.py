@@ -323,3 +323,81 @@ def grain_kappa_shares(grain_tally, npaths: int):
     (3/4) sum_g tally[g, 0] / npaths; share_g = (3/4) tally[g, 0] / npaths."""
     t = np.asarray(grain_tally, dtype=np.float64)
     return 0.75 * t[:, 0] / float(npaths)
+
+
+# ---------------------------------------------------------------------------- trace.out
+
+def _flang_shortest(f: np.float32):
+    """Shortest decimal digits of a positive REAL(4) the way the flang runtime picks them:
+    with lo, hi the midpoints to the neighbouring REAL(4) values, the common leading digits
+    of lo and hi, then at the first digit where they differ (dl < dh) the digit
+    (dl + dh + 1) // 2.  Returns (digit string, decimal exponent e of d.ddd x 10^e)."""
+    from decimal import Decimal, ROUND_FLOOR, localcontext
+    with localcontext() as ctx:
+        ctx.prec = 80
+        x = Decimal(float(f))
+        lo = (x + Decimal(float(np.nextafter(f, np.float32(0))))) / 2
+        hi = (x + Decimal(float(np.nextafter(f, np.float32(np.inf))))) / 2
+        top = hi.adjusted()
+        for k in range(1, 40):
+            q = Decimal(10) ** (top - k + 1)
+            a = int((lo / q).to_integral_value(rounding=ROUND_FLOOR))
+            b = int((hi / q).to_integral_value(rounding=ROUND_FLOOR))
+            if a != b:
+                n = b - b % 10 + (a % 10 + b % 10 + 1) // 2
+                digits = str(n).rstrip("0") or "0"
+                return digits, top - k + len(str(n))
+    raise ValueError("no shortest form for %r" % f)
+
+
+def fortran_list_real4(x: float) -> str:
+    """One REAL(4) value as the flang runtime's list-directed output writes it
+    (write(fd,*) real(Position), RT:1663, 1820): the shortest digits that read back to the
+    same REAL(4) (chosen as _flang_shortest does); F editing when the value rounded to one
+    digit is 0.d x 10^e with 0 <= e <= 6, else E editing with a two-digit exponent
+    ("54.351204", ".5", "100.", "0.", "6.893895E-02", "1.E+07")."""
+    f = np.float32(x)
+    if f == 0:
+        return "-0." if np.signbit(f) else "0."
+    sign = "-" if f < 0 else ""
+    digits, e = _flang_shortest(abs(f))            # value = d.ddd x 10^e
+    e1 = int(("%.0e" % abs(float(f))).split("e")[1]) + 1    # one-digit rounding, 0.d x 10^e1
+    if 0 <= e1 <= 6:
+        k = e + 1                                  # digits before the point
+        if k <= 0:
+            s = "." + "0" * (-k) + digits
+        elif k >= len(digits):
+            s = digits + "0" * (k - len(digits)) + "."
+        else:
+            s = digits[:k] + "." + digits[k:]
+    else:
+        s = digits[0] + "." + digits[1:] + "E" + ("-" if e < 0 else "+") + "%02d" % abs(e)
+    return sign + s
+
+
+def trace_out_lines(trajectories, particle_trace: Optional[int] = None,
+                    interval_trace: Optional[int] = None, fp64: bool = False):
+    """Lines of trace.out (RT:3377-3382 header, then per traced particle RT:1661-1665,
+    1818-1831): the start position and the position after every event, one per line, and
+    two empty lines after each particle.  fp64=True writes full FP64 positions ("%.17g")
+    instead of the reference's REAL(4) list-directed values.  The header is written when
+    particle_trace and interval_trace are given."""
+    out = []
+    if particle_trace is not None and interval_trace is not None:
+        out += ["# Datatype = trace", "# particle_trace = %8d" % particle_trace,
+                "# interval_trace = %8d" % interval_trace, ""]
+    for t in trajectories:
+        for p in np.asarray(t, dtype=np.float64).reshape(-1, 3):
+            if fp64:
+                out.append(" " + " ".join("%.17g" % v for v in p))
+            else:
+                out.append(" " + " ".join(fortran_list_real4(v) for v in p))
+        out += ["", ""]
+    return out
+
+
+def write_trace(path: str, trajectories, particle_trace: Optional[int] = None,
+                interval_trace: Optional[int] = None, fp64: bool = False):
+    with open(path, "w") as f:
+        for ln in trace_out_lines(trajectories, particle_trace, interval_trace, fp64):
+            f.write(ln + "\n")

@@ -152,6 +152,23 @@ class Context:
             int(precision), out.ctypes.data))
         return out
 
+    def trace_log(self, geom: Geom, modes: Modes, mfp_nm: float, nangle: int, np_: int,
+                  first: int, n: int, stride: int = 1, max_points: int = 4096, vangle=None,
+                  precision: int = PREC_FP64):
+        """pt_trace_log: the trace.out positions of paths first + i * stride (i < n): a
+        list of (npoints_i, 3) FP64 arrays (cut at max_points) and the per-path records."""
+        pts = np.zeros((n, max_points, 3), dtype=np.float64)
+        npts = np.zeros(n, dtype=np.int64)
+        rec = np.zeros(n, dtype=_lib.RECORD_DTYPE)
+        va = None
+        if vangle is not None:
+            va = np.ascontiguousarray(vangle, dtype=np.float64)
+        _lib.check(_lib.load().pt_trace_log(
+            self.handle, geom.handle, modes.handle, float(mfp_nm), int(nangle),
+            va.ctypes.data if va is not None else None, int(np_), int(first), int(stride), int(n),
+            int(precision), int(max_points), pts.ctypes.data, npts.ctypes.data, rec.ctypes.data))
+        return [pts[i, :min(int(npts[i]), max_points)].copy() for i in range(n)], rec
+
     def trace_alias(self, geom: Geom, modes: Modes, npaths: int, branch_scale=None,
                     precision: int = PREC_FP64):
         """pt_trace_alias: per-mode (nb, nk) int64 path and transmitted counts, stats."""

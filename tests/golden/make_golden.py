@@ -100,6 +100,9 @@ def _parse_sites(path):
     return per
 
 
+TRACE_TEXT_CASES = ("c1_wire_gray", "c4_voronoi_gamma")
+
+
 def run_case(name, spec, exe_tag):
     factory, (nb, nk, tseed), mfp, pseed, npaths, gseed = spec
     model = factory()
@@ -120,6 +123,13 @@ def run_case(name, spec, exe_tag):
         trans = np.array([int(r[2]) for r in recs], dtype=np.int8)
         draws = np.array([int(r[3]) for r in recs], dtype=np.int64)
         blocks = _parse_trace(os.path.join(tmp, "trace.out"))
+        if exe_tag == "own" and name in TRACE_TEXT_CASES:
+            # the reference's own trace.out text of the first 40 paths (writer fixture)
+            with open(os.path.join(tmp, "trace.out")) as f:
+                text = f.read()
+            parts = text.split("\n\n\n")
+            with open(os.path.join(outdir, "trace_ref.out"), "w") as f:
+                f.write("\n\n\n".join(parts[:40]) + "\n\n\n")
         sites = _parse_sites(os.path.join(tmp, "draws.log"))
         if len(blocks) != len(path) or len(sites) != len(path):
             raise RuntimeError("%s: %d paths, %d trace blocks, %d site lists"
