@@ -172,8 +172,10 @@ def test_fp32_trace_statistically_equals_fp64(gpu_ctx, builder):
     gpu_ctx.set_seed(37)
     g = gpu_ctx.geometry(geo)
     m = gpu_ctx.modes(omega=om)
-    a = gpu_ctx.replay(g, m, mfp, 11, 20000, 0, 20000)
-    b = gpu_ctx.replay(g, m, mfp, 11, 20000, 0, 20000, precision=FP32)
+    # paired sample: every 10th path of the whole grid (all 60 cells), both precisions
+    _, a = gpu_ctx.trace_log(g, m, mfp, 11, 20000, 0, 120000, stride=10, max_points=1)
+    _, b = gpu_ctx.trace_log(g, m, mfp, 11, 20000, 0, 120000, stride=10, max_points=1,
+                             precision=FP32)
     ta = (a["end_side"] != 1)
     tb = (b["end_side"] != 1)
     agree = (a["end_side"] == b["end_side"]).mean()
