@@ -113,7 +113,48 @@ def cpu_baseline(geo, omega, mfp, nangle, np_per_cell, seed, seconds, threads):
     return state["paths"] / dt, state["paths"], min(state["next"], ncell), dt
 
 
-def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
+def cpu_rate(work, units, seconds: float, threads: int):
+    """The oracle on `threads` host threads (ctypes releases the GIL): each thread takes
+    the next unit (a callable returning its path count) until `seconds` have passed.
+    Returns (paths/s, paths, units done, seconds)."""
+    lock = threading.Lock()
+    state = {"next": 0, "paths": 0}
+    t_end = time.perf_counter() + seconds
+
+    def worker():
+        while time.perf_counter() < t_end:
+            with lock:
+                i = state["next"]
+                state["next"] += 1
+            if i >= len(units):
+                return
+            n = work(units[i])
+            with lock:
+                state["paths"] += n
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    return state["paths"] / dt, state["paths"], min(state["next"], len(units)), dt
+
+
+def alias_weights(omega, vel, mfp, weight, temperature=300.0):
+    """C v Lambda w per mode (column-major), the alias weights of the library's sampler."""
+    cap = np.vectorize(lambda o: post.heat_capacity(float(o), temperature))(omega)
+    return (cap * vel * mfp * weight).reshape(-1, order="F")
+
+
+def cpu_leg_baseline(work, units, seconds, threads, what):
+    rate, npaths, nunits, dt = cpu_rate(work, units, seconds, threads)
+    return {"value": rate, "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": "%d paths (%s) in %.1f s: the oracle's C restatement, %d host threads"
+                      % (npaths, what, dt, threads)}
+
+
+def adjoint_leg(ctx, npaths: int, table_seed: int = 20, cpu=None):
     """C3 inclusions: forward alias trace (FP32 trace and FP64) and the LR adjoint."""
     from ptrans.constants import PREC_FP32_RETEST as FAST
     model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.5, nphonon=1, nangle=3)
@@ -150,7 +191,7 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
     snorm = float(np.linalg.norm(gr.grad_stderr))
     n2s = npaths * (snorm / (0.5 * gnorm)) ** 2 if gnorm > 0 else None
     adj_rate = npaths / t_adj
-    return {
+    out = {
         "workload": "1000 nm cube, 8 spheres + 4 z-cylinders (Gamma law, per-inclusion "
                     "specularity U(0.05, 0.95)), 6x2000 modes, alias mode, FP32 trace",
         "paths": npaths,
@@ -171,10 +212,25 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20):
         "paths_for_1e-3_rel_stderr": npaths * (k32.stderr / (1e-3 * k32.ratio)) ** 2,
         "seconds_for_1e-3_rel_stderr": npaths * (k32.stderr / (1e-3 * k32.ratio)) ** 2 * t_fwd / npaths,
         "events_per_path": ast["events"] / max(1, ast["paths"]),
+        "cpu_baseline": None,
     }
+    if cpu:
+        from oracle import oracle as O
+        gdesc, _ga = geo.desc()
+        mdesc, _ma = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
+        prob, alias = O.build_alias(alias_weights(omega, t.vel, t.mfp_nm, t.weight))
+        gsurf = np.ascontiguousarray(group, dtype=np.int32)
+
+        def work(first):
+            O.adjoint_alias(gdesc, mdesc, t.mfp_nm, prob, alias, 777, first, 2000, gsurf, ngroup,
+                            base, nbatch=1)
+            return 2000
+        out["cpu_baseline"] = cpu_leg_baseline(work, list(range(0, npaths, 2000)), *cpu,
+                                               "adjoint paths of the same workload, 2000 per task")
+    return out
 
 
-def grains_leg(ctx, npaths: int, table_seed: int = 20):
+def grains_leg(ctx, npaths: int, table_seed: int = 20, cpu=None):
     """C4 polycrystal with Lorentzian boundaries and per-grain flux tallies (see the
     module docstring)."""
     model = gen.voronoi_cube(seed=4, L=1000.0, ngrain=5, kind="Gamma", value=2.0, p=0.2,
@@ -195,7 +251,21 @@ def grains_leg(ctx, npaths: int, table_seed: int = 20):
     g.close()
     a, b = out["fp32"], out["fp64"]
     sig = float(np.hypot(a["k"].stderr, b["k"].stderr))
+    cpu_b = None
+    if cpu:
+        from oracle import oracle as O
+        gdesc, _ga = geo.desc()
+        mdesc, _ma = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
+        prob, alias = O.build_alias(alias_weights(omega, t.vel, t.mfp_nm, t.weight))
+        ngt = len(geo.grains)
+
+        def work(first):
+            O.trace_alias_grains(gdesc, mdesc, t.mfp_nm, prob, alias, 2024, first, 5000, ngt)
+            return 5000
+        cpu_b = cpu_leg_baseline(work, list(range(0, npaths, 5000)), *cpu,
+                                 "alias paths with grain tallies, 5000 per task")
     return {
+        "cpu_baseline": cpu_b,
         "workload": "1000 nm cube, 5 Voronoi grains, Lorentzian grain-boundary transmission "
                     "(f0 ~ U(3, 10) THz, %d boundaries), p = 0.2, 6x2000 modes, alias mode, "
                     "per-grain heat-flux tallies, FP32 trace" % len(w0),
@@ -217,7 +287,7 @@ def grains_leg(ctx, npaths: int, table_seed: int = 20):
     }
 
 
-def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
+def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20, cpu=None):
     """C5 screening batch (see the module docstring)."""
     from ptrans import autotune
     from ptrans.trace import Batch
@@ -271,7 +341,24 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, table_seed: int = 20):
     rate = st["paths"] / (st["kernel_ms"] * 1e-3)
     ks = c * np.sqrt(var32 / n)
     need = n * (ks / (1e-3 * k32)) ** 2
+    cpu_b = None
+    if cpu:
+        from oracle import oracle as O
+        mdesc, _ma = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
+        prob, alias = O.build_alias(alias_weights(omega, t.vel, t.mfp_nm, t.weight))
+        poff = np.concatenate([[0], np.cumsum(np.full(nstruct, paths_per_struct))])
+        descs = {s: geos[s].desc() for s in range(0, nstruct, 16)}     # 16 structures
+        units = [(s, int(poff[s]) + k) for k in range(0, min(10000, paths_per_struct), 500)
+                 for s in descs]
+
+        def work(u):
+            s, first = u
+            O.trace_alias(descs[s][0], mdesc, t.mfp_nm, prob, alias, 4244, first, 500)
+            return 500
+        cpu_b = cpu_leg_baseline(work, units, *cpu, "16 structures (every 16th) x up to 1e4 "
+                                 "paths each, brute-force candidate scans, 500 per task")
     return {
+        "cpu_baseline": cpu_b,
         "workload": "%d jittered 4x4x4 simple-cubic grain structures (1000 nm cube, +-5 nm "
                     "jitter, Gamma law, p = 0.3), %d - %d triangles each, 3x1000 modes, "
                     "%d paths per structure, one graph-replayed persistent launch, FP32 trace"
@@ -390,11 +477,15 @@ def main():
                              cstats=cstats)
             print("film: precision %d, %.2f ms per step" % (prec, dt / args.steps * 1e3),
                   file=sys.stderr, flush=True)
-        adj = adjoint_leg(ctx, int(args.adjoint_paths)) if args.adjoint_paths > 0 else None
+        leg_cpu = None
+        if not args.no_cpu:
+            leg_cpu = (min(args.cpu_seconds, 10.0), args.cpu_threads or min(16, os.cpu_count() or 1))
+        adj = (adjoint_leg(ctx, int(args.adjoint_paths), cpu=leg_cpu)
+               if args.adjoint_paths > 0 else None)
         if adj:
             print("adjoint: %.1f ms" % adj["adjoint_kernel_ms"], file=sys.stderr, flush=True)
-        grn = grains_leg(ctx, int(args.grain_paths)) if args.grain_paths > 0 else None
-        bat = (batch_leg(ctx, args.batch_structures, int(args.batch_paths))
+        grn = grains_leg(ctx, int(args.grain_paths), cpu=leg_cpu) if args.grain_paths > 0 else None
+        bat = (batch_leg(ctx, args.batch_structures, int(args.batch_paths), cpu=leg_cpu)
                if args.batch_structures > 0 else None)
     fast, plain, filt = res[PREC_FP32_RETEST], res[PREC_FP64], res[PREC_FP32_FILTER]
     dt = fast["dt"]
