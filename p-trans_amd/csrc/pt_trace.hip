@@ -429,6 +429,25 @@ __device__ int filt_box_axis(const DevGeom &g, int a, const double *rs, const do
     return F_HIT;
 }
 
+// Side of the start point relative to a z-cylinder in the reference's own terms: with
+// rr = rsc0*rsc0 + rsc1*rsc1 (the same FP64 operations as RT:3040-3088), +1 when
+// rr - r^2 > 1e-10 (rr + r^2), -1 when below minus that, else 0.  The margin exceeds
+// the rounding of the reference's d = rd^2 + r^2 - rr (at most ~16 eps (rr + r^2)) by
+// five orders of magnitude, so outside (+1) with rd < 0 makes its larger root
+// rd + sqrt(d) negative (a miss), and inside (-1) makes its smaller root rd - sqrt(d)
+// negative (the larger one is then the hit), whatever the rounding.
+__device__ inline int cyl_side64(const DSurf &s, const double *rs, const double *nd, bool *leaving)
+{
+    const double a = s.c[0] - rs[0], b = s.c[1] - rs[1];
+    const double rr = a * a + b * b, r2 = s.r * s.r;
+    const double m = 1.0e-10 * (rr + r2);
+    const double d = rr - r2;
+    // moving away from the axis: (c - p) . d_xy < 0, with a relative margin
+    const double rdu = a * nd[0] + b * nd[1];
+    *leaving = rdu < -1.0e-9 * (fabs(a * nd[0]) + fabs(b * nd[1]));
+    return d > m ? 1 : (d < -m ? -1 : 0);
+}
+
 // z-cylinder (calcSurfaceDistanceCylinder, RT:3040-3088)
 __device__ int filt_cylinder(const DSurf &s, const double *rs, const double *nd, double boxz,
                              float *lo, float *hi)
@@ -452,11 +471,17 @@ __device__ int filt_cylinder(const DSurf &s, const double *rs, const double *nd,
     const float ml = mrd + mD / sq + kMarg * (fabsf(rd) + sq) + 1e-30f;
     const float l1 = rd - sq, l2 = rd + sq;
     float l;
-    if (l1 > ml) l = l1;
-    else if (l1 >= -ml) return F_UNSURE;
-    else if (l2 > ml) l = l2;
-    else if (l2 >= -ml) return F_UNSURE;
-    else return F_MISS;
+    if (l1 > ml) {
+        l = l1;
+    } else {
+        // a root at the start point (the surface just left): its sign from FP64
+        bool leaving = false;
+        const int side = (l1 >= -ml || (l2 >= -ml && l2 <= ml)) ? cyl_side64(s, rs, nd, &leaving) : 0;
+        if (l1 >= -ml && side >= 0) return F_UNSURE;         // certified l1 < 0 only inside
+        if (l2 > ml) l = l2;
+        else if (l2 >= -ml) return (side > 0 && leaving) ? F_MISS : F_UNSURE;
+        else return F_MISS;
+    }
     const float L = l * inv;
     const float mL = ml * inv + kMarg * L;
     const float nz = (float)nd[2], rz = (float)rs[2];
@@ -468,6 +493,16 @@ __device__ int filt_cylinder(const DSurf &s, const double *rs, const double *nd,
     *lo = L - mL;
     *hi = L + mL;
     return F_HIT;
+}
+
+// The reference's outside-and-leaving guard of calcSurfaceDistanceSphere (RT:2985-3038),
+// with its own FP64 operations (dist_sphere's rd and rr): true when it reports no hit.
+__device__ inline bool sphere_guard64(const DSurf &s, const double *rs, const double *nd)
+{
+    double rsc[3] = {s.c[0] - rs[0], s.c[1] - rs[1], s.c[2] - rs[2]};
+    const double rd = dot3(rsc, nd);
+    const double rr = dot3(rsc, rsc);
+    return rr > s.r * s.r - PT_SPHERE_EPS && rd <= 0.0;
 }
 
 // sphere (calcSurfaceDistanceSphere, RT:2985-3038), including its outside-and-leaving guard
@@ -485,9 +520,10 @@ __device__ int filt_sphere(const DSurf &s, const double *rs, const double *nd, f
     const float out = rr - r2;                           // guard: rr > r^2 - 1e-10
     if (out > mrr) {
         if (rd < -mrd) return F_MISS;
-        if (rd <= mrd) return F_UNSURE;
+        if (rd <= mrd) return sphere_guard64(s, rs, nd) ? F_MISS : F_UNSURE;
     } else if (out >= -mrr) {
-        return F_UNSURE;
+        // on the surface (the one just left): the reference's own guard, in FP64
+        return sphere_guard64(s, rs, nd) ? F_MISS : F_UNSURE;
     }
     const float rd2 = rd * rd;
     const float D = rd2 + r2 - rr;
