@@ -173,6 +173,22 @@ module ptrans
       integer(c_int) :: st
     end function pt_trace_grid
 
+    function pt_trace_log(ctx, geom, modes, mfp_nm, nangle, vangle, np, first_path, stride, n, &
+         precision, max_points, points, npoints, records) bind(C, name='pt_trace_log') result(st)
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, geom, modes
+      real(c_double), value :: mfp_nm
+      integer(c_int32_t), value :: nangle
+      type(c_ptr), value :: vangle                 ! double [nangle] or c_null_ptr
+      integer(c_int64_t), value :: np, first_path, stride, n
+      integer(c_int32_t), value :: precision
+      integer(c_int64_t), value :: max_points
+      type(c_ptr), value :: points                 ! double [3, max_points, n]
+      type(c_ptr), value :: npoints                ! int64 [n]
+      type(c_ptr), value :: records                ! pt_path_record [n] or c_null_ptr
+      integer(c_int) :: st
+    end function pt_trace_log
+
     function pt_trace_alias(ctx, geom, modes, npaths, precision, branch_scale, mode_paths, &
          mode_trans, stats) bind(C, name='pt_trace_alias') result(st)
       import :: c_int, c_int32_t, c_int64_t, c_ptr

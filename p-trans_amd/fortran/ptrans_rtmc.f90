@@ -25,7 +25,7 @@ module ptrans_rtmc
   implicit none
   private
   public :: ptrans_rtmc_init, ptrans_transmission_function, ptrans_rtmc_finalize, &
-       ptrans_precision
+       ptrans_precision, ptrans_write_trace
 
   type(c_ptr), save :: ctx = c_null_ptr, geom = c_null_ptr, modes = c_null_ptr
   integer(c_int32_t), save :: ptrans_precision = PT_PREC_FP64
@@ -149,6 +149,39 @@ contains
     end if
     transmit(:,:,:) = dble(counts) / dble(param%nphonon)
   end subroutine ptrans_transmission_function
+
+  ! trace.out for the current sampling mean free path (RT:1661-1665, 1818-1831): the
+  ! paths first, first + stride, ... (n of them) of TransmissionFunction's numbering,
+  ! each as its start position and the position after every event, written with the
+  ! reference's own statement (write(fd_trace,*) real(Position)) and two empty records.
+  ! The reference traces every interval_trace-th particle of its OpenMP master thread;
+  ! stride = interval_trace gives the same density on the deterministic numbering.
+  subroutine ptrans_write_trace(first, stride, n, max_points, status)
+    integer(c_int64_t), intent(in) :: first, stride, n, max_points
+    integer, intent(out) :: status
+    real(c_double), allocatable, target :: pts(:,:,:), va(:)
+    integer(c_int64_t), allocatable, target :: npts(:)
+    integer(c_int64_t) :: i, k
+
+    allocate(va(param%nangle))
+    va = param%vangle
+    allocate(pts(3, max_points, n), npts(n))
+    status = pt_trace_log(ctx, geom, modes, param%MeanFreePath_current, &
+         int(param%nangle, c_int32_t), c_loc(va), int(param%nphonon, c_int64_t), first, &
+         stride, n, ptrans_precision, max_points, c_loc(pts), c_loc(npts), c_null_ptr)
+    if (status /= PT_OK) then
+      write(fd_stdout, '(" libptrans: ", a)') ptrans_error_message()
+      return
+    end if
+    if (fd_trace <= 0) return
+    do i = 1, n
+      do k = 1, min(npts(i), max_points)
+        write(fd_trace, *) real(pts(:, k, i))
+      end do
+      write(fd_trace, *)
+      write(fd_trace, *)
+    end do
+  end subroutine ptrans_write_trace
 
   subroutine ptrans_rtmc_finalize()
     integer(c_int) :: st

@@ -1,6 +1,11 @@
 ! ft_trace.f90 -- a Fortran host calling libptrans through module ptrans (iso_c_binding).
 !
 !   ft_trace <geom.bin> <modes.bin> <counts.bin> <seed> <mfp_nm> <nangle> <np> <precision>
+!            [<trace.out> <npaths> <stride>]
+!
+! With the optional arguments it also records paths 0, stride, ... with pt_trace_log and
+! writes them as the reference writes trace.out (write(u,*) real(position), two empty
+! records after each path).
 !
 ! Reads a flattened geometry and a mode table written by the Python test (native
 ! byte order, stream access; surface records read straight into type(pt_surface), so
@@ -18,7 +23,10 @@ program ft_trace
   integer(c_int32_t), allocatable, target :: goff(:), ginc(:)
   real(c_double), allocatable, target :: gen(:), vtx(:), omega(:)
   type(pt_surface), allocatable, target :: surf(:)
-  integer(c_int64_t), allocatable, target :: counts(:)
+  integer(c_int64_t), allocatable, target :: counts(:), npts(:)
+  real(c_double), allocatable, target :: pts(:,:,:)
+  character(len=512) :: ftrace
+  integer(c_int64_t) :: ntr, stride, i, k, maxp
   type(pt_geom_desc) :: gd
   type(pt_modes_desc) :: md
   type(pt_stats), target :: st
@@ -77,6 +85,28 @@ program ft_trace
   if (rc /= PT_OK) then
     write(*, '(a, i0, a, a)') 'ft_trace error ', rc, ': ', ptrans_error_message()
     stop 1
+  end if
+  if (command_argument_count() >= 11) then
+    call get_command_argument(9, ftrace)
+    call get_command_argument(10, arg); read(arg, *) ntr
+    call get_command_argument(11, arg); read(arg, *) stride
+    maxp = 4096
+    allocate(pts(3, maxp, ntr), npts(ntr))
+    rc = pt_trace_log(ctx, geom, modes, mfp, nangle, c_null_ptr, np, 0_c_int64_t, stride, ntr, &
+                      prec, maxp, c_loc(pts), c_loc(npts), c_null_ptr)
+    if (rc /= PT_OK) then
+      write(*, '(a, i0, a, a)') 'ft_trace error ', rc, ': ', ptrans_error_message()
+      stop 1
+    end if
+    open(newunit=u, file=trim(ftrace), status='replace')
+    do i = 1, ntr
+      do k = 1, min(npts(i), maxp)
+        write(u, *) real(pts(:, k, i))
+      end do
+      write(u, *)
+      write(u, *)
+    end do
+    close(u)
   end if
   rc = pt_modes_destroy(modes)
   rc = pt_geom_destroy(geom)

@@ -91,3 +91,33 @@ def test_fortran_host_counts_equal_ctypes_host(gpu_ctx, tmp_path, precision):
     c, _ = gpu_ctx.trace_grid(g, m, 300.0, 7, 2000, precision=precision)
     np.testing.assert_array_equal(fc, c)
     assert fc.sum() > 0
+
+
+@pytest.mark.gpu
+def test_fortran_host_trace_out_equals_python_writer(gpu_ctx, tmp_path):
+    """pt_trace_log through module ptrans, written by Fortran's own list-directed output
+    (as the reference writes trace.out), equals post.trace_out_lines on the positions the
+    ctypes host records: same paths, same text."""
+    from ptrans import post
+    exe = _built("ft_trace")
+    model = gen.film_holes(seed=2, nphonon=200, nangle=7, mfp=(300.0,))
+    geo = geometry.build_geometry(model)
+    om = np.linspace(1e12, 9e13, 6).reshape(2, 3)
+    write_geometry_bin(tmp_path / "geom.bin", geo)
+    with open(tmp_path / "modes.bin", "wb") as f:
+        f.write(np.array([2, 3], dtype=np.int32).tobytes())
+        f.write(np.float64(om.max()).tobytes())
+        f.write(np.asfortranarray(om).reshape(-1, order="F").astype(np.float64).tobytes())
+    seed = 17
+    r = subprocess.run([exe, str(tmp_path / "geom.bin"), str(tmp_path / "modes.bin"),
+                        str(tmp_path / "counts.bin"), str(seed), "300.0", "7", "200", "0",
+                        str(tmp_path / "trace.out"), "30", "7"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "ft_trace ok" in r.stdout, r.stdout + r.stderr
+    gpu_ctx.set_seed(seed)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om, max_omega=float(om.max()))
+    tr, _ = gpu_ctx.trace_log(g, m, 300.0, 7, 200, 0, 30, stride=7)
+    with open(tmp_path / "trace.out") as f:
+        text = f.read()
+    assert text == "\n".join(post.trace_out_lines(tr)) + "\n"
