@@ -181,7 +181,7 @@ def adjoint_leg(ctx, npaths: int, table_seed: int = 20, cpu=None):
     ctx.set_seed(777)              # the forward run's paths: both launches end with the same
     t0 = time.perf_counter()       # longest path, so the ratio is the adjoint's own cost
     _, _, sums, ast = ctx.adjoint_alias(g, m, npaths, surface_group=group, ngroup=ngroup,
-                                        baseline=base, nbatch=256, precision=FAST)
+                                        baseline=base, nbatch=64, precision=FAST)
     t_adj = time.perf_counter() - t0
     gr = post.alias_gradients(sums, length, ngroup)
     ctx.set_seed(777)
