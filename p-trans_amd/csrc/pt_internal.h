@@ -148,8 +148,8 @@ struct TraceParams {
 __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
 inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 {
-    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt +
-            (size_t)kBatchSums * p.bt.nstruct) * sizeof(double) +
+    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt) *
+               sizeof(double) +
            (p.bt.nstruct ? (size_t)kBvhStack * threads * sizeof(int32_t) : 0);
 }
 

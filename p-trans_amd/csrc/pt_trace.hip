@@ -1519,13 +1519,12 @@ trace_kernel(TraceParams P)
     al.gacc = (kAdj && P.grain_tally) ? adj_acc + P.nbatch * adj_k : nullptr;
     al.fgrain = -1;
     al.fdx = al.fl = al.fw = 0.0;
-    // batch: per-structure sums [n, x, x^2, (x - b) score, ((x - b) score)^2] and the
-    // lane's BVH stack
-    double *st_acc = adj_acc + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
-    int *stk = (int *)(st_acc + (kBatch ? kBatchSums * P.bt.nstruct : 0)) + threadIdx.x;
+    // batch: the lane's BVH stack (per-structure sums go straight to global atomics: a
+    // path end is rare next to its events, and the LDS they would take costs a workgroup
+    // per CU)
+    int *stk = (int *)(adj_acc + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0)) + threadIdx.x;
     if (kAdj) {
-        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0) +
-                         (kBatch ? kBatchSums * P.bt.nstruct : 0);
+        const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
         for (int i = threadIdx.x; i < nlds; i += blockDim.x) adj_lds[i] = 0.0;
         if (kBatch && blockIdx.x == 0 && threadIdx.x == 0) *P.bt.start = wall_clock64();
         __syncthreads();
@@ -1593,7 +1592,7 @@ trace_kernel(TraceParams P)
                 if (kBatch) {
                     const double x = transmitted ? 1.0 / P.m.mfp[mode] : 0.0;
                     const int sid = ph.sid;
-                    double *acc = st_acc + kBatchSums * sid;
+                    double *acc = P.bt.sums + kBatchSums * sid;
                     atomicAdd(acc + 0, 1.0);
                     atomicAdd(acc + 1, x);
                     atomicAdd(acc + 2, x * x);
@@ -1642,9 +1641,6 @@ trace_kernel(TraceParams P)
         if (al.gacc)
             for (int i = threadIdx.x; i < 2 * P.ngt; i += blockDim.x)
                 if (al.gacc[i] != 0.0) atomicAdd(P.grain_tally + i, al.gacc[i]);
-        if (kBatch)
-            for (int i = threadIdx.x; i < kBatchSums * P.bt.nstruct; i += blockDim.x)
-                if (st_acc[i] != 0.0) atomicAdd(P.bt.sums + i, st_acc[i]);
     }
     // stats slots: paths, events, draws, lost, mismatch, limit -> stats[0..5];
     // retests, fallbacks -> stats[10], stats[11]
