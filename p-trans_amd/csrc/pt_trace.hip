@@ -1171,6 +1171,46 @@ struct AdjLane {
     }
 };
 
+// createRandomDirectionAllSphere (RT:2192-2204) from its two draws: FP32 with the
+// renormalisation in the fast precision, the reference's FP64 otherwise.
+template <int kPrec>
+__device__ inline void scatter_dir(double u_ct, double u_ph, double *dir)
+{
+    if (kPrec == kPrecFast) {
+        const float ct = (float)u_ct * 2.0f - 1.0f;
+        float sph, cph;
+        fsincos_rev((float)u_ph, &sph, &cph);
+        const float st = __builtin_amdgcn_sqrtf(1.0f - ct * ct);
+        dir[0] = ct; dir[1] = st * cph; dir[2] = st * sph;
+        renorm_dir(dir);
+    } else {
+        double ct = u_ct;                                                   // RT:2197
+        double ph_ = u_ph;                                                  // RT:2198
+        ct = ct * 2.0 - 1.0;
+        ph_ = ph_ * kTwoPi;
+        const double st = sqrt(1.0 - ct * ct);
+        double sph, cph;
+        pt_sincos(ph_, &sph, &cph);
+        dir[0] = ct; dir[1] = st * cph; dir[2] = st * sph;
+    }
+}
+
+// createRandomFreePath (RT:2167-2173) from its draw
+template <int kPrec>
+__device__ inline double free_path_of(double mfp, double u)
+{
+    if (kPrec == kPrecFast) return free_path32(mfp, u);
+    return -(mfp * pt_log(u));                                              // RT:2171
+}
+
+// isInsideBox, RT:2177-2187
+__device__ inline bool inside_box(const DevGeom &g, const double *p)
+{
+    return p[0] >= 0.0 - kSmallVal && p[0] <= g.box[0] + kSmallVal &&
+           p[1] >= 0.0 - kSmallVal && p[1] <= g.box[1] + kSmallVal &&
+           p[2] >= 0.0 - kSmallVal && p[2] <= g.box[2] + kSmallVal;
+}
+
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
 template <int kPrec, bool kAdj, bool kBatch = false>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
@@ -1178,10 +1218,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
 {
     OpCount &oc = *rng.oc;
     oc.flops += 3;                       // box + SMALL_VAL
-    // isInsideBox, RT:2177-2187
-    if (!(ph.pos[0] >= 0.0 - kSmallVal && ph.pos[0] <= g.box[0] + kSmallVal &&
-          ph.pos[1] >= 0.0 - kSmallVal && ph.pos[1] <= g.box[1] + kSmallVal &&
-          ph.pos[2] >= 0.0 - kSmallVal && ph.pos[2] <= g.box[2] + kSmallVal)) {
+    if (!inside_box(g, ph.pos)) {
         ph.flags |= PT_PATH_LOST;
         return true;
     }
@@ -1230,23 +1267,9 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         }
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
-        if (kPrec == kPrecFast) {
-            const float ct = (float)rng.next() * 2.0f - 1.0f;
-            float sph, cph;
-            fsincos_rev((float)rng.next(), &sph, &cph);
-            const float st = __builtin_amdgcn_sqrtf(1.0f - ct * ct);
-            ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
-            renorm_dir(ph.dir);
-        } else {
-            double ct = rng.next();                                         // RT:2197
-            double ph_ = rng.next();                                        // RT:2198
-            ct = ct * 2.0 - 1.0;
-            ph_ = ph_ * kTwoPi;
-            const double st = sqrt(1.0 - ct * ct);
-            double sph, cph;
-            pt_sincos(ph_, &sph, &cph);
-            ph.dir[0] = ct; ph.dir[1] = st * cph; ph.dir[2] = st * sph;
-        }
+        const double u_ct = rng.next();
+        const double u_ph = rng.next();
+        scatter_dir<kPrec>(u_ct, u_ph, ph.dir);
         ph.cur = -1;
         oc.flops += 8 + kFlopsSincos;    // + 6 above
     } else {
@@ -1369,12 +1392,8 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         ph.fp = ph.fp - distance;
         oc.flops += 1;
         if (ph.fp < 1.0e-10) {
-            if (kPrec == kPrecFast) {
-                ph.fp = free_path32(ph.mfp, rng.next());
-            } else {
-                ph.fp = -(ph.mfp * pt_log(rng.next()));                      // RT:2171
-                oc.flops += 1 + kFlopsLog;
-            }
+            ph.fp = free_path_of<kPrec>(ph.mfp, rng.next());
+            if (kPrec != kPrecFast) oc.flops += 1 + kFlopsLog;
         }
     }
     // a diffusive path (mean free path under 5% of the smallest box side) that scattered
@@ -1397,6 +1416,102 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         }
     }
     return ph.end_side != 0;
+}
+
+__device__ inline double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ inline int64_t readlane_i64(int64_t v, int l)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)v & 0xffffffffull), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// A wave down to one path whose next free flight stays inside its safety ball (the tail
+// of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An in-ball
+// event is an internal scattering whose new direction and next free path depend only on
+// the path's three draws for it (direction: 2, free path: 1), so the 64 lanes compute
+// those for the next 64 events at once.  The sequential part of each event (inside-box
+// and event-limit checks, the ball test, the flight, the nudge, the resampled free path)
+// then runs on a copy of the path broadcast to every lane: all lanes hold the same
+// values, take the same branches and stay active, so each lane-indexed read below reads
+// a lane that computed its draws.  The loop stops at the first flight that leaves the
+// ball; the path's lane then takes the copy back.  Every value and every operation on
+// the path's state is the one step_path would make: the path is unchanged, bit for bit,
+// and its Rng continues at the right draw.
+template <int kPrec, bool kAdj>
+__device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, Path &ph,
+                              AdjLane &al, int src)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t path = (uint64_t)readlane_i64((int64_t)rng.path, src);
+    const double mfp = readlane_f64(ph.mfp, src);
+    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)rng.draw, src);
+    Path u;                                  // the fields in_ball and the event use
+    for (int k = 0; k < 3; ++k) {
+        u.pos[k] = readlane_f64(ph.pos[k], src);
+        u.dir[k] = readlane_f64(ph.dir[k], src);
+        u.bc[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ph.bc[k]), src));
+    }
+    u.br = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ph.br), src));
+    u.fp = readlane_f64(ph.fp, src);
+    u.events = readlane_i64(ph.events, src);
+    const int which = __builtin_amdgcn_readlane(ph.which, src);
+    uint32_t d = d0;
+    double spare = 0.0;
+    while (true) {
+        const uint64_t di = (uint64_t)d + 3ull * (uint64_t)lane;
+        const double u0 = pt_philox_draw(rng.seed, path, di);
+        const double u1 = pt_philox_draw(rng.seed, path, di + 1);
+        const double u2 = pt_philox_draw(rng.seed, path, di + 2);
+        double nd[3];
+        scatter_dir<kPrec>(u0, u1, nd);
+        const double nfp = free_path_of<kPrec>(mfp, u2);
+        int n = 0;
+        for (; n < 64; ++n) {
+            if (!inside_box(g, u.pos) || u.events >= event_limit || !in_ball(u)) break;
+            u.events++;
+            const double distance = u.fp;
+            if (kAdj && lane == src) {
+                al.nscat++;
+                al.flight(which, distance, u.dir[0]);
+                al.len += distance;
+            }
+            for (int k = 0; k < 3; ++k) u.pos[k] = u.pos[k] + distance * u.dir[k];
+            for (int k = 0; k < 3; ++k) u.dir[k] = readlane_f64(nd[k], n);
+            for (int k = 0; k < 3; ++k) u.pos[k] = u.pos[k] + PT_NUDGE * u.dir[k];
+            u.fp = u.fp - distance;                          // 0: resampled, RT:1783-1790
+            if (u.fp < 1.0e-10) u.fp = readlane_f64(nfp, n);
+        }
+        d += 3u * (uint32_t)n;
+        if (n < 64) {
+            // an odd next draw is the second half of a block: the Rng keeps it as its spare
+            spare = readlane_f64(u0, n);
+            break;
+        }
+    }
+    if (lane == src && d != d0) {
+        const uint32_t nev = (d - d0) / 3u;
+        for (int k = 0; k < 3; ++k) {
+            ph.pos[k] = u.pos[k];
+            ph.dir[k] = u.dir[k];
+        }
+        ph.fp = u.fp;
+        ph.events = u.events;
+        ph.cur = -1;
+        rng.draw = d;
+        if (d & 1u) rng.spare = spare;
+        OpCount &oc = *rng.oc;
+        // Philox blocks the path would have evaluated: one per even draw index
+        oc.philox += ((d + 1u) >> 1) - ((d0 + 1u) >> 1);
+        oc.flops += (uint64_t)nev * (3 + 6 + 8 + kFlopsSincos + 6 + 1 + (kPrec != kPrecFast ? 1 + kFlopsLog : 0));
+    }
 }
 
 // floor(a / b) for 0 <= a < 2^53, 1 <= b < 2^53: the FP64 quotient is within one of it
@@ -1577,6 +1692,16 @@ trace_kernel(TraceParams P)
         if (!__any(active)) {
             if (exhausted) break;
             continue;
+        }
+        if (kPrec != kPrecFP64 && exhausted) {
+            // one path left in the wave after the queue ran dry (the launch's tail), about
+            // to scatter inside its ball: run its in-ball events with the whole wave's help
+            const unsigned long long am = __ballot(active && ph.end_side == 0);
+            if (am != 0ull && (am & (am - 1ull)) == 0ull) {
+                const int src = (int)__builtin_ctzll(am);
+                const int go = __builtin_amdgcn_readlane((lane == src && in_ball(ph)) ? 1 : 0, src);
+                if (go) lone_ball_run<kPrec, kAdj>(P.g, P.event_limit, rng, ph, al, src);
+            }
         }
         if (active) {
             if (ph.end_side != 0 ||

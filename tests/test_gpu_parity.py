@@ -277,3 +277,36 @@ def test_safety_ball_alias_inclusions_bitexact(gpu_ctx):
     for f in ("end_side", "flags", "events", "draws", "which_grain"):
         np.testing.assert_array_equal(rec[f], orec[f], err_msg=f)
     np.testing.assert_array_equal(rec["pos"].view(np.uint64), orec["pos"].view(np.uint64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mfp", [3.0, 30.0])
+def test_lone_path_in_ball_fast_path_bitexact(gpu_ctx, mfp):
+    """Diffusive paths (one mode, mfp 3 or 30 nm, inclusions) traced by the persistent
+    kernel, whose waves end with single paths running their in-ball events with the whole
+    wave's help, against paths traced one per lane without it: identical per-mode tallies
+    and identical event and draw totals; and, in the bit-exact FP32 filter, the oracle's."""
+    from test_alias import _weights
+    model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.5, nphonon=1, nangle=3)
+    geo = geometry.build_geometry(model)
+    om = np.array([[2e13]])
+    vel, w = np.array([[5000.0]]), np.array([[1.0]])
+    lam = np.array([[mfp]])
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om, vel=vel, mfp=lam, weight=w)
+    n = 512
+    for prec in (1, 2):
+        gpu_ctx.set_seed(71)
+        mp, mt, st = gpu_ctx.trace_alias(g, m, n, precision=prec)
+        rec = gpu_ctx.replay_alias(g, m, 0, n, precision=prec)
+        print("mfp", mfp, "prec", prec, "events", st["events"], flush=True)
+        assert st["events"] == int(rec["events"].sum()) and st["draws"] == int(rec["draws"].sum())
+        assert int(mt.sum()) == int((rec["end_side"] != 1).sum())
+        if prec == 2:
+            gdesc, _g = geo.desc()
+            mdesc, _m = geometry.modes_desc(om, vel, lam, w)
+            prob, alias = O.build_alias(_weights(om, vel, lam, w))
+            op, ot, orec = O.trace_alias(gdesc, mdesc, lam, prob, alias, 71, 0, n, records=True)
+            np.testing.assert_array_equal(mt, ot)
+            assert st["events"] == int(orec["events"].sum())
+            np.testing.assert_array_equal(rec["pos"].view(np.uint64), orec["pos"].view(np.uint64))
