@@ -845,6 +845,25 @@ __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, d
 // event then skips the scan and is otherwise traced unchanged.  Distances are bounded in
 // FP32 and the radius shrunk by 1e-5 relative + 1e-2 nm, far above the rounding of both
 // the bound and the reference's FP64 hit distances.
+// Lower bound on the distance from (px, py, pz) to one surface: the plane of a polygon,
+// the shell of a sphere or cylinder (FP32; the ball's radius takes a relative and an
+// absolute margin off the minimum).  V: the vertex pool's address space.
+template <typename V>
+__device__ inline float surf_bound(const DSurf &s, V *vtx, float px, float py, float pz)
+{
+    if (s.kind == PT_SHAPE_POLYGON) {
+        V *v = vtx + 3 * (int64_t)s.voff;
+        const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
+        const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny + (pz - (float)v[2]) * nz;
+        return fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
+    }
+    const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
+    const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
+    return fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
+}
+
+__device__ inline float ball_radius_of(float r) { return r * (1.0f - 1.0e-5f) - 1.0e-2f; }
+
 __device__ void ball_setup(const DevGeom &g, Path &ph)
 {
     const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
@@ -860,19 +879,7 @@ __device__ void ball_setup(const DevGeom &g, Path &ph)
                 for (int j = j0; j < j1; ++j) {
                     const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
                     const DSurf s = load_surf_c(g.surf, idx);
-                    float d;
-                    if (s.kind == PT_SHAPE_POLYGON) {
-                        cdouble *v = (cdouble *)g.vtx + 3 * (int64_t)s.voff;
-                        const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
-                        const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny +
-                                        (pz - (float)v[2]) * nz;
-                        d = fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
-                    } else {
-                        const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
-                        const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
-                        d = fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
-                    }
-                    r = fminf(r, d);
+                    r = fminf(r, surf_bound(s, (cdouble *)g.vtx, px, py, pz));
                 }
                 if (todo0) todo0 = false;
                 else todo1 = false;
@@ -882,7 +889,84 @@ __device__ void ball_setup(const DevGeom &g, Path &ph)
     ph.bc[0] = px;
     ph.bc[1] = py;
     ph.bc[2] = pz;
-    ph.br = r * (1.0f - 1.0e-5f) - 1.0e-2f;
+    ph.br = ball_radius_of(r);
+}
+
+// The scan order of findCollisionSurface for a path in grain `which`: the box grain's
+// candidate list, then the grain's own (RT:1989-2096).  Position k of that order is the
+// global surface index cand(k); n positions in all.
+struct ScanOrder {
+    int a0, n0, b0, n;
+    __device__ ScanOrder(const DevGeom &g, int which)
+    {
+        a0 = g.cand_off[0];
+        n0 = g.cand_off[1] - a0;
+        b0 = which != 0 ? g.cand_off[which] : 0;
+        n = n0 + (which != 0 ? g.cand_off[which + 1] - b0 : 0);
+    }
+    __device__ int cand(const DevGeom &g, int k) const
+    {
+        return k < n0 ? g.cand_idx[a0 + k] : g.cand_idx[b0 + k - n0];
+    }
+};
+
+// ball_setup for one path of a wave down to a few, with the candidates spread over the
+// lanes: the minimum of the same per-surface bounds (an exact, order-free FP32 minimum),
+// so the ball equals ball_setup's.  Uniform control flow, every lane active, arguments
+// broadcast.
+__device__ float lone_ball_radius(const DevGeom &g, int which, float px, float py, float pz)
+{
+    const int lane = threadIdx.x & 63;
+    const ScanOrder so(g, which);
+    float r = 3.0e38f;
+    for (int c = 0; c < so.n; c += 64) {
+        const int k = c + lane;
+        if (k < so.n) r = fminf(r, surf_bound(g.surf[so.cand(g, k)], g.vtx, px, py, pz));
+    }
+    for (int m = 1; m < 64; m <<= 1) r = fminf(r, __shfl_xor(r, m, 64));
+    return ball_radius_of(r);
+}
+
+// findCollisionSurface (RT:1989-2096) for one path of a wave down to a few: lane k
+// evaluates the candidate at position k of the scan order with the reference's FP64
+// routine, and the wave keeps the smallest distance below 1e10, the earliest position
+// among equals: what scan_full's sequential strict update `dmin > l` ends with.  Uniform
+// control flow, every lane active, arguments broadcast.  Returns the winner's surface
+// index (-1: no hit) and its distance (1e10 with no hit).
+__device__ int lone_scan(const DevGeom &g, int which, const double *pos, const double *dir,
+                         double *dmin, OpCount &oc)
+{
+    const int lane = threadIdx.x & 63;
+    const ScanOrder so(g, which);
+    double bl = 1.0e10;
+    int bi = -1;
+    for (int c = 0; c < so.n; c += 64) {
+        const int k = c + lane;
+        double l = __builtin_huge_val();
+        int key = 0x7fffffff, idx = -1;
+        if (k < so.n) {
+            idx = so.cand(g, k);
+            double d;
+            if (surface_distance(g, g.surf[idx], pos, dir, &d, oc) && d < bl) {
+                l = d;
+                key = k;
+            }
+        }
+        for (int m = 1; m < 64; m <<= 1) {
+            const double ol = __shfl_xor(l, m, 64);
+            const int ok = __shfl_xor(key, m, 64);
+            if (ol < l || (ol == l && ok < key)) {
+                l = ol;
+                key = ok;
+            }
+        }
+        if (l < bl) {
+            bl = l;
+            bi = __shfl(idx, key - c, 64);
+        }
+    }
+    *dmin = bl;
+    return bi;
 }
 
 // Lower bound on the distance from a point to the surfaces of one BVH leaf, as in
@@ -891,21 +975,8 @@ __device__ inline float bvh_leaf_bound(const DevGeom &g, const DevBatch &bt, int
                                        float px, float py, float pz)
 {
     float r = 3.0e38f;
-    for (int k = first; k < first + count; ++k) {
-        const DSurf s = g.surf[bt.prims[k]];
-        float d;
-        if (s.kind == PT_SHAPE_POLYGON) {
-            const double *v = g.vtx + 3 * (int64_t)s.voff;
-            const float nx = (float)s.n[0], ny = (float)s.n[1], nz = (float)s.n[2];
-            const float a = (px - (float)v[0]) * nx + (py - (float)v[1]) * ny + (pz - (float)v[2]) * nz;
-            d = fabsf(a) * __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
-        } else {
-            const float dx = px - (float)s.c[0], dy = py - (float)s.c[1];
-            const float dz = s.kind == PT_SHAPE_SPHERE ? pz - (float)s.c[2] : 0.0f;
-            d = fabsf(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - (float)s.r);
-        }
-        r = fminf(r, d);
-    }
+    for (int k = first; k < first + count; ++k)
+        r = fminf(r, surf_bound(g.surf[bt.prims[k]], g.vtx, px, py, pz));
     return r;
 }
 
@@ -941,7 +1012,7 @@ __device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, i
     ph.bc[0] = px;
     ph.bc[1] = py;
     ph.bc[2] = pz;
-    ph.br = r * (1.0f - 1.0e-5f) - 1.0e-2f;
+    ph.br = ball_radius_of(r);
 }
 
 // True when the free flight from the current position stays inside the safety ball.
@@ -1212,9 +1283,12 @@ __device__ inline bool inside_box(const DevGeom &g, const double *p)
 }
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
+// pre_best != -2: the wave has already run this event's scan (lone_scan: winner pre_best,
+// -1 for none, at distance pre_dmin), and sets up a new ball itself (br = NaN until then).
 template <int kPrec, bool kAdj, bool kBatch = false>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
-                          Path &ph, AdjLane &al, const DevBatch *bt = nullptr, int *stk = nullptr)
+                          Path &ph, AdjLane &al, const DevBatch *bt = nullptr, int *stk = nullptr,
+                          int pre_best = -2, double pre_dmin = 0.0)
 {
     OpCount &oc = *rng.oc;
     oc.flops += 3;                       // box + SMALL_VAL
@@ -1237,11 +1311,16 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     // stays beyond it and the event is an internal scattering, as the scan would find)
     const bool in_safe = kPrec != kPrecFP64 && in_ball(ph);
     if (in_safe) decided = true;
+    if (!in_safe && pre_best != -2) {
+        dmin = pre_dmin;
+        if (pre_best >= 0) best = pre_best;
+        decided = true;
+    }
     if (kBatch && !in_safe) {
         scan_bvh(g, *bt, ph, &dmin, &best, oc, stk);
         decided = true;
     }
-    if (!kBatch && kPrec != kPrecFP64 && !in_safe) {
+    if (!kBatch && kPrec != kPrecFP64 && !decided) {
         decided = scan_fast(g, ph, &dmin, &best, oc);
         if (!decided) {
             dmin = 1.0e10;
@@ -1402,6 +1481,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         const double lim = 0.05 * fmin(g.box[0], fmin(g.box[1], g.box[2]));
         if (scattered && !in_safe && ph.mfp < lim) {
             if (kBatch) ball_setup_bvh(g, *bt, ph, stk, oc);
+            else if (pre_best != -2) ph.br = __builtin_nanf("");
             else ball_setup(g, ph);
         }
     }
@@ -1416,6 +1496,11 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         }
     }
     return ph.end_side != 0;
+}
+
+__device__ inline float readlane_f32(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 __device__ inline double readlane_f64(double v, int l)
@@ -1433,8 +1518,12 @@ __device__ inline int64_t readlane_i64(int64_t v, int l)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// A wave down to one path whose next free flight stays inside its safety ball (the tail
-// of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An in-ball
+// Paths left in a wave after its queue ran dry: at most this many are served one at a
+// time by the whole wave (ball set-up, in-ball events, scan), the rest step together.
+constexpr int kFewPaths = 4;
+
+// One path of a wave down to a few, whose next free flight stays inside its safety ball
+// (the tail of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An in-ball
 // event is an internal scattering whose new direction and next free path depend only on
 // the path's three draws for it (direction: 2, free path: 1), so the 64 lanes compute
 // those for the next 64 events at once.  The sequential part of each event (inside-box
@@ -1693,20 +1782,50 @@ trace_kernel(TraceParams P)
             if (exhausted) break;
             continue;
         }
-        if (kPrec != kPrecFP64 && exhausted) {
-            // one path left in the wave after the queue ran dry (the launch's tail), about
-            // to scatter inside its ball: run its in-ball events with the whole wave's help
-            const unsigned long long am = __ballot(active && ph.end_side == 0);
-            if (am != 0ull && (am & (am - 1ull)) == 0ull) {
-                const int src = (int)__builtin_ctzll(am);
-                const int go = __builtin_amdgcn_readlane((lane == src && in_ball(ph)) ? 1 : 0, src);
-                if (go) lone_ball_run<kPrec, kAdj>(P.g, P.event_limit, rng, ph, al, src);
+        int pre_best = -2;
+        double pre_dmin = 0.0;
+        if (kPrec != kPrecFP64 && kMode != TRACE_GRID && exhausted) {
+            // at most kFewPaths paths left in the wave after the queue ran dry (the launch's
+            // tail): for each in turn, the whole wave sets up its ball, runs its in-ball
+            // events, and scans its next event; the event itself then runs in step_path
+            unsigned long long am = __ballot(active && ph.end_side == 0);
+            if (am != 0ull && __popcll(am) <= kFewPaths) {
+                while (am != 0ull) {
+                    const int src = (int)__builtin_ctzll(am);
+                    am &= am - 1ull;
+                    const int which = __builtin_amdgcn_readlane(ph.which, src);
+                    if (!kBatch && __builtin_isnan(readlane_f32(ph.br, src))) {
+                        float c[3];
+                        for (int k = 0; k < 3; ++k) c[k] = readlane_f32((float)ph.pos[k], src);
+                        const float r = lone_ball_radius(P.g, which, c[0], c[1], c[2]);
+                        if (lane == src) {
+                            for (int k = 0; k < 3; ++k) ph.bc[k] = c[k];
+                            ph.br = r;
+                        }
+                    }
+                    const int go = __builtin_amdgcn_readlane((lane == src && in_ball(ph)) ? 1 : 0, src);
+                    if (go) lone_ball_run<kPrec, kAdj>(P.g, P.event_limit, rng, ph, al, src);
+                    const int scan = __builtin_amdgcn_readlane((lane == src && !in_ball(ph)) ? 1 : 0, src);
+                    if (!kBatch && scan) {
+                        double p[3], d[3];
+                        for (int k = 0; k < 3; ++k) {
+                            p[k] = readlane_f64(ph.pos[k], src);
+                            d[k] = readlane_f64(ph.dir[k], src);
+                        }
+                        double dl;
+                        const int bi = lone_scan(P.g, which, p, d, &dl, oc);
+                        if (lane == src) {
+                            pre_best = bi;
+                            pre_dmin = dl;
+                        }
+                    }
+                }
             }
         }
         if (active) {
             if (ph.end_side != 0 ||
                 step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
-                                               &P.bt, stk)) {
+                                               &P.bt, stk, pre_best, pre_dmin)) {
                 const bool transmitted = ph.end_side != 1;          // RT:1833
                 if (kMode == TRACE_GRID) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
