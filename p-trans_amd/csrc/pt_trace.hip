@@ -87,9 +87,12 @@ typedef const __attribute__((address_space(4))) double cdouble;
 typedef const __attribute__((address_space(4))) DSurf CSurf;
 typedef const __attribute__((address_space(4))) int32_t cint;
 
+// calcSurfaceDistancePolygon, RT:2907-2959, in two parts: the one-sided plane hit
+// (cosang > 0, l >= 0), then the convex point-in-polygon test of the hit point with
+// on-edge counted inside.
 template <class VT>
-__device__ bool dist_polygon_v(VT *vtx, const DSurf &s, const double *rs, const double *nd,
-                               double *lhit, OpCount &oc)
+__device__ inline bool polygon_plane(VT *vtx, const DSurf &s, const double *rs, const double *nd,
+                                     double *lhit, OpCount &oc)
 {
     const double cosang = nd[0] * s.n[0] + nd[1] * s.n[1] + nd[2] * s.n[2];
     oc.flops += 5;
@@ -99,6 +102,15 @@ __device__ bool dist_polygon_v(VT *vtx, const DSurf &s, const double *rs, const 
     double d[3] = {v[0] - rs[0], v[1] - rs[1], v[2] - rs[2]};
     const double l = (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]) / cosang;
     if (l < 0.0) return false;
+    *lhit = l;
+    return true;
+}
+
+template <class VT>
+__device__ inline bool polygon_contains(VT *vtx, const DSurf &s, const double *rs,
+                                        const double *nd, double l, OpCount &oc)
+{
+    VT *v = vtx + 3 * (int64_t)s.voff;
     double h[3];
     for (int k = 0; k < 3; ++k) h[k] = rs[k] + nd[k] * l;
     oc.flops += 6;
@@ -116,6 +128,16 @@ __device__ bool dist_polygon_v(VT *vtx, const DSurf &s, const double *rs, const 
             return false;
         }
     }
+    return true;
+}
+
+template <class VT>
+__device__ bool dist_polygon_v(VT *vtx, const DSurf &s, const double *rs, const double *nd,
+                               double *lhit, OpCount &oc)
+{
+    double l;
+    if (!polygon_plane(vtx, s, rs, nd, &l, oc) || !polygon_contains(vtx, s, rs, nd, l, oc))
+        return false;
     *lhit = l;
     return true;
 }
@@ -550,10 +572,16 @@ struct Pick {
     int idx_hi, idx_lo1;
     double exact_l;       // FP64 distance of idx_hi when it was evaluated exactly
     bool exact;
-    __device__ void init() { hi_min = lo1 = lo2 = 1.0e300; idx_hi = idx_lo1 = -1; exact = false; }
-    __device__ void add(int idx, double lo, double hi, bool is_exact)
+    bool plane;           // idx_hi is a polygon's plane hit, its in-polygon test still due
+    __device__ void init()
     {
-        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = is_exact; exact_l = lo; }
+        hi_min = lo1 = lo2 = 1.0e300;
+        idx_hi = idx_lo1 = -1;
+        exact = plane = false;
+    }
+    __device__ void add(int idx, double lo, double hi, bool is_exact, bool is_plane = false)
+    {
+        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = is_exact; exact_l = lo; plane = is_plane; }
         if (lo < lo1) { lo2 = lo1; lo1 = lo; idx_lo1 = idx; }
         else if (lo < lo2) lo2 = lo;
     }
@@ -633,21 +661,28 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                 for (int j = j0; j < j1; ++j) {
                     const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
                     const DSurf s = load_surf_c(g.surf, idx);
+                    if (s.kind == PT_SHAPE_POLYGON) {
+                        // the FP64 plane hit is the polygon's distance if the hit point is
+                        // inside it: a lower bound that only the nearest candidate needs to
+                        // turn into a hit (the in-polygon test below)
+                        double l;
+                        if (polygon_plane((cdouble *)g.vtx, s, rs, nd, &l, oc)) pk.add(idx, l, l, true, true);
+                        continue;
+                    }
                     float lo = 0.f, hi = 0.f;
                     int r;
                     if (s.kind == PT_SHAPE_CYLINDER) { r = filt_cylinder(s, rs, nd, g.box[2], &lo, &hi); oc.flops += 30; }
-                    else if (s.kind == PT_SHAPE_SPHERE) { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
-                    else r = F_UNSURE;                        // polygons: FP64 routine
+                    else { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
                     if (r == F_HIT) {
                         pk.add(idx, lo, hi, false);
                     } else if (r == F_UNSURE) {
-                        if (s.kind != PT_SHAPE_POLYGON && q0 < 0) {
+                        if (q0 < 0) {
                             q0 = idx;
-                        } else if (s.kind != PT_SHAPE_POLYGON && q1 < 0) {
+                        } else if (q1 < 0) {
                             q1 = idx;
                         } else {
                             double l;
-                            if (s.kind != PT_SHAPE_POLYGON) ph.retests++;
+                            ph.retests++;
                             if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
                         }
                     }
@@ -667,6 +702,7 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
         }
     }
     if (pk.idx_hi < 0 || pk.idx_lo1 != pk.idx_hi || !(pk.lo2 > pk.hi_min)) return false;
+    if (pk.plane && !polygon_contains(g.vtx, g.surf[pk.idx_hi], rs, nd, pk.exact_l, oc)) return false;
     double l = pk.exact_l;
     if (!pk.exact) {
         if (g.box_aligned && pk.idx_hi < 6) {

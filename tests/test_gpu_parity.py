@@ -130,15 +130,20 @@ def test_fast_replay_identical_to_fp64(gpu_ctx, name):
     np.testing.assert_array_equal(fast["dir"].view(np.uint64), ref["dir"].view(np.uint64))
 
 
-@pytest.mark.parametrize("builder", ["film", "inclusions"])
+@pytest.mark.parametrize("builder", ["film", "inclusions", "voronoi"])
 def test_fast_grid_counts_identical_to_fp64_large(gpu_ctx, builder):
     """Half a million to a few million paths: FP32-filter counts, events and draws equal the
-    FP64 trace's exactly; the fraction of events needing the FP64 full scan is small."""
+    FP64 trace's exactly; the fraction of events needing the FP64 full scan is small.
+    Voronoi cells are all polygons, which the filter bounds by their FP64 plane hits and
+    tests for containment only when nearest."""
     if builder == "film":
         model = gen.film_holes(seed=2, nphonon=20000, nangle=11, mfp=(300.0,))
-    else:
+    elif builder == "inclusions":
         model = gen.inclusions(seed=3, kind="Gamma", value=2.0, p=0.4, nphonon=20000, nangle=11,
                                mfp=(500.0,))
+    else:
+        model = gen.voronoi_cube(seed=4, kind="Gamma", value=1.5, p=0.2, nphonon=20000, nangle=11,
+                                 mfp=(150.0,))
     geo = geometry.build_geometry(model)
     om = np.linspace(1e12, 9e13, 6).reshape(2, 3)
     gpu_ctx.set_seed(31)
