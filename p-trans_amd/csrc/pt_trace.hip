@@ -796,10 +796,21 @@ __device__ inline void bvh_leaf_hits(const DevGeom &g, const DevBatch &bt, const
 {
     for (int k = first; k < first + count; ++k) {
         const int idx = bt.prims[k];
+        const DSurf &s = g.surf[idx];
         double l;
         ++oc.tests;
-        if (surface_distance(g, g.surf[idx], ph.pos, ph.dir, &l, oc) &&
-            (l < *bl || (*hit && l == *bl && idx < *bi))) {
+        bool h;
+        if (s.kind == PT_SHAPE_POLYGON) {
+            // plane hit first: the in-polygon test (and its vertex loads) only for a hit
+            // that would become the nearest
+            h = polygon_plane(g.vtx, s, ph.pos, ph.dir, &l, oc) &&
+                (l < *bl || (*hit && l == *bl && idx < *bi)) &&
+                polygon_contains(g.vtx, s, ph.pos, ph.dir, l, oc);
+        } else {
+            h = surface_distance(g, s, ph.pos, ph.dir, &l, oc) &&
+                (l < *bl || (*hit && l == *bl && idx < *bi));
+        }
+        if (h) {
             *bl = l;
             *bi = idx;
             *hit = true;
