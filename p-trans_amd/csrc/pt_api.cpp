@@ -276,6 +276,8 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     if (!d->grain_offset || !d->surfaces || (ns > 0 && d->nvertex > 0 && !d->vertices) ||
         !d->grain_incident || !d->generatrix)
         return fail(PT_E_INVALID, "pt_geom_create: NULL array in the descriptor");
+    if (ns >= (1 << 30))      // the trace flags candidate indices with bit 30
+        return fail(PT_E_GEOMETRY, "nsurface %d: at most 2^30 - 1 surfaces", ns);
     if (d->grain_offset[0] != 0 || d->grain_offset[ng] != ns)
         return fail(PT_E_GEOMETRY, "grain_offset must run from 0 to nsurface");
     for (int g = 0; g < ng; ++g)

@@ -567,21 +567,19 @@ __device__ int filt_sphere(const DSurf &s, const double *rs, const double *nd, f
 }
 
 // Running selection over candidate intervals in scan order.
+// Candidate index flag: a polygon entered with its FP64 plane hit, whose in-polygon test
+// is still due (surface indices stay below 2^30).
+constexpr int kPlaneBit = 1 << 30;
+
 struct Pick {
     double hi_min, lo1, lo2;
     int idx_hi, idx_lo1;
     double exact_l;       // FP64 distance of idx_hi when it was evaluated exactly
     bool exact;
-    bool plane;           // idx_hi is a polygon's plane hit, its in-polygon test still due
-    __device__ void init()
+    __device__ void init() { hi_min = lo1 = lo2 = 1.0e300; idx_hi = idx_lo1 = -1; exact = false; }
+    __device__ void add(int idx, double lo, double hi, bool is_exact)
     {
-        hi_min = lo1 = lo2 = 1.0e300;
-        idx_hi = idx_lo1 = -1;
-        exact = plane = false;
-    }
-    __device__ void add(int idx, double lo, double hi, bool is_exact, bool is_plane = false)
-    {
-        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = is_exact; exact_l = lo; plane = is_plane; }
+        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = is_exact; exact_l = lo; }
         if (lo < lo1) { lo2 = lo1; lo1 = lo; idx_lo1 = idx; }
         else if (lo < lo2) lo2 = lo;
     }
@@ -661,25 +659,25 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                 for (int j = j0; j < j1; ++j) {
                     const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
                     const DSurf s = load_surf_c(g.surf, idx);
-                    if (s.kind == PT_SHAPE_POLYGON) {
-                        // the FP64 plane hit is the polygon's distance if the hit point is
-                        // inside it: a lower bound that only the nearest candidate needs to
-                        // turn into a hit (the in-polygon test below)
-                        double l;
-                        if (polygon_plane((cdouble *)g.vtx, s, rs, nd, &l, oc)) pk.add(idx, l, l, true, true);
-                        continue;
-                    }
                     float lo = 0.f, hi = 0.f;
                     int r;
                     if (s.kind == PT_SHAPE_CYLINDER) { r = filt_cylinder(s, rs, nd, g.box[2], &lo, &hi); oc.flops += 30; }
-                    else { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
+                    else if (s.kind == PT_SHAPE_SPHERE) { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
+                    else r = F_UNSURE;                        // polygons: FP64 plane hit
                     if (r == F_HIT) {
                         pk.add(idx, lo, hi, false);
                     } else if (r == F_UNSURE) {
-                        if (q0 < 0) {
+                        if (s.kind != PT_SHAPE_POLYGON && q0 < 0) {
                             q0 = idx;
-                        } else if (q1 < 0) {
+                        } else if (s.kind != PT_SHAPE_POLYGON && q1 < 0) {
                             q1 = idx;
+                        } else if (s.kind == PT_SHAPE_POLYGON) {
+                            // the plane hit is the polygon's distance if the hit point is
+                            // inside it: a lower bound that only the winner needs to turn
+                            // into a hit (the in-polygon test below)
+                            double l;
+                            if (polygon_plane((cdouble *)g.vtx, s, rs, nd, &l, oc))
+                                pk.add(idx | kPlaneBit, l, l, true);
                         } else {
                             double l;
                             ph.retests++;
@@ -702,19 +700,21 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
         }
     }
     if (pk.idx_hi < 0 || pk.idx_lo1 != pk.idx_hi || !(pk.lo2 > pk.hi_min)) return false;
-    if (pk.plane && !polygon_contains(g.vtx, g.surf[pk.idx_hi], rs, nd, pk.exact_l, oc)) return false;
+    const int win = pk.idx_hi & ~kPlaneBit;
+    if ((pk.idx_hi & kPlaneBit) && !polygon_contains(g.vtx, g.surf[win], rs, nd, pk.exact_l, oc))
+        return false;
     double l = pk.exact_l;
     if (!pk.exact) {
-        if (g.box_aligned && pk.idx_hi < 6) {
-            l = box_wall_distance(g, pk.idx_hi, rs, nd);
+        if (g.box_aligned && win < 6) {
+            l = box_wall_distance(g, win, rs, nd);
             oc.flops += 2;
-        } else if (!surface_distance(g, g.surf[pk.idx_hi], rs, nd, &l, oc)) {
+        } else if (!surface_distance(g, g.surf[win], rs, nd, &l, oc)) {
             return false;
         }
     }
     if (!(1.0e10 > l)) return false;                      // reference: dmin starts at 1e10
     *dmin = l;
-    *best = pk.idx_hi;
+    *best = win;
     return true;
 }
 
