@@ -224,7 +224,8 @@ int pt_ctx_set_op_counting(pt_ctx *ctx, int32_t on);
 int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t blocks_per_cu);
 
 /* ---- geometry ------------------------------------------------------------------ */
-/* Uploads vgrain (RT:122) as flattened by the host.  Validates indices. */
+/* Uploads vgrain (RT:122) as flattened by the host.  Validates indices; at most
+ * 2^30 - 1 surfaces. */
 int pt_geom_create(pt_ctx *ctx, const pt_geom_desc *desc, pt_geom **out);
 int pt_geom_destroy(pt_geom *geom);
 

@@ -94,6 +94,12 @@ def test_gpu_size_limits_and_invalid_arguments(gpu_ctx):
                           2 ** 30, small.ctypes.data, small.ctypes.data, None) == -1
     assert "4 GiB" in _lib.last_error()
     assert not np.any(small)
+    # 2^30 surfaces: beyond the candidate-index range, refused before the arrays are read
+    gdesc, _keep = geo.desc()
+    gdesc.nsurface = 2 ** 30
+    out = ctypes.c_void_p()
+    assert L.pt_geom_create(gpu_ctx.handle, ctypes.byref(gdesc), ctypes.byref(out)) == -4
+    assert "2^30" in _lib.last_error() and not out.value
     # the last path of the numbering is valid
     assert len(gpu_ctx.replay(g, m, 300.0, 5, 10, 239, 1)) == 1
 
