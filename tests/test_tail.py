@@ -1,0 +1,70 @@
+"""The last few paths of a wave.  Once the work queue is empty, a wave with four or fewer
+paths left serves them one at a time with all 64 lanes: it sets up a pending safety ball,
+runs batches of in-ball events (their draws computed 64 at a time) and spreads the next
+candidate scan over the lanes.  Diffusive modes make such paths common (they end a launch
+alone), so these tests trace them at the FP32-filter precision, whose paths are bit-identical
+to FP64, and pin everything such a path updates against the oracle: adjoint scores
+(scattering counts and path lengths), per-grain flux tallies (flights inside one grain) and
+the per-mode tallies.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ptrans import gen, geometry
+from ptrans.anphon import synthetic_table
+from test_adjoint import OracleSetup, _weights
+
+M_TWOPI = 3.14159265358979323846 * 2.0
+FILTER = 2
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_diffusive_tail_equals_oracle(gpu_ctx):
+    """Mean free paths of 2 - 20 nm in a 300 x 100 x 100 nm wire (safety balls below 5 nm),
+    branch scales 0.8 and 1.25: adjoint batch sums equal the oracle's."""
+    model = gen.nanowire(length=300.0, width=100.0, height=100.0, p_wall=0.5, nphonon=1, nangle=3)
+    nsurf = len(geometry.build_geometry(model).flatten()["surfaces"])
+    group = [-1] * nsurf
+    group[2] = group[3] = 0
+    group[4] = group[5] = 1
+    s = OracleSetup(model, 2, 3, 11, (2.0, 20.0), [0.35, 0.6], group=group)
+    scale = np.array([0.8, 1.25])
+    gpu_ctx.set_seed(19)
+    g = gpu_ctx.geometry(s.geo)
+    g.set_specularity(s.p0)
+    m = gpu_ctx.modes(omega=s.omega, vel=s.vel, mfp=s.mfp, weight=s.weight)
+    mp, mt, gs, st = gpu_ctx.adjoint_alias(g, m, 20000, surface_group=s.group, ngroup=s.ngroup,
+                                           baseline=0.002, nbatch=16, branch_scale=scale,
+                                           precision=FILTER)
+    print("events per path", st["events"] / st["paths"], flush=True)
+    assert st["events"] / st["paths"] > 30
+    os_ = s.sums(19, 20000, 16, baseline=0.002, scale=scale)
+    np.testing.assert_array_equal(gs[:, 0], os_[:, 0])
+    np.testing.assert_allclose(gs, os_, rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_grain_tallies_diffusive_tail_equal_oracle(gpu_ctx):
+    """Mean free paths of 3 - 30 nm in a 300 nm Voronoi cube with Lorentzian boundaries
+    (safety balls below 15 nm): per-mode and per-grain tallies equal the oracle's."""
+    m_ = gen.voronoi_cube(seed=4, L=300.0, ngrain=5, kind="Gamma", value=1.5, p=0.3, nphonon=1,
+                          nangle=3)
+    geo = geometry.build_geometry(m_)
+    geometry.set_lorentz_boundaries(geo, np.random.default_rng(3))
+    t = synthetic_table(2, 5, 7, mfp_min_nm=3.0, mfp_max_nm=30.0)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    prob, alias = O.build_alias(_weights(omega, t.vel, t.mfp_nm, t.weight))
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
+    mp, mt, gt = O.trace_alias_grains(gdesc, mdesc, t.mfp_nm, prob, alias, 23, 0, 20000,
+                                      len(geo.grains))
+    gpu_ctx.set_seed(23)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    gp, gtr, gg, st = gpu_ctx.trace_alias_grains(g, m, 20000, precision=FILTER)
+    print("events per path", st["events"] / st["paths"], flush=True)
+    assert st["events"] / st["paths"] > 30
+    np.testing.assert_array_equal(gp, mp)
+    np.testing.assert_array_equal(gtr, mt)
+    np.testing.assert_allclose(gg, gt, rtol=1e-11, atol=1e-13)
