@@ -618,6 +618,15 @@ hipError_t launch_blocks(const pt_ctx *c, int mode, int precision, size_t lds, i
     return hipSuccess;
 }
 
+// Paths a wave takes from the queue at once: the context's chunk, cut down for a launch too
+// small to give every wave eight chunks, so that the queue still balances long paths over
+// the waves.  Results do not depend on it (paths are keyed by id).
+int32_t wave_chunk(const pt_ctx *c, int64_t total, int blocks)
+{
+    const int64_t waves = std::max<int64_t>(1, (int64_t)blocks * (c->threads / 64));
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(c->chunk, total / (8 * waves)));
+}
+
 int set_scale(pt_ctx *c, const pt_modes *M, const double *scale, TraceParams &P)
 {
     P.m.scale = nullptr;
@@ -764,6 +773,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         P.adj = d_sum;
         P.grain_tally = d_gt;
         if ((e = launch_blocks(c, TRACE_ADJOINT, precision, lds, &blocks))) break;
+        P.chunk = wave_chunk(c, P.total, blocks);
         if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
@@ -848,13 +858,19 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     unsigned long long host_ctr[16] = {0};
     float ms = 0.f, ms_red = 0.f;
     int blocks = 0;
+    // a diffusive sampling mean free path (one that gets safety balls) gets the kernel that
+    // serves the last paths of each wave with the whole wave
+    const double *bx = G->dg.box;
+    const double lim = kBallMfpFrac * std::fmin(bx[0], std::fmin(bx[1], bx[2]));
+    const int gmode = precision != PT_PREC_FP64 && mfp < lim ? TRACE_GRID_TAIL : TRACE_GRID;
     do {
-        if ((e = launch_blocks(c, TRACE_GRID, precision, 0, &blocks))) break;
+        if ((e = launch_blocks(c, gmode, precision, 0, &blocks))) break;
+        P.chunk = wave_chunk(c, P.total, blocks);
         if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_GRID, blocks, c->threads, c->count_ops, 0, c->stream))) break;
+        if ((e = launch_trace(P, gmode, blocks, c->threads, c->count_ops, 0, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
         if ((e = hipEventRecord(c->ev2, c->stream))) break;
@@ -965,6 +981,7 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     hipError_t e;
     do {
         if ((e = launch_blocks(c, TRACE_ALIAS, precision, 0, &blocks))) break;
+        P.chunk = wave_chunk(c, P.total, blocks);
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;

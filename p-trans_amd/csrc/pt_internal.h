@@ -68,8 +68,17 @@ enum : uint8_t {
 };
 
 // TRACE_ADJOINT: the alias-mode trace plus likelihood-ratio scores of each path;
-// TRACE_BATCH: the adjoint-mode trace over a batch of mesh structures (BVH scans)
-enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3 };
+// TRACE_BATCH: the adjoint-mode trace over a batch of mesh structures (BVH scans);
+// TRACE_GRID_TAIL: the grid trace of a diffusive sampling mean free path (one that gets
+// safety balls, below kBallMfpFrac of the smallest box side), with the few-path service at
+// the end of each wave.  The ballistic grid trace keeps the leaner TRACE_GRID kernel.
+enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3, TRACE_GRID_TAIL = 4 };
+
+constexpr bool is_grid_mode(int mode) { return mode == TRACE_GRID || mode == TRACE_GRID_TAIL; }
+
+// Safety balls (accelerated precisions) are set up for paths whose mean free path is below
+// this fraction of the smallest box side.
+constexpr double kBallMfpFrac = 0.05;
 
 // Bounding volume hierarchy node: FP32 bounds padded outward on the host (so FP32 slab
 // tests never drop a true hit); a >= 0: children a and b; a < 0: leaf of prims

@@ -1522,10 +1522,10 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             if (kPrec != kPrecFast) oc.flops += 1 + kFlopsLog;
         }
     }
-    // a diffusive path (mean free path under 5% of the smallest box side) that scattered
-    // outside its ball takes a new ball around the scattering point
+    // a diffusive path (mean free path under kBallMfpFrac of the smallest box side) that
+    // scattered outside its ball takes a new ball around the scattering point
     if (kPrec != kPrecFP64) {
-        const double lim = 0.05 * fmin(g.box[0], fmin(g.box[1], g.box[2]));
+        const double lim = kBallMfpFrac * fmin(g.box[0], fmin(g.box[1], g.box[2]));
         if (scattered && !in_safe && ph.mfp < lim) {
             if (kBatch) ball_setup_bvh(g, *bt, ph, stk, oc);
             else if (pre_best != -2) ph.br = __builtin_nanf("");
@@ -1700,7 +1700,7 @@ template <int kMode, int kPrec>
 __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, Path &ph, int *mode)
 {
     rng.reset((uint64_t)pid);
-    if (kMode == TRACE_GRID) {
+    if (is_grid_mode(kMode)) {
         int ib, ik, ia;
         cell_of_path(pid, P.np, P.m.nb, P.nangle, &ib, &ik, &ia);
         ph.omega = cell_omega(P.m, ib, ik);
@@ -1816,7 +1816,7 @@ trace_kernel(TraceParams P)
                         active = true;
                         if (kAdj) al.fw = 1.0 / P.m.mfp[mode];
                     } else {
-                        if (kMode == TRACE_GRID) P.outcome[pid] = OUT_MISMATCH;
+                        if (is_grid_mode(kMode)) P.outcome[pid] = OUT_MISMATCH;
                         st[0] += 1ull;
                         st[256 * 4] += 1ull;
                         st[256 * 2] += rng.draw;
@@ -1874,7 +1874,7 @@ trace_kernel(TraceParams P)
                 step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
                                                &P.bt, stk, pre_best, pre_dmin)) {
                 const bool transmitted = ph.end_side != 1;          // RT:1833
-                if (kMode == TRACE_GRID) {
+                if (is_grid_mode(kMode)) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
                 } else {
                     atomicAdd(P.mode_paths + mode, 1ull);
@@ -2060,6 +2060,9 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
         else PT_LAUNCH_C(M, kPrecFP64);                                   \
     } while (0)
     if (mode == TRACE_GRID) PT_LAUNCH_R(TRACE_GRID);
+    else if (mode == TRACE_GRID_TAIL && p.precision == kPrecFast) PT_LAUNCH_C(TRACE_GRID_TAIL, kPrecFast);
+    else if (mode == TRACE_GRID_TAIL && p.precision == kPrecFilter) PT_LAUNCH_C(TRACE_GRID_TAIL, kPrecFilter);
+    else if (mode == TRACE_GRID_TAIL) PT_LAUNCH_C(TRACE_GRID, kPrecFP64);
     else if (mode == TRACE_ALIAS) PT_LAUNCH_R(TRACE_ALIAS);
     else if (mode == TRACE_ADJOINT) PT_LAUNCH_R(TRACE_ADJOINT);
     else PT_LAUNCH_R(TRACE_BATCH);
@@ -2075,7 +2078,10 @@ hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int
 #define PT_OCC_R(M)                                                                  \
     (precision == kPrecFast ? PT_OCC(M, kPrecFast)                                   \
                             : precision == kPrecFilter ? PT_OCC(M, kPrecFilter) : PT_OCC(M, kPrecFP64))
-    if (mode == TRACE_GRID) return PT_OCC_R(TRACE_GRID);
+    if (mode == TRACE_GRID || (mode == TRACE_GRID_TAIL && precision == kPrecFP64))
+        return PT_OCC_R(TRACE_GRID);
+    if (mode == TRACE_GRID_TAIL)
+        return precision == kPrecFast ? PT_OCC(TRACE_GRID_TAIL, kPrecFast) : PT_OCC(TRACE_GRID_TAIL, kPrecFilter);
     if (mode == TRACE_ALIAS) return PT_OCC_R(TRACE_ALIAS);
     if (mode == TRACE_ADJOINT) return PT_OCC_R(TRACE_ADJOINT);
     return PT_OCC_R(TRACE_BATCH);
@@ -2088,7 +2094,10 @@ const void *trace_kernel_fn(int mode, int precision)
 #define PT_FN(M) (precision == kPrecFast ? (const void *)trace_kernel<M, false, kPrecFast>            \
                   : precision == kPrecFilter ? (const void *)trace_kernel<M, false, kPrecFilter>      \
                                              : (const void *)trace_kernel<M, false, kPrecFP64>)
-    if (mode == TRACE_GRID) return PT_FN(TRACE_GRID);
+    if (mode == TRACE_GRID || (mode == TRACE_GRID_TAIL && precision == kPrecFP64)) return PT_FN(TRACE_GRID);
+    if (mode == TRACE_GRID_TAIL)
+        return precision == kPrecFast ? (const void *)trace_kernel<TRACE_GRID_TAIL, false, kPrecFast>
+                                      : (const void *)trace_kernel<TRACE_GRID_TAIL, false, kPrecFilter>;
     if (mode == TRACE_ALIAS) return PT_FN(TRACE_ALIAS);
     if (mode == TRACE_ADJOINT) return PT_FN(TRACE_ADJOINT);
     return PT_FN(TRACE_BATCH);
