@@ -1570,17 +1570,17 @@ __device__ inline int64_t readlane_i64(int64_t v, int l)
 constexpr int kFewPaths = 4;
 
 // One path of a wave down to a few, whose next free flight stays inside its safety ball
-// (the tail of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An in-ball
-// event is an internal scattering whose new direction and next free path depend only on
-// the path's three draws for it (direction: 2, free path: 1), so the 64 lanes compute
-// those for the next 64 events at once.  The sequential part of each event (inside-box
-// and event-limit checks, the ball test, the flight, the nudge, the resampled free path)
-// then runs on a copy of the path broadcast to every lane: all lanes hold the same
-// values, take the same branches and stay active, so each lane-indexed read below reads
-// a lane that computed its draws.  The loop stops at the first flight that leaves the
-// ball; the path's lane then takes the copy back.  Every value and every operation on
-// the path's state is the one step_path would make: the path is unchanged, bit for bit,
-// and its Rng continues at the right draw.
+// (the tail of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An
+// in-ball event is an internal scattering whose new direction and next free path depend
+// only on the path's three draws for it (direction: 2, free path: 1), so lane i computes
+// those of event i of a batch of 64.  Everything runs in uniform control flow on a copy of
+// the path broadcast to every lane, so every lane-indexed read reads a lane that computed
+// its values.  The positions at which the events begin come from one sequential chain of
+// FP64 additions (the flight, then the nudge, as step_path rounds them); then each lane
+// applies step_path's checks (inside the box, below the event limit, flight inside the
+// ball) to its event, and the batch keeps the events before the first that fails.  The
+// path's lane takes the copy back.  Every value of the path's state is the one step_path
+// would make: the path is unchanged, bit for bit, and its Rng continues at the right draw.
 template <int kPrec, bool kAdj>
 __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, Path &ph,
                               AdjLane &al, int src)
@@ -1609,22 +1609,72 @@ __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
         double nd[3];
         scatter_dir<kPrec>(u0, u1, nd);
         const double nfp = free_path_of<kPrec>(mfp, u2);
-        int n = 0;
-        for (; n < 64; ++n) {
-            if (!inside_box(g, u.pos) || u.events >= event_limit || !in_ball(u)) break;
-            u.events++;
-            const double distance = u.fp;
-            if (kAdj && lane == src) {
-                al.nscat++;
-                al.flight(which, distance, u.dir[0]);
-                al.len += distance;
-            }
-            for (int k = 0; k < 3; ++k) u.pos[k] = u.pos[k] + distance * u.dir[k];
-            for (int k = 0; k < 3; ++k) u.dir[k] = readlane_f64(nd[k], n);
-            for (int k = 0; k < 3; ++k) u.pos[k] = u.pos[k] + PT_NUDGE * u.dir[k];
-            u.fp = u.fp - distance;                          // 0: resampled, RT:1783-1790
-            if (u.fp < 1.0e-10) u.fp = readlane_f64(nfp, n);
+        // lane i: the state in which event i of the batch begins.  Its free path and
+        // direction come from event i - 1's draws (lane 0: the path's own); its position
+        // from the exact chain p(i+1) = (p(i) + fp(i) dir(i)) + nudge nd(i), rounded in
+        // step_path's order, run once for the wave with the displacements broadcast.
+        double fi = __shfl_up(nfp, 1, 64), di3[3];
+        for (int k = 0; k < 3; ++k) di3[k] = __shfl_up(nd[k], 1, 64);
+        if (lane == 0) {
+            fi = u.fp;
+            for (int k = 0; k < 3; ++k) di3[k] = u.dir[k];
         }
+        double fly[3], nud[3], pi[3], p[3];
+        for (int k = 0; k < 3; ++k) {
+            fly[k] = fi * di3[k];
+            nud[k] = PT_NUDGE * nd[k];
+            pi[k] = p[k] = u.pos[k];
+        }
+        // in steps of 16 events: the chain, then step_path's checks of those events on
+        // their start states, all lanes at once; the batch runs events 0 .. n-1, and event
+        // n (if any) is left to step_path
+        Path e;
+        for (int k = 0; k < 3; ++k) {
+            e.dir[k] = di3[k];
+            e.bc[k] = u.bc[k];
+        }
+        e.fp = fi;
+        e.br = u.br;
+        int n = 64;
+        for (int s0 = 0; s0 < 64; s0 += 16) {
+            for (int i = s0; i < s0 + 16; ++i) {
+                for (int k = 0; k < 3; ++k)
+                    p[k] = (p[k] + readlane_f64(fly[k], i)) + readlane_f64(nud[k], i);
+                if (lane == i + 1)
+                    for (int k = 0; k < 3; ++k) pi[k] = p[k];
+            }
+            for (int k = 0; k < 3; ++k) e.pos[k] = pi[k];
+            const bool ok = inside_box(g, pi) && u.events + lane < event_limit && in_ball(e);
+            const unsigned long long okm = __ballot(ok) | ~(0xffffull << s0);
+            if (okm != ~0ull) {
+                n = (int)__builtin_ctzll(~okm);
+                break;
+            }
+        }
+        if (kAdj) {
+            for (int i = 0; i < n; ++i) {
+                const double l = readlane_f64(fi, i), dx = readlane_f64(di3[0], i);
+                if (lane == src) {
+                    al.nscat++;
+                    al.flight(which, l, dx);
+                    al.len += l;
+                }
+            }
+        }
+        if (n == 64) {
+            for (int k = 0; k < 3; ++k) {
+                u.pos[k] = p[k];
+                u.dir[k] = readlane_f64(nd[k], 63);
+            }
+            u.fp = readlane_f64(nfp, 63);                    // fp - distance = 0: resampled
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                u.pos[k] = readlane_f64(pi[k], n);
+                u.dir[k] = readlane_f64(di3[k], n);
+            }
+            u.fp = readlane_f64(fi, n);
+        }
+        u.events += n;
         d += 3u * (uint32_t)n;
         if (n < 64) {
             // an odd next draw is the second half of a block: the Rng keeps it as its spare
