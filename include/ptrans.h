@@ -219,8 +219,10 @@ int pt_ctx_set_event_limit(pt_ctx *ctx, int64_t limit);
  * pt_stats.fp64_ops and philox_blocks (for roofline accounting; slower). */
 int pt_ctx_set_op_counting(pt_ctx *ctx, int32_t on);
 /* Launch tuning of the persistent trace kernels: paths a wave takes from the work queue
- * at once (chunk), idle lanes that trigger a refill (refill_min, 1..64) and workgroups
- * per CU (0: the occupancy maximum).  Results do not depend on these. */
+ * at once (chunk; grid, alias and adjoint launches too small to give every wave eight
+ * chunks use total / (8 waves) instead), idle lanes that trigger a refill (refill_min,
+ * 1..64) and workgroups per CU (0: the occupancy maximum).  Results do not depend on
+ * these. */
 int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t blocks_per_cu);
 
 /* ---- geometry ------------------------------------------------------------------ */
