@@ -1277,15 +1277,17 @@ struct AdjLane {
         fdx = 0.0;
         fl = 0.0;
     }
-    __device__ void flight(int grain, double l, double dx)
+    __device__ void flight(int grain, double l, double dx) { flights(grain, l, l * dx); }
+    // several flights in one grain: sums of l and of l dx
+    __device__ void flights(int grain, double sl, double sldx)
     {
         if (!gacc) return;
         if (grain != fgrain) {
             flush();
             fgrain = grain;
         }
-        fdx += l * dx;
-        fl += l;
+        fdx += sldx;
+        fl += sl;
     }
 };
 
@@ -1651,14 +1653,19 @@ __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
                 break;
             }
         }
-        if (kAdj) {
-            for (int i = 0; i < n; ++i) {
-                const double l = readlane_f64(fi, i), dx = readlane_f64(di3[0], i);
-                if (lane == src) {
-                    al.nscat++;
-                    al.flight(which, l, dx);
-                    al.len += l;
-                }
+        if (kAdj && n > 0) {
+            // the batch's n scatterings and flights for the likelihood-ratio score and the
+            // grain tally, summed over the lanes (FP64 sums of a path's flights, whose
+            // order the batch sums do not keep anyway)
+            double sl = lane < n ? fi : 0.0, sx = lane < n ? fi * di3[0] : 0.0;
+            for (int m = 1; m < 64; m <<= 1) {
+                sl += __shfl_xor(sl, m, 64);
+                sx += __shfl_xor(sx, m, 64);
+            }
+            if (lane == src) {
+                al.nscat += n;
+                al.flights(which, sl, sx);
+                al.len += sl;
             }
         }
         if (n == 64) {
