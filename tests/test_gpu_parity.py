@@ -262,6 +262,33 @@ def test_safety_ball_diffusive_paths_bitexact(gpu_ctx):
         gpu_ctx.set_event_limit(100_000_000)
 
 
+def test_diffusive_grid_fp32_trace_statistically_equals_fp64(gpu_ctx):
+    """A grid trace with a 3 nm sampling mean free path in a 1000 x 100 x 100 nm wire runs
+    the kernel variant that serves the last paths of each wave with the whole wave (safety
+    balls form below 5 nm).  Paths of ~1e3 events decorrelate between the FP32 and FP64
+    traces, so their transmitted totals agree within 3 sigma of the difference of two
+    binomial counts; the bit-identical filter precision gives the same counts, events and
+    draws as FP64."""
+    model = gen.nanowire(length=1000.0, width=100.0, height=100.0, p_wall=0.0, nphonon=100,
+                         nangle=11)
+    geo = geometry.build_geometry(model)
+    om = np.linspace(1e12, 9e13, 6).reshape(2, 3)
+    gpu_ctx.set_seed(43)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    a, sa = gpu_ctx.trace_grid(g, m, 3.0, 11, 1000)
+    b, sb = gpu_ctx.trace_grid(g, m, 3.0, 11, 1000, precision=FAST)
+    c, sc = gpu_ctx.trace_grid(g, m, 3.0, 11, 1000, precision=FP32)
+    print("events", sa["events"], "transmitted fp64", a.sum(), "fp32", c.sum(), "ms", sa["kernel_ms"],
+          sb["kernel_ms"], sc["kernel_ms"], flush=True)
+    np.testing.assert_array_equal(a, b)
+    assert sa["events"] == sb["events"] and sa["draws"] == sb["draws"]
+    n, ta, tc = sa["paths"], int(a.sum()), int(c.sum())
+    assert ta > 100
+    sig = np.sqrt(n * (ta / n) * (1 - ta / n))
+    assert abs(tc - ta) <= 3 * np.sqrt(2) * sig
+
+
 def test_safety_ball_alias_inclusions_bitexact(gpu_ctx):
     """Alias mode on the C3 inclusions with mean free paths 1 - 40 nm: curved candidates
     bound the ball too; FP32-filter records equal the oracle's."""
