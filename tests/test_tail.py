@@ -68,3 +68,27 @@ def test_gpu_grain_tallies_diffusive_tail_equal_oracle(gpu_ctx):
     np.testing.assert_array_equal(gp, mp)
     np.testing.assert_array_equal(gtr, mt)
     np.testing.assert_allclose(gg, gt, rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_gpu_many_candidates_tail_scan_equals_oracle(gpu_ctx):
+    """80 spheres in a 1000 nm cube: a path in the matrix has 86 candidates, so the wave's
+    scan for a path left alone takes two passes of 64 lanes (the first strict minimum kept
+    across passes).  Per-mode tallies equal the oracle's."""
+    m_ = gen.inclusions(seed=7, L=1000.0, nsphere=80, ncyl=0, rs=(20.0, 40.0), kind="Gamma",
+                        value=1.0, p=0.5, nphonon=1, nangle=3)
+    geo = geometry.build_geometry(m_)
+    assert len(geo.grains[-1].surfaces) == 80
+    t = synthetic_table(2, 5, 11, mfp_min_nm=20.0, mfp_max_nm=400.0)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    prob, alias = O.build_alias(_weights(omega, t.vel, t.mfp_nm, t.weight))
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
+    mp, mt = O.trace_alias(gdesc, mdesc, t.mfp_nm, prob, alias, 29, 0, 20000)
+    gpu_ctx.set_seed(29)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    gp, gt, st = gpu_ctx.trace_alias(g, m, 20000, precision=FILTER)
+    print("events per path", st["events"] / st["paths"], "fallbacks", st["fallbacks"], flush=True)
+    np.testing.assert_array_equal(gp, mp)
+    np.testing.assert_array_equal(gt, mt)
