@@ -444,7 +444,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(paths shard by id range; see DESIGN.md)", file=sys.stderr)
+              "(paths would shard by id range, one range per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
