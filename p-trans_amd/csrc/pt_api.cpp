@@ -744,6 +744,9 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
     P.nbatch = nbatch;
     P.baseline = baseline;
     P.ngt = grain_tally ? G->dg.ngrain_total : 0;
+    // grain tallies without scores or batch sums: the leaner kernel without the likelihood
+    // ratio bookkeeping
+    const int amode = grain_tally && ngroup == 0 && !batch_sums ? TRACE_GRAINS : TRACE_ADJOINT;
     const size_t nsum = (size_t)nbatch * adj_stride(P);
     const size_t ntal = 2 * (size_t)P.ngt;
     const size_t lds = adj_lds_bytes(P, c->threads);
@@ -772,7 +775,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         P.surf_group = d_grp;
         P.adj = d_sum;
         P.grain_tally = d_gt;
-        if ((e = launch_blocks(c, TRACE_ADJOINT, precision, lds, &blocks))) break;
+        if ((e = launch_blocks(c, amode, precision, lds, &blocks))) break;
         P.chunk = wave_chunk(c, P.total, blocks);
         if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
@@ -780,7 +783,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         if ((e = hipMemsetAsync(d_sum, 0, nsum * sizeof(double), c->stream))) break;
         if (ntal && (e = hipMemsetAsync(d_gt, 0, ntal * sizeof(double), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_ADJOINT, blocks, c->threads, c->count_ops, lds, c->stream))) break;
+        if ((e = launch_trace(P, amode, blocks, c->threads, c->count_ops, lds, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipMemcpyAsync(h_sum.data(), d_sum, nsum * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;

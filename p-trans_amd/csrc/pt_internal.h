@@ -72,7 +72,9 @@ enum : uint8_t {
 // TRACE_GRID_TAIL: the grid trace of a diffusive sampling mean free path (one that gets
 // safety balls, below kBallMfpFrac of the smallest box side), with the few-path service at
 // the end of each wave.  The ballistic grid trace keeps the leaner TRACE_GRID kernel.
-enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3, TRACE_GRID_TAIL = 4 };
+// TRACE_GRAINS: the alias-mode trace plus per-grain flux tallies (no likelihood ratio).
+enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3, TRACE_GRID_TAIL = 4,
+       TRACE_GRAINS = 5 };
 
 constexpr bool is_grid_mode(int mode) { return mode == TRACE_GRID || mode == TRACE_GRID_TAIL; }
 

@@ -1332,9 +1332,11 @@ __device__ inline bool inside_box(const DevGeom &g, const double *p)
 }
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
+// kAdj: 0 plain trace, 1 flights for the grain flux tallies, 2 also the likelihood-ratio
+// bookkeeping (scattering count, path length, specularity scores).
 // pre_best != -2: the wave has already run this event's scan (lone_scan: winner pre_best,
 // -1 for none, at distance pre_dmin), and sets up a new ball itself (br = NaN until then).
-template <int kPrec, bool kAdj, bool kBatch = false>
+template <int kPrec, int kAdj, bool kBatch = false>
 __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limit, Rng &rng,
                           Path &ph, AdjLane &al, const DevBatch *bt = nullptr, int *stk = nullptr,
                           int pre_best = -2, double pre_dmin = 0.0)
@@ -1389,10 +1391,8 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
         distance = ph.fp;
         scattered = true;
-        if (kAdj) {
-            al.nscat++;
-            al.flight(ph.which, distance, ph.dir[0]);
-        }
+        if (kAdj >= 2) al.nscat++;
+        if (kAdj) al.flight(ph.which, distance, ph.dir[0]);
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
         const double u_ct = rng.next();
@@ -1423,7 +1423,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         } else {
             const DSurf s = g.surf[ph.cur];
             const double spec = rng.next();                                 // RT:2230
-            if (kAdj) {
+            if (kAdj >= 2) {
                 const int gi = al.group[ph.cur];
                 if (gi >= 0) al.score[gi * al.stride] += spec <= s.p ? 1.0 / s.p : -1.0 / (1.0 - s.p);
             }
@@ -1509,7 +1509,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             }
         }
     }
-    if (kAdj) al.len += distance;
+    if (kAdj >= 2) al.len += distance;
     // nudge, RT:1775-1780
     for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + PT_NUDGE * ph.dir[k];
     oc.flops += 6;
@@ -1583,7 +1583,7 @@ constexpr int kFewPaths = 4;
 // ball) to its event, and the batch keeps the events before the first that fails.  The
 // path's lane takes the copy back.  Every value of the path's state is the one step_path
 // would make: the path is unchanged, bit for bit, and its Rng continues at the right draw.
-template <int kPrec, bool kAdj>
+template <int kPrec, int kAdj>
 __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, Path &ph,
                               AdjLane &al, int src)
 {
@@ -1663,9 +1663,11 @@ __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
                 sx += __shfl_xor(sx, m, 64);
             }
             if (lane == src) {
-                al.nscat += n;
                 al.flights(which, sl, sx);
-                al.len += sl;
+                if (kAdj >= 2) {
+                    al.nscat += n;
+                    al.len += sl;
+                }
             }
         }
         if (n == 64) {
@@ -1809,7 +1811,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 trace_kernel(TraceParams P)
 {
     constexpr bool kBatch = kMode == TRACE_BATCH;
-    constexpr bool kAdj = kMode == TRACE_ADJOINT || kBatch;
+    constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
     extern __shared__ double adj_lds[];
     // per-lane statistics live in LDS, not in registers held through the whole trace:
     // paths, events, draws, lost, start mismatch, event limit, FP64 retests, fallbacks
@@ -1953,6 +1955,8 @@ trace_kernel(TraceParams P)
                     al.score[0] = 0.0;
                     const unsigned n_s = (unsigned)(P.bt.path_off[sid + 1] - P.bt.path_off[sid]);
                     if (atomicAdd(P.bt.done + sid, 1u) + 1u == n_s) P.bt.finish[sid] = wall_clock64();
+                } else if (kAdj == 1) {
+                    al.flush();
                 } else if (kAdj) {
                     const double lam = P.m.mfp[mode];
                     const int ib = mode % P.m.nb;
@@ -2122,6 +2126,7 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
     else if (mode == TRACE_GRID_TAIL) PT_LAUNCH_C(TRACE_GRID, kPrecFP64);
     else if (mode == TRACE_ALIAS) PT_LAUNCH_R(TRACE_ALIAS);
     else if (mode == TRACE_ADJOINT) PT_LAUNCH_R(TRACE_ADJOINT);
+    else if (mode == TRACE_GRAINS) PT_LAUNCH_R(TRACE_GRAINS);
     else PT_LAUNCH_R(TRACE_BATCH);
 #undef PT_LAUNCH_R
 #undef PT_LAUNCH_C
@@ -2141,6 +2146,7 @@ hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int
         return precision == kPrecFast ? PT_OCC(TRACE_GRID_TAIL, kPrecFast) : PT_OCC(TRACE_GRID_TAIL, kPrecFilter);
     if (mode == TRACE_ALIAS) return PT_OCC_R(TRACE_ALIAS);
     if (mode == TRACE_ADJOINT) return PT_OCC_R(TRACE_ADJOINT);
+    if (mode == TRACE_GRAINS) return PT_OCC_R(TRACE_GRAINS);
     return PT_OCC_R(TRACE_BATCH);
 #undef PT_OCC_R
 #undef PT_OCC
@@ -2157,6 +2163,7 @@ const void *trace_kernel_fn(int mode, int precision)
                                       : (const void *)trace_kernel<TRACE_GRID_TAIL, false, kPrecFilter>;
     if (mode == TRACE_ALIAS) return PT_FN(TRACE_ALIAS);
     if (mode == TRACE_ADJOINT) return PT_FN(TRACE_ADJOINT);
+    if (mode == TRACE_GRAINS) return PT_FN(TRACE_GRAINS);
     return PT_FN(TRACE_BATCH);
 #undef PT_FN
 }
