@@ -744,9 +744,10 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
     P.nbatch = nbatch;
     P.baseline = baseline;
     P.ngt = grain_tally ? G->dg.ngrain_total : 0;
-    // grain tallies without scores or batch sums: the leaner kernel without the likelihood
-    // ratio bookkeeping
-    const int amode = grain_tally && ngroup == 0 && !batch_sums ? TRACE_GRAINS : TRACE_ADJOINT;
+    // grain tallies (no scores, no batch sums) or the likelihood ratio: two kernel modes
+    if (grain_tally && (ngroup > 0 || batch_sums))
+        return fail(PT_E_UNSUPPORTED, "%s: grain tallies and adjoint scores in one launch", fn);
+    const int amode = grain_tally ? TRACE_GRAINS : TRACE_ADJOINT;
     const size_t nsum = (size_t)nbatch * adj_stride(P);
     const size_t ntal = 2 * (size_t)P.ngt;
     const size_t lds = adj_lds_bytes(P, c->threads);

@@ -1332,8 +1332,8 @@ __device__ inline bool inside_box(const DevGeom &g, const double *p)
 }
 
 // One iteration of the event loop (RT:1693-1824).  Returns true when the path ended.
-// kAdj: 0 plain trace, 1 flights for the grain flux tallies, 2 also the likelihood-ratio
-// bookkeeping (scattering count, path length, specularity scores).
+// kAdj: bit 1 the flights of the grain flux tallies, bit 2 the likelihood-ratio
+// bookkeeping (scattering count, path length, specularity scores); 0 the plain trace.
 // pre_best != -2: the wave has already run this event's scan (lone_scan: winner pre_best,
 // -1 for none, at distance pre_dmin), and sets up a new ball itself (br = NaN until then).
 template <int kPrec, int kAdj, bool kBatch = false>
@@ -1391,8 +1391,8 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         // internal scattering, RT:1735-1742; createRandomDirectionAllSphere RT:2192-2204
         distance = ph.fp;
         scattered = true;
-        if (kAdj >= 2) al.nscat++;
-        if (kAdj) al.flight(ph.which, distance, ph.dir[0]);
+        if (kAdj & 2) al.nscat++;
+        if (kAdj & 1) al.flight(ph.which, distance, ph.dir[0]);
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
         const double u_ct = rng.next();
@@ -1401,7 +1401,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         ph.cur = -1;
         oc.flops += 8 + kFlopsSincos;    // + 6 above
     } else {
-        if (kAdj) al.flight(ph.which, distance, ph.dir[0]);
+        if (kAdj & 1) al.flight(ph.which, distance, ph.dir[0]);
         for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + distance * ph.dir[k];
         oc.flops += 6;
         if (ph.cur == 0) {
@@ -1423,7 +1423,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         } else {
             const DSurf s = g.surf[ph.cur];
             const double spec = rng.next();                                 // RT:2230
-            if (kAdj >= 2) {
+            if (kAdj & 2) {
                 const int gi = al.group[ph.cur];
                 if (gi >= 0) al.score[gi * al.stride] += spec <= s.p ? 1.0 / s.p : -1.0 / (1.0 - s.p);
             }
@@ -1509,7 +1509,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
             }
         }
     }
-    if (kAdj >= 2) al.len += distance;
+    if (kAdj & 2) al.len += distance;
     // nudge, RT:1775-1780
     for (int k = 0; k < 3; ++k) ph.pos[k] = ph.pos[k] + PT_NUDGE * ph.dir[k];
     oc.flops += 6;
@@ -1663,8 +1663,8 @@ __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
                 sx += __shfl_xor(sx, m, 64);
             }
             if (lane == src) {
-                al.flights(which, sl, sx);
-                if (kAdj >= 2) {
+                if (kAdj & 1) al.flights(which, sl, sx);
+                if (kAdj & 2) {
                     al.nscat += n;
                     al.len += sl;
                 }
@@ -1826,7 +1826,7 @@ trace_kernel(TraceParams P)
     al.len = 0.0;
     double *adj_acc = adj_lds + (kAdj ? P.ngroup * blockDim.x : 0);
     const int adj_k = kAdj ? adj_stride(P) : 0;
-    al.gacc = (kAdj && P.grain_tally) ? adj_acc + P.nbatch * adj_k : nullptr;
+    al.gacc = ((kAdj & 1) && P.grain_tally) ? adj_acc + P.nbatch * adj_k : nullptr;
     al.fgrain = -1;
     al.fdx = al.fl = al.fw = 0.0;
     // batch: the lane's BVH stack (per-structure sums go straight to global atomics: a
@@ -1873,7 +1873,7 @@ trace_kernel(TraceParams P)
                     al.len = 0.0;
                     if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
                         active = true;
-                        if (kAdj) al.fw = 1.0 / P.m.mfp[mode];
+                        if (kAdj & 1) al.fw = 1.0 / P.m.mfp[mode];
                     } else {
                         if (is_grid_mode(kMode)) P.outcome[pid] = OUT_MISMATCH;
                         st[0] += 1ull;
@@ -1973,7 +1973,6 @@ trace_kernel(TraceParams P)
                         al.score[gi * al.stride] = 0.0;
                     }
                     atomicAdd(acc + 3 + P.ngroup + ib, w * ((double)al.nscat / sb - al.len / lam));
-                    al.flush();
                 }
                 st[0] += 1ull;
                 st[256 * 1] += (unsigned long long)ph.events;
