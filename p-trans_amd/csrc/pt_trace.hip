@@ -32,6 +32,9 @@ struct OpCount {
     unsigned long long philox;     // Philox4x32-10 blocks (integer VALU work)
     unsigned long long nodes;      // BVH nodes visited (scan and ball)
     unsigned long long tests;      // BVH leaf surfaces tested in FP64
+    // the lane's event statistics in LDS (counter k at st[256 * k]): FP64 retests of the
+    // filter at k = 6, full-scan fallbacks at k = 7
+    unsigned long long *st;
 };
 
 constexpr unsigned kFlopsLog = 33;     // pt_log: 4 + Horner 20 + 5 + 4
@@ -399,8 +402,6 @@ struct Path {
     int flags;
     int start_grain;
     int64_t events;
-    unsigned retests;       // fast path: candidates evaluated in FP64 by the filter
-    unsigned fallbacks;     // fast path: events decided by the full FP64 scan
     float bc[3];            // safety ball: centre and radius (br <= 0: none); no surface
     float br;               //   the scan can hit lies within it
     int sid;                // batch structure of the path
@@ -643,7 +644,7 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                 pk.add(w, lo, hi, false);
             } else if (r == F_UNSURE) {
                 double l;
-                ph.retests++;
+                ++oc.st[256 * 6];
                 if (surface_distance(g, g.surf[w], rs, nd, &l, oc)) pk.add(w, l, l, true);
             }
         }
@@ -680,7 +681,7 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                                 pk.add(idx | kPlaneBit, l, l, true);
                         } else {
                             double l;
-                            ph.retests++;
+                            ++oc.st[256 * 6];
                             if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
                         }
                     }
@@ -693,7 +694,7 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
     while (__any(q0 >= 0)) {
         if (q0 >= 0) {
             double l;
-            ph.retests++;
+            ++oc.st[256 * 6];
             if (surface_distance(g, g.surf[q0], rs, nd, &l, oc)) pk.add(q0, l, l, true);
             q0 = q1;
             q1 = -1;
@@ -1230,8 +1231,6 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     ph.cur = 0;                                                             // igrain=0, isurface=1
     ph.flags = 0;
     ph.events = 0;
-    ph.retests = 0;
-    ph.fallbacks = 0;
     ph.br = -1.0f;
     int which = 0;
     if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
@@ -1376,7 +1375,7 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         if (!decided) {
             dmin = 1.0e10;
             best = ph.cur;
-            ph.fallbacks++;
+            ++oc.st[256 * 7];
         }
     }
     if (!decided) {
@@ -1839,7 +1838,8 @@ trace_kernel(TraceParams P)
         if (kBatch && blockIdx.x == 0 && threadIdx.x == 0) *P.bt.start = wall_clock64();
         __syncthreads();
     }
-    OpCount oc = {0ull, 0ull, 0ull, 0ull};
+    unsigned long long *st = &lane_stats[0][threadIdx.x];   // counter k at st[256 * k]
+    OpCount oc = {0ull, 0ull, 0ull, 0ull, st};
     Rng rng;
     rng.oc = &oc;
     rng.seed = P.seed;
@@ -1847,7 +1847,6 @@ trace_kernel(TraceParams P)
     bool active = false;
     int64_t pid = 0;
     int mode = 0;
-    unsigned long long *st = &lane_stats[0][threadIdx.x];   // counter k at st[256 * k]
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
 
@@ -1979,8 +1978,6 @@ trace_kernel(TraceParams P)
                 st[256 * 2] += rng.draw;
                 if (ph.flags & PT_PATH_LOST) st[256 * 3] += 1ull;
                 if (ph.flags & PT_PATH_EVENT_LIMIT) st[256 * 5] += 1ull;
-                st[256 * 6] += ph.retests;
-                st[256 * 7] += ph.fallbacks;
                 active = false;
             }
         }
@@ -2067,7 +2064,8 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    OpCount oc = {0ull, 0ull, 0ull, 0ull};
+    __shared__ unsigned long long unused_stats[8][256];      // event statistics, not reported
+    OpCount oc = {0ull, 0ull, 0ull, 0ull, &unused_stats[0][threadIdx.x]};
     Rng rng;
     rng.oc = &oc;
     rng.seed = P.seed;
