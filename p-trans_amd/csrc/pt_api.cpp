@@ -66,9 +66,12 @@ struct pt_ctx {
     int chunk = 256;
     int refill_min = 16;
     int64_t event_limit = 100000000;
-    // grow-only scratch of the grid trace
+    // grow-only scratch of the grid trace: outcome bytes (padded by 16 for the reduction's
+    // 16-byte loads), and the angle tables and cell counts of a call
     uint8_t *outcome = nullptr;
     size_t outcome_bytes = 0;
+    void *grid_buf = nullptr;
+    size_t grid_buf_bytes = 0;
     unsigned long long *counters = nullptr;   // queue + 6 stats + skipped
     double *scale = nullptr;                   // per-branch scale of the current call
     int scale_n = 0;
@@ -150,6 +153,7 @@ int pt_ctx_destroy(pt_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->outcome) (void)hipFree(c->outcome);
+    if (c->grid_buf) (void)hipFree(c->grid_buf);
     if (c->counters) (void)hipFree(c->counters);
     if (c->scale) (void)hipFree(c->scale);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -836,18 +840,24 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     PT_HIP(hipSetDevice(c->device));
     std::vector<double> ct, st;
     angle_tables(nangle, vangle, ct, st);
-    if ((size_t)total > c->outcome_bytes) {
+    if ((size_t)total + 16 > c->outcome_bytes) {
         if (c->outcome) PT_HIP(hipFree(c->outcome));
         c->outcome = nullptr;
         c->outcome_bytes = 0;
-        PT_HIP(hipMalloc((void **)&c->outcome, (size_t)total));
-        c->outcome_bytes = (size_t)total;
+        PT_HIP(hipMalloc((void **)&c->outcome, (size_t)total + 16));
+        c->outcome_bytes = (size_t)total + 16;
     }
-    double *d_ang = nullptr;
-    long long *d_counts = nullptr;
-    PT_HIP(hipMalloc((void **)&d_ang, 2 * nangle * sizeof(double)));
-    hipError_t e = hipMalloc((void **)&d_counts, ncount * sizeof(long long));
-    if (e != hipSuccess) { (void)hipFree(d_ang); return fail(PT_E_HIP, "hipMalloc: %s", hipGetErrorString(e)); }
+    const size_t gbytes = 2 * (size_t)nangle * sizeof(double) + ncount * sizeof(long long);
+    if (gbytes > c->grid_buf_bytes) {
+        if (c->grid_buf) PT_HIP(hipFree(c->grid_buf));
+        c->grid_buf = nullptr;
+        c->grid_buf_bytes = 0;
+        PT_HIP(hipMalloc(&c->grid_buf, gbytes));
+        c->grid_buf_bytes = gbytes;
+    }
+    double *d_ang = (double *)c->grid_buf;
+    long long *d_counts = (long long *)(d_ang + 2 * nangle);
+    hipError_t e = hipSuccess;
     std::vector<double> ang(ct);
     ang.insert(ang.end(), st.begin(), st.end());
     TraceParams P = base_params(c, G, M);
@@ -884,8 +894,6 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         if ((e = hipEventElapsedTime(&ms, c->ev0, c->ev1))) break;
         e = hipEventElapsedTime(&ms_red, c->ev1, c->ev2);
     } while (0);
-    (void)hipFree(d_ang);
-    (void)hipFree(d_counts);
     if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_grid: %s", hipGetErrorString(e));
     fill_stats(stats, host_ctr, ms, ms_red);
     if ((int64_t)host_ctr[1] != total)

@@ -2008,45 +2008,56 @@ trace_kernel(TraceParams P)
     }
 }
 
-// Per-cell counts from the outcome bytes: transmitted paths with a particle index
-// below the first start-grain failure of the cell (the reference EXITs its particle
-// loop there, RT:1681).  One block per cell.
+// Per (b,k,θ) cell, one wave: the cell's np outcome bytes in steps of 1 KB, 16 bytes per
+// lane.  The count is the number of transmitted paths before the cell's first start-grain
+// mismatch (the reference EXITs the particle loop there, RT:1673-1682); the paths after
+// it are added to `skipped`.  The outcome buffer is padded by 16 bytes, so a lane's
+// 16-byte block that starts inside it stays inside it.
 __global__ void __launch_bounds__(256) reduce_grid_kernel(const uint8_t *__restrict__ outcome,
-                                                          int64_t np, int64_t nb, int64_t nk,
-                                                          int32_t nangle, long long *counts,
+                                                          int64_t ncell, int64_t np, int64_t nb,
+                                                          int64_t nk, int32_t nangle,
+                                                          long long *counts,
                                                           unsigned long long *skipped)
 {
-    __shared__ long long s_first[4];
-    __shared__ long long s_cnt[4];
-    const int64_t cell = blockIdx.x;
-    const uint8_t *o = outcome + cell * np;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    long long first = np;
-    for (int64_t p = threadIdx.x; p < np; p += blockDim.x)
-        if ((o[p] & OUT_MISMATCH) && p < first) first = p;
-    for (int off = 32; off > 0; off >>= 1) {
-        const long long t = __shfl_xor(first, off, 64);
-        first = t < first ? t : first;
-    }
-    if (lane == 0) s_first[wid] = first;
-    __syncthreads();
-    first = s_first[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) first = s_first[w] < first ? s_first[w] : first;
+    const int lane = threadIdx.x & 63;
+    const int64_t cell = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (cell >= ncell) return;                       // whole waves
+    const int64_t base = cell * np, end = base + np;
     long long cnt = 0;
-    for (int64_t p = threadIdx.x; p < first; p += blockDim.x) cnt += (o[p] & OUT_TRANSMITTED) ? 1 : 0;
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    __syncthreads();
-    if (lane == 0) s_cnt[wid] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long tot = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += s_cnt[w];
+    int64_t first = end;                             // first mismatch (absolute position)
+    for (int64_t c = base & ~(int64_t)15; c < end; c += 1024) {
+        const int64_t q = c + 16 * lane;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (q < end) v = *(const uint4 *)(outcome + q);
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+        int64_t lf = end;
+        for (int j = 15; j >= 0; --j) {
+            const unsigned byte = (w[j >> 2] >> (8 * (j & 3))) & 0xffu;
+            if (q + j >= base && q + j < end && (byte & OUT_MISMATCH)) lf = q + j;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const int64_t t = __shfl_xor(lf, off, 64);
+            lf = t < lf ? t : lf;
+        }
+        long long lc = 0;
+        for (int j = 0; j < 16; ++j) {
+            const unsigned byte = (w[j >> 2] >> (8 * (j & 3))) & 0xffu;
+            lc += (q + j >= base && q + j < lf && (byte & OUT_TRANSMITTED)) ? 1 : 0;
+        }
+        for (int off = 32; off > 0; off >>= 1) lc += __shfl_xor(lc, off, 64);
+        cnt += lc;
+        if (lf < end) {
+            first = lf;
+            break;
+        }
+    }
+    if (lane == 0) {
         // cell = ((ik-1)*nb + (ib-1))*(nangle-1) + (ia-1)  ->  transmit(ib,ik,ia)
         const int64_t na = nangle - 1;
         const int64_t ia = cell % na, rest = cell / na;
         const int64_t ib = rest % nb, ik = rest / nb;
-        counts[ib + nb * (ik + nk * ia)] = tot;
-        if (first < np) atomicAdd(skipped, (unsigned long long)(np - first - 1));
+        counts[ib + nb * (ik + nk * ia)] = cnt;
+        if (first < end) atomicAdd(skipped, (unsigned long long)(end - first - 1));
     }
 }
 
@@ -2188,8 +2199,8 @@ hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
                               unsigned long long *skipped, hipStream_t s)
 {
     if (ncell == 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_grid_kernel, dim3((unsigned)ncell), dim3(256), 0, s, outcome, np, nb,
-                       nk, nangle, counts, skipped);
+    hipLaunchKernelGGL(reduce_grid_kernel, dim3((unsigned)((ncell + 3) / 4)), dim3(256), 0, s,
+                       outcome, ncell, np, nb, nk, nangle, counts, skipped);
     return hipGetLastError();
 }
 
