@@ -72,6 +72,8 @@ struct pt_ctx {
     size_t outcome_bytes = 0;
     void *grid_buf = nullptr;
     size_t grid_buf_bytes = 0;
+    void *host_stage = nullptr;                // pinned host copy of the cell counts
+    size_t host_stage_bytes = 0;
     unsigned long long *counters = nullptr;   // queue + 6 stats + skipped
     double *scale = nullptr;                   // per-branch scale of the current call
     int scale_n = 0;
@@ -154,6 +156,7 @@ int pt_ctx_destroy(pt_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->outcome) (void)hipFree(c->outcome);
     if (c->grid_buf) (void)hipFree(c->grid_buf);
+    if (c->host_stage) (void)hipHostFree(c->host_stage);
     if (c->counters) (void)hipFree(c->counters);
     if (c->scale) (void)hipFree(c->scale);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -857,6 +860,13 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     }
     double *d_ang = (double *)c->grid_buf;
     long long *d_counts = (long long *)(d_ang + 2 * nangle);
+    if (ncount * sizeof(long long) > c->host_stage_bytes) {
+        if (c->host_stage) PT_HIP(hipHostFree(c->host_stage));
+        c->host_stage = nullptr;
+        c->host_stage_bytes = 0;
+        PT_HIP(hipHostMalloc(&c->host_stage, ncount * sizeof(long long), hipHostMallocDefault));
+        c->host_stage_bytes = ncount * sizeof(long long);
+    }
     hipError_t e = hipSuccess;
     std::vector<double> ang(ct);
     ang.insert(ang.end(), st.begin(), st.end());
@@ -888,13 +898,14 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
         if ((e = hipEventRecord(c->ev2, c->stream))) break;
-        if ((e = hipMemcpyAsync(counts, d_counts, ncount * sizeof(long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(c->host_stage, d_counts, ncount * sizeof(long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipStreamSynchronize(c->stream))) break;
         if ((e = hipEventElapsedTime(&ms, c->ev0, c->ev1))) break;
         e = hipEventElapsedTime(&ms_red, c->ev1, c->ev2);
     } while (0);
     if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_grid: %s", hipGetErrorString(e));
+    memcpy(counts, c->host_stage, ncount * sizeof(long long));
     fill_stats(stats, host_ctr, ms, ms_red);
     if ((int64_t)host_ctr[1] != total)
         return fail(PT_E_HIP, "pt_trace_grid: %lld of %lld paths completed",
