@@ -58,6 +58,31 @@ def test_grid_counts_vs_reference(gpu_ctx, name):
     assert stats["lost"] == 0 and stats["start_mismatch"] == 0
 
 
+def test_grid_start_mismatch_exits_cell_like_oracle(gpu_ctx):
+    """A Voronoi generator moved 10% toward another: some start points then pick an
+    incident grain that does not contain them, and the reference EXITs the cell's particle
+    loop at the first such path (RT:1673-1682).  The GPU traces every path and its
+    reduction counts only the transmitted paths before each cell's first mismatch: counts
+    and skipped paths equal the oracle's, in both bit-identical precisions."""
+    model = gen.voronoi_cube(seed=4, kind="Value", value=0.5, p=0.3, nphonon=300, nangle=5)
+    geo = geometry.build_geometry(model)
+    g2, g5 = np.array(geo.grains[2].generatrix), np.array(geo.grains[5].generatrix)
+    geo.grains[5].generatrix = tuple(g5 + 0.1 * (g2 - g5))
+    om = np.linspace(1e12, 9e13, 6).reshape(2, 3)
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(om)
+    oc, ost = O.trace_grid(gdesc, mdesc, 200.0, 5, 300, 17, math=O.MATH_OWN)
+    assert ost[4] > 0 and ost[5] > 0                 # mismatching cells, skipped paths
+    gpu_ctx.set_seed(17)
+    g = gpu_ctx.geometry(geo)
+    m = gpu_ctx.modes(omega=om)
+    for prec in (0, FAST):
+        counts, st = gpu_ctx.trace_grid(g, m, 200.0, 5, 300, precision=prec)
+        print("precision", prec, "transmitted", counts.sum(), "skipped", st["skipped"], flush=True)
+        np.testing.assert_array_equal(counts, oc)
+        assert st["skipped"] == ost[5]
+
+
 def test_grid_larger_vs_oracle_and_stats(gpu_ctx):
     """A few hundred thousand paths of the film-with-holes input with internal
     scattering: counts and the total event and draw counts equal the oracle's."""
