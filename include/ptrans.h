@@ -164,7 +164,8 @@ typedef struct pt_stats {
     int64_t fallbacks;       /* PT_PREC_FP32_RETEST: events decided by the FP64 scan  */
     int64_t bvh_nodes;       /* batch trace: BVH nodes visited (op counting on only)  */
     int64_t bvh_tests;       /* batch trace: leaf surfaces tested (op counting on)    */
-    int64_t event_limited;   /* paths stopped by pt_ctx_set_event_limit's limit       */
+    int64_t event_limited;   /* paths stopped by pt_ctx_set_event_limit's limit,
+                                counted as not transmitted                           */
 } pt_stats;
 
 /* Per-path record of pt_replay(). */
@@ -213,7 +214,11 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out);
 int pt_ctx_destroy(pt_ctx *ctx);
 int pt_ctx_set_seed(pt_ctx *ctx, uint64_t seed);
 /* Safety stop: a path ends (flag PT_PATH_EVENT_LIMIT) after this many events.  The
- * reference loops without bound (RT:1693); default 1e8. */
+ * reference loops without bound (RT:1693); default 1e8, at most 715827881 (a path's draw
+ * counter is 32 bits; larger limits return PT_E_INVALID).  A stopped path has not ended
+ * on either side: every tally counts it as traced and NOT transmitted, and
+ * pt_stats.event_limited reports how many there were -- a caller must check it (a
+ * nonzero count biases the transmission low). */
 int pt_ctx_set_event_limit(pt_ctx *ctx, int64_t limit);
 /* on != 0: later trace calls run the instrumented kernel that also fills
  * pt_stats.fp64_ops and philox_blocks (for roofline accounting; slower). */

@@ -49,6 +49,7 @@ def lib():
                                            dbl, i32, vp]
         L.oracle_trace_alias_grains.argtypes = [vp, vp, vp, vp, vp, vp, u64, i32, i64, i64, vp,
                                                 vp, vp]
+        L.oracle_set_event_limit.argtypes = [i64]
         L.oracle_philox_draw.restype = dbl
         L.oracle_philox_draw.argtypes = [u64, u64, u64]
         L.oracle_pt_log.restype = dbl
@@ -63,6 +64,11 @@ def lib():
         L.oracle_surface_rotation.argtypes = [i32, vp, vp, vp, vp]
         _lib = L
     return _lib
+
+
+def set_event_limit(limit: int) -> None:
+    """Events after which the oracle stops a path (PT_PATH_EVENT_LIMIT); default 1e8."""
+    lib().oracle_set_event_limit(int(limit))
 
 
 def _p(a):

@@ -41,7 +41,20 @@ static const double O_PI = 3.14159265358979323846;
 static const double O_SMALL_VAL = 1.0e-5;            /* RT:69 */
 #define O_NUDGE ((double)1.0e-7f)                    /* REAL(4) literal, RT:1779 */
 #define O_SPHERE_EPS ((double)1e-10f)                /* REAL(4) literal, RT:3001 */
-#define O_EVENT_LIMIT 100000000LL                    /* safety stop, not in the reference */
+/* Safety stop, not in the reference (whose event loop has no bound, RT:1693): a path
+ * still inside after this many events ends with PT_PATH_EVENT_LIMIT and is tallied as
+ * not transmitted.  Settable for tests of that rule (oracle_set_event_limit). */
+static int64_t o_event_limit = 100000000LL;
+
+void oracle_set_event_limit(int64_t limit) { o_event_limit = limit; }
+
+/* StartSide /= EndSide (RT:1833-1835), StartSide = 1: a path counts as transmitted unless
+ * it ended on the x- side -- including a path that left the box (the reference's lost
+ * particle) -- except a path stopped by the safety limit, which has not ended at all. */
+static int o_transmitted(const pt_path_record *r)
+{
+    return r->end_side != 1 && !(r->flags & PT_PATH_EVENT_LIMIT);
+}
 
 typedef struct {
     const pt_geom_desc *g;
@@ -501,7 +514,7 @@ static int o_trace_particle(ostate *s, double cos_t, double sin_t, int lambert,
             flags |= PT_PATH_LOST;
             break;
         }
-        if (events >= O_EVENT_LIMIT) { flags |= PT_PATH_EVENT_LIMIT; break; }
+        if (events >= o_event_limit) { flags |= PT_PATH_EVENT_LIMIT; break; }
         events++;
 
         /* findCollisionSurface, RT:1989-2096: grains 0 then whichV, first strict minimum */
@@ -720,7 +733,7 @@ int oracle_trace_grid(const pt_geom_desc *g, const pt_modes_desc *m, double mfp,
                 stats[0]++; stats[1] += r.events; stats[2] += r.draws;
                 if (r.flags & PT_PATH_LOST) stats[3]++;
             }
-            if (r.end_side != 1) trans++;     /* StartSide /= EndSide, RT:1833-1835 */
+            if (o_transmitted(&r)) trans++;     /* StartSide /= EndSide, RT:1833-1835 */
         }
         counts[(ib - 1) + (int64_t)m->nbranch * ((ik - 1) + (int64_t)m->nkpoint * (ia - 1))] = trans;
     }
@@ -874,7 +887,7 @@ int oracle_trace_alias(const pt_geom_desc *g, const pt_modes_desc *m, const doub
         o_trace_particle(&s, 0.0, 0.0, 1, &r);
         if (!(r.flags & PT_PATH_START_MISMATCH)) {
             if (mode_paths) mode_paths[mode]++;
-            if (mode_trans && r.end_side != 1) mode_trans[mode]++;
+            if (mode_trans && o_transmitted(&r)) mode_trans[mode]++;
         }
         if (records) records[i] = r;
     }
@@ -927,7 +940,7 @@ int oracle_adjoint_alias(const pt_geom_desc *g, const pt_modes_desc *m, const do
         memset(&r, 0, sizeof r);
         o_trace_particle(&s, 0.0, 0.0, 1, &r);
         if (r.flags & PT_PATH_START_MISMATCH) continue;
-        x = r.end_side != 1 ? 1.0 / lam : 0.0;
+        x = o_transmitted(&r) ? 1.0 / lam : 0.0;
         w = x - baseline;
         acc = sums + (pid % nbatch) * K;
         acc[0] += 1.0;
@@ -971,7 +984,7 @@ int oracle_trace_alias_grains(const pt_geom_desc *g, const pt_modes_desc *m,
         o_trace_particle(&s, 0.0, 0.0, 1, &r);
         if (!(r.flags & PT_PATH_START_MISMATCH)) {
             mode_paths[mode]++;
-            if (r.end_side != 1) mode_trans[mode]++;
+            if (o_transmitted(&r)) mode_trans[mode]++;
         }
     }
     return 0;

@@ -177,6 +177,10 @@ int pt_ctx_set_seed(pt_ctx *c, uint64_t seed)
 int pt_ctx_set_event_limit(pt_ctx *c, int64_t limit)
 {
     if (!c || limit < 1) return fail(PT_E_INVALID, "pt_ctx_set_event_limit: bad argument");
+    if (limit > kMaxEventLimit)
+        return fail(PT_E_INVALID, "pt_ctx_set_event_limit: %lld events exceed the maximum %lld "
+                                  "(a path's 32-bit draw counter must not wrap)",
+                    (long long)limit, (long long)kMaxEventLimit);
     c->event_limit = limit;
     return PT_OK;
 }
@@ -760,9 +764,10 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
     const size_t lds = adj_lds_bytes(P, c->threads);
     hipDeviceProp_t prop;
     PT_HIP(hipGetDeviceProperties(&prop, c->device));
-    if (lds > prop.sharedMemPerBlock)
+    if (lds + kTraceStaticLds > prop.sharedMemPerBlock)
         return fail(PT_E_UNSUPPORTED, "adjoint needs %zu bytes of LDS per workgroup (limit %zu): "
-                                      "fewer groups or batches", lds, (size_t)prop.sharedMemPerBlock);
+                                      "fewer groups or batches", lds + kTraceStaticLds,
+                    (size_t)prop.sharedMemPerBlock);
     unsigned long long *d_tal = nullptr;
     int32_t *d_grp = nullptr;
     double *d_sum = nullptr, *d_gt = nullptr;
@@ -1489,9 +1494,9 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
     *lds = adj_lds_bytes(P, c->threads);
     hipDeviceProp_t prop;
     PT_HIP(hipGetDeviceProperties(&prop, c->device));
-    if (*lds > prop.sharedMemPerBlock)
+    if (*lds + kTraceStaticLds > prop.sharedMemPerBlock)
         return fail(PT_E_UNSUPPORTED, "batch of %d structures needs %zu bytes of LDS per workgroup",
-                    B->nstruct, *lds);
+                    B->nstruct, *lds + kTraceStaticLds);
     PT_HIP(launch_blocks(c, TRACE_BATCH, precision, *lds, blocks));
     return PT_OK;
 }

@@ -157,6 +157,13 @@ struct TraceParams {
 };
 
 __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
+// Static LDS of trace_kernel: eight per-lane event counters of a 256-lane workgroup.
+constexpr size_t kTraceStaticLds = 8 * 256 * sizeof(unsigned long long);
+// Largest event limit: a path draws at most 7 numbers before its first event and 6 per
+// event (specular decision, two direction draws, the transmission draw, the nmaterial > 1
+// draw of RT:2399 and a new free path), and its draw counter is 32 bits wide.
+constexpr int64_t kMaxEventLimit = (((int64_t)1 << 32) - 8) / 6;
+// Dynamic LDS of trace_kernel (the static kTraceStaticLds comes on top)
 inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 {
     return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt) *

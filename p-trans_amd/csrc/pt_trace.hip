@@ -1815,6 +1815,7 @@ trace_kernel(TraceParams P)
     // per-lane statistics live in LDS, not in registers held through the whole trace:
     // paths, events, draws, lost, start mismatch, event limit, FP64 retests, fallbacks
     __shared__ unsigned long long lane_stats[8][256];
+    static_assert(sizeof(lane_stats) == kTraceStaticLds, "static LDS of the trace kernel");
     for (int k = 0; k < 8; ++k) lane_stats[k][threadIdx.x] = 0ull;
     const int lane = threadIdx.x & 63;
     AdjLane al;
@@ -1931,7 +1932,9 @@ trace_kernel(TraceParams P)
             if (ph.end_side != 0 ||
                 step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
                                                &P.bt, stk, pre_best, pre_dmin)) {
-                const bool transmitted = ph.end_side != 1;          // RT:1833
+                // StartSide /= EndSide (RT:1833): lost paths count, as in the reference;
+                // a path stopped at the event limit has not ended and does not
+                const bool transmitted = ph.end_side != 1 && !(ph.flags & PT_PATH_EVENT_LIMIT);
                 if (is_grid_mode(kMode)) {
                     P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
                 } else {

@@ -131,6 +131,7 @@ contains
     integer, intent(out) :: status
     integer(c_int64_t), allocatable, target :: counts(:,:,:)
     real(c_double), allocatable, target :: va(:)
+    type(pt_stats), target :: stats
     integer :: ia
 
     do ia = 1, param%nangle                                   ! RT:1555-1557
@@ -142,11 +143,15 @@ contains
     counts = 0
     status = pt_trace_grid(ctx, geom, modes, param%MeanFreePath_current, &
          int(param%nangle, c_int32_t), c_loc(va), int(param%nphonon, c_int64_t), &
-         ptrans_precision, c_loc(counts), c_null_ptr)
+         ptrans_precision, c_loc(counts), c_loc(stats))
     if (status /= PT_OK) then
       write(fd_stdout, '(" libptrans: ", a)') ptrans_error_message()
       return
     end if
+    ! paths stopped by the library's event safety limit are counted as not transmitted
+    ! (the reference's event loop has no bound, RT:1693): say so, they bias T low
+    if (stats%event_limited > 0) write(fd_stdout, '(" libptrans: warning: ", i0, &
+         & " paths stopped at the event limit (counted as not transmitted)")') stats%event_limited
     transmit(:,:,:) = dble(counts) / dble(param%nphonon)
   end subroutine ptrans_transmission_function
 

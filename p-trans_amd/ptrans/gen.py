@@ -203,6 +203,28 @@ def voronoi_cube(seed=4, L=1000.0, ngrain=5, kind="Gamma", value=2.0, p=0.2, p_b
                     **_one_material(kind, value, p))
 
 
+def two_grain_slab(L=1000.0, W=200.0, a=400.0, kind="Value", value=1.0, p=1.0, p_side=1.0,
+                   nphonon=100, nangle=3, mfp=None) -> McpModel:
+    """Box L x W x W cut by the plane x = a into two Voronoi grains: grain 1 = [0, a] (the
+    only incident grain), grain 2 = [a, L].  The shared facet carries the material pair's
+    law and specularity; side walls y, z have specularity p_side (1: mirrors, which keep a
+    path's |d_x|, so every path meets the boundary once from grain 1)."""
+    box = (L, W, W)
+    halves = [(_clip_polyhedron(_box_faces(box), (1.0, 0.0, 0.0), a, 2), (0.5 * a, 0.5 * W, 0.5 * W)),
+              (_clip_polyhedron(_box_faces(box), (-1.0, 0.0, 0.0), -a, 1),
+               (0.5 * (a + L), 0.5 * W, 0.5 * W))]
+    grains = []
+    for faces, gen in halves:
+        facets = []
+        for verts, tag in _clean_faces(faces):
+            v = [tuple(float(c) for c in q) for q in verts]
+            facets.append(FacetSpec(v, "Boundary", 0) if tag < 0 else FacetSpec(v, "Si", int(tag)))
+        grains.append(GrainSpec(gen, "length", (25.0, 25.0, 25.0), "Si", facets))
+    return McpModel(ncell=(1, 1, 1), lcell=box, cellspec=(1.0, 1.0, p_side, p_side, p_side, p_side),
+                    nphonon=nphonon, nangle=nangle, mfp=mfp, kind_grain="voronoi", grains=grains,
+                    **_one_material(kind, value, p))
+
+
 def cube_triangles(lo: Sequence[float], hi: Sequence[float], nsub: int = 1, jitter: float = 0.0,
                    rng=None):
     """The surface of an axis-aligned brick as 12*nsub^2 triangles, counter-clockwise

@@ -12,12 +12,23 @@ module array (count/np per cell, last angle 0), and the raw int64 counts.
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import numpy as np
 
 from . import _lib
 from .constants import BALLISTIC_MFP, PREC_FP64
 from .geometry import Geometry, modes_desc
+
+
+def _stats(stats) -> dict:
+    """The stats dict of a trace call; warns when paths stopped at the event limit (the
+    library counts them as not transmitted, which biases the transmission low)."""
+    d = stats.as_dict()
+    if d.get("event_limited", 0) > 0:
+        warnings.warn("%d paths stopped at the event limit and were counted as not "
+                      "transmitted" % d["event_limited"], RuntimeWarning, stacklevel=3)
+    return d
 
 
 class Geom:
@@ -130,7 +141,7 @@ class Context:
             self.handle, geom.handle, modes.handle, float(mfp_nm), int(nangle),
             va.ctypes.data if va is not None else None, int(np_), int(precision),
             counts.ctypes.data, ctypes.byref(stats)))
-        return counts.reshape((nb, nk, nangle), order="F"), stats.as_dict()
+        return counts.reshape((nb, nk, nangle), order="F"), _stats(stats)
 
     def transmission_function(self, geom: Geom, modes: Modes, mfp_nm: float = BALLISTIC_MFP,
                               nangle: int = 91, np_: int = 1000, vangle=None):
@@ -182,7 +193,7 @@ class Context:
             sc.ctypes.data if sc is not None else None, paths.ctypes.data, trans.ctypes.data,
             ctypes.byref(stats)))
         return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"),
-                stats.as_dict())
+                _stats(stats))
 
     def replay_alias(self, geom: Geom, modes: Modes, first: int, n: int, branch_scale=None,
                      precision: int = PREC_FP64):
@@ -214,7 +225,7 @@ def _adjoint_alias(self, geom: Geom, modes: Modes, npaths: int, surface_group=No
         int(ngroup), float(baseline), int(nbatch), paths.ctypes.data, trans.ctypes.data,
         sums.ctypes.data, ctypes.byref(stats)))
     return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"), sums,
-            stats.as_dict())
+            _stats(stats))
 
 
 Context.adjoint_alias = _adjoint_alias
@@ -235,7 +246,7 @@ def _trace_alias_grains(self, geom: Geom, modes: Modes, npaths: int, branch_scal
         sc.ctypes.data if sc is not None else None, paths.ctypes.data, trans.ctypes.data,
         tally.ctypes.data, ctypes.byref(stats)))
     return (paths.reshape((nb, nk), order="F"), trans.reshape((nb, nk), order="F"), tally,
-            stats.as_dict())
+            _stats(stats))
 
 
 Context.trace_alias_grains = _trace_alias_grains
@@ -283,7 +294,7 @@ class Batch:
         _lib.check(fn(self.ctx.handle, self.handle, modes.handle, int(precision),
                       b.ctypes.data if b is not None else None, sums.ctypes.data, fin.ctypes.data,
                       ctypes.byref(st)))
-        return sums, fin, st.as_dict()
+        return sums, fin, _stats(st)
 
     def close(self):
         if self.handle:

@@ -57,6 +57,73 @@ def test_lorentz_intermediate_differs_and_is_between():
     assert min(tot["t0"], tot["t1"]) < tot["lor"] < max(tot["t0"], tot["t1"]), tot
 
 
+def _slab(law, param, L=1000.0, a=400.0, p=1.0, nphonon=100):
+    """Two grains split at x = a, mirror side walls: a ballistic path meets the boundary
+    once from grain 1 and leaves through x = L if it crosses, through x = 0 if not."""
+    geo = geometry.build_geometry(gen.two_grain_slab(L=L, a=a, p=p, nphonon=nphonon))
+    for g in geo.grains[1:]:
+        for s in g.surfaces:
+            if s.internal and s.grain_outside > 0:
+                s.trans_kind, s.trans_param = law, param
+    return geo
+
+
+def test_lorentz_law_closed_form_on_a_slab():
+    """Ballistic two-grain slab with a specular (p = 1) Lorentzian boundary: every path
+    meets the boundary exactly once, so T(omega) = t(omega) = 1/(1 + (omega/omega0)^2)
+    exactly in expectation.  Modes at omega0/2, omega0 and 2 omega0 (t = 0.8, 0.5, 0.2),
+    both traced angles pooled: T is within 3 binomial sigma of t, and at least 8 sigma
+    from the law without the square (0.667, 0.5, 0.333), so a dropped square fails."""
+    w0 = 2 * np.pi * 5.0e12
+    om = np.array([[0.5 * w0, w0, 2.0 * w0]])                  # nb = 1, nk = 3
+    geo = _slab(TRANS_LORENTZ, w0)
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(om)
+    np_ = 20000
+    counts, st = O.trace_grid(gdesc, mdesc, 1.0e10, 3, np_, 31, math=O.MATH_OWN)
+    assert st[3] == 0 and st[4] == 0                           # none lost, no start mismatch
+    n = 2 * np_
+    for k, ratio in enumerate((0.5, 1.0, 2.0)):
+        T = counts[0, k, :2].sum() / n
+        t = 1.0 / (1.0 + ratio * ratio)
+        sig = np.sqrt(t * (1 - t) / n)
+        assert abs(T - t) <= 3 * sig, (ratio, T, t)
+        wrong = 1.0 / (1.0 + ratio)
+        if ratio != 1.0:
+            assert abs(T - wrong) >= 8 * sig, (ratio, T, wrong)
+
+
+def test_grain_flux_attribution_closed_form_on_a_slab():
+    """Alias mode on the slab with a boundary of t = 1, one ballistic mode: every path
+    transmits, crossing grain 1 (x in [0, a]) and then grain 2 ([a, L]) with |d_x| =
+    cos(theta) kept by the mirror side walls.  Per path the flights in grain 1 sum to
+    a in x and a / cos(theta) in length, those in grain 2 to L - a and (L - a) /
+    cos(theta), up to the 1e-7 nm nudges (at most 1e-7 nm per event), with cos(theta) =
+    sqrt(u) from the path's Lambertian draw (draw 4: two mode draws, y, z, then u).
+    Crediting a flight to the grain after a crossing, or dropping the box-grain order,
+    moves a, L - a or the lengths between the grains and fails."""
+    L, a, lam = 1000.0, 400.0, 1.0e10
+    geo = _slab(TRANS_VALUE, 1.0, L=L, a=a)
+    om = np.array([[3.0e13]])
+    mfp = np.array([[lam]])
+    one = np.ones((1, 1))
+    gdesc, _g = geo.desc()
+    mdesc, _m = geometry.modes_desc(om, one, mfp, one)
+    prob, alias = O.build_alias(np.ones(1))
+    n, seed = 4000, 23
+    mp, mt, gt = O.trace_alias_grains(gdesc, mdesc, mfp, prob, alias, seed, 0, n, len(geo.grains))
+    _, _, rec = O.trace_alias(gdesc, mdesc, mfp, prob, alias, seed, 0, n, records=True)
+    assert int(mp.sum()) == n and int(mt.sum()) == n           # t = 1: all transmit
+    assert np.all(rec["end_side"] == 2)
+    inv_cos = np.array([1.0 / np.sqrt(O.philox_draw(seed, i, 4)) for i in range(n)])
+    tol = 1.0e-7 * float(rec["events"].sum()) + 1e-9 * n * L
+    dx, dl = gt[:, 0] * lam, gt[:, 1] * lam
+    assert dx[0] == 0.0 and dl[0] == 0.0                       # never a flight in the box grain
+    assert abs(dx[1] - n * a) <= tol and abs(dx[2] - n * (L - a)) <= tol, (dx, n * a)
+    assert abs(dl[1] - a * inv_cos.sum()) <= tol, (dl[1], a * inv_cos.sum())
+    assert abs(dl[2] - (L - a) * inv_cos.sum()) <= tol, (dl[2], (L - a) * inv_cos.sum())
+
+
 def _film(periodic, holes):
     m = gen.film_holes(seed=2, nholes=holes, p_side=1.0, p_film=0.2, nphonon=4000, nangle=5)
     geo = geometry.build_geometry(m)
