@@ -316,6 +316,25 @@ int pt_trace_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes, int6
                    int32_t precision, const double *branch_scale, int64_t *mode_paths,
                    int64_t *mode_trans, pt_stats *stats);
 
+/*
+ * pt_trace_alias without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and errors as
+ * pt_trace_alias; it calls no hipMalloc/hipFree and allocates no host memory.  Its work
+ * space (queue and statistics counters, per-mode tallies, branch scales) is one static
+ * device array of 1 MiB per GPU, held from the library's load until the process ends
+ * and shared by every call on that GPU: calls that overlap on one GPU (from any thread,
+ * context or stream) race and must be serialised by the caller.  At most
+ * PT_NOALLOC_MAX_MODES modes (nb * nk) and PT_NOALLOC_MAX_BRANCHES branches; larger
+ * tables return PT_E_UNSUPPORTED.  The context must have been created on the GPU
+ * (pt_ctx_create looks up the array's device address).
+ */
+#define PT_NOALLOC_MAX_MODES 65536
+#define PT_NOALLOC_MAX_BRANCHES 64
+int pt_trace_alias_noalloc(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
+                           int64_t npaths, int32_t precision, const double *branch_scale,
+                           int64_t *mode_paths, int64_t *mode_trans, pt_stats *stats);
+
 /* Per-path records of alias-mode paths first .. first+n-1 (parity checks). */
 int pt_replay_alias(pt_ctx *ctx, const pt_geom *geom, const pt_modes *modes,
                     const double *branch_scale, int64_t first, int64_t n, int32_t precision,

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,38 @@ int upload(const std::vector<T> &v, T **dptr, hipStream_t s)
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Scratch of the allocation-free alias trace (pt_trace_alias_noalloc): the queue and
+// statistics counters, the per-mode tallies and the per-branch scales of one call.  Each
+// device holds its own copy of the array from the library's load until the process ends;
+// its device address is looked up once per device when a context is created there.
+constexpr size_t kNoAllocMaxModes = 65536;
+constexpr int kNoAllocMaxBranches = 64;
+constexpr size_t kNoAllocCounters = 16;
+constexpr size_t kNoAllocWords = kNoAllocCounters + 2 * kNoAllocMaxModes + kNoAllocMaxBranches;
+static_assert(kNoAllocMaxModes == PT_NOALLOC_MAX_MODES && kNoAllocMaxBranches == PT_NOALLOC_MAX_BRANCHES,
+              "limits of the allocation-free trace");
+static __device__ __align__(256) unsigned long long scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_addr[kMaxDevices] = {};   // device address of `scratch` per device
+
+// The device address of `scratch` on the current device `device` (looked up once).
+hipError_t scratch_address(int device, unsigned long long **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = (unsigned long long *)p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct pt_ctx {
@@ -77,6 +110,7 @@ struct pt_ctx {
     unsigned long long *counters = nullptr;   // queue + 6 stats + skipped
     double *scale = nullptr;                   // per-branch scale of the current call
     int scale_n = 0;
+    unsigned long long *noalloc = nullptr;     // this device's `scratch` (device address)
 };
 
 struct pt_geom {
@@ -141,6 +175,7 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = scratch_address(device, &c->noalloc);
     if (e != hipSuccess) {
         pt_ctx_destroy(c);
         return fail(PT_E_HIP, "pt_ctx_create: %s", hipGetErrorString(e));
@@ -1029,6 +1064,72 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     fill_stats(stats, host_ctr, ms, 0.f);
     if ((int64_t)host_ctr[1] != npaths)
         return fail(PT_E_HIP, "pt_trace_alias: %lld of %lld paths completed",
+                    (long long)host_ctr[1], (long long)npaths);
+    return PT_OK;
+}
+
+int pt_trace_alias_noalloc(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npaths,
+                           int32_t precision, const double *branch_scale, int64_t *mode_paths,
+                           int64_t *mode_trans, pt_stats *stats)
+{
+    if (!c || !G || !M) return fail(PT_E_INVALID, "NULL context, geometry or modes");
+    if (G->ctx != c || M->ctx != c) return fail(PT_E_INVALID, "handles belong to another context");
+    if (!M->has_alias) return fail(PT_E_INVALID, "modes were created without vel, mfp and weight");
+    if (precision != PT_PREC_FP64 && precision != PT_PREC_FP32_RETEST &&
+        precision != PT_PREC_FP32_FILTER)
+        return fail(PT_E_UNSUPPORTED, "precision %d", precision);
+    if (npaths < 0) return fail(PT_E_INVALID, "npaths must be >= 0");
+    if (!mode_paths || !mode_trans) return fail(PT_E_INVALID, "tally arrays are NULL");
+    const size_t nm = (size_t)M->nb * M->nk;
+    if (nm > kNoAllocMaxModes || M->nb > kNoAllocMaxBranches)
+        return fail(PT_E_UNSUPPORTED, "pt_trace_alias_noalloc: %d x %d modes exceed its scratch "
+                                      "(at most %zu modes and %d branches)",
+                    M->nb, M->nk, kNoAllocMaxModes, kNoAllocMaxBranches);
+    if (branch_scale)
+        for (int b = 0; b < M->nb; ++b)
+            if (!(branch_scale[b] > 0) || !std::isfinite(branch_scale[b]))
+                return fail(PT_E_INVALID, "branch_scale[%d] must be positive and finite", b);
+    if (!c->noalloc) return fail(PT_E_HIP, "pt_trace_alias_noalloc: no scratch on device %d", c->device);
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (npaths == 0) return PT_OK;
+    PT_HIP(hipSetDevice(c->device));
+    // scratch: counters (queue, statistics) | mode_paths | mode_trans | branch scales
+    unsigned long long *w = c->noalloc;
+    TraceParams P = base_params(c, G, M);
+    P.precision = precision;
+    P.total = npaths;
+    P.queue = w;
+    P.stats = w + 1;
+    P.mode_paths = w + kNoAllocCounters;
+    P.mode_trans = P.mode_paths + nm;
+    double *d_scale = (double *)(w + kNoAllocCounters + 2 * kNoAllocMaxModes);
+    P.m.scale = branch_scale ? d_scale : nullptr;
+    unsigned long long host_ctr[16] = {0};
+    float ms = 0.f;
+    int blocks = 0;
+    hipError_t e;
+    do {
+        if ((e = launch_blocks(c, TRACE_ALIAS, precision, 0, &blocks))) break;
+        P.chunk = wave_chunk(c, P.total, blocks);
+        // the counters start at zero (a fill kernel, no memset node); the tallies start at
+        // the caller's values, to which the kernel adds
+        if ((e = launch_fill(w, kNoAllocCounters, c->stream))) break;
+        if ((e = hipMemcpyAsync(P.mode_paths, mode_paths, nm * sizeof(int64_t), hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = hipMemcpyAsync(P.mode_trans, mode_trans, nm * sizeof(int64_t), hipMemcpyHostToDevice, c->stream))) break;
+        if (branch_scale && (e = hipMemcpyAsync(d_scale, branch_scale, M->nb * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
+        if ((e = hipEventRecord(c->ev0, c->stream))) break;
+        if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, 0, c->stream))) break;
+        if ((e = hipEventRecord(c->ev1, c->stream))) break;
+        if ((e = hipMemcpyAsync(mode_paths, P.mode_paths, nm * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(mode_trans, P.mode_trans, nm * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, w, kNoAllocCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipStreamSynchronize(c->stream))) break;
+        e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    } while (0);
+    if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_alias_noalloc: %s", hipGetErrorString(e));
+    fill_stats(stats, host_ctr, ms, 0.f);
+    if ((int64_t)host_ctr[1] != npaths)
+        return fail(PT_E_HIP, "pt_trace_alias_noalloc: %lld of %lld paths completed",
                     (long long)host_ctr[1], (long long)npaths);
     return PT_OK;
 }

@@ -21,7 +21,7 @@ EXPORTS = (
     "pt_geom_create", "pt_geom_destroy", "pt_geom_set_specularity", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
     "pt_trace_grid", "pt_replay", "pt_trace_log", "pt_alias_build", "pt_trace_alias",
-    "pt_replay_alias",
+    "pt_trace_alias_noalloc", "pt_replay_alias",
     "pt_adjoint_alias", "pt_trace_alias_grains",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_info", "pt_batch_set_specularity",
     "pt_batch_trace", "pt_batch_trace_graph",
@@ -84,6 +84,7 @@ def load():
     L.pt_replay.argtypes = [vp, vp, vp, dbl, i32, vp, i64, i64, i64, i32, vp]
     L.pt_alias_build.argtypes = [i64, vp, vp, vp]
     L.pt_trace_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
+    L.pt_trace_alias_noalloc.argtypes = [vp, vp, vp, i64, i32, vp, vp, vp, vp]
     L.pt_replay_alias.argtypes = [vp, vp, vp, vp, i64, i64, i32, vp]
     L.pt_geom_set_specularity.argtypes = [vp, vp]
     L.pt_adjoint_alias.argtypes = [vp, vp, vp, i64, i32, vp, vp, i32, dbl, i32, vp, vp, vp, vp]

@@ -181,14 +181,17 @@ class Context:
         return [pts[i, :min(int(npts[i]), max_points)].copy() for i in range(n)], rec
 
     def trace_alias(self, geom: Geom, modes: Modes, npaths: int, branch_scale=None,
-                    precision: int = PREC_FP64):
-        """pt_trace_alias: per-mode (nb, nk) int64 path and transmitted counts, stats."""
+                    precision: int = PREC_FP64, noalloc: bool = False):
+        """pt_trace_alias (noalloc: pt_trace_alias_noalloc, the allocation-free variant):
+        per-mode (nb, nk) int64 path and transmitted counts, stats."""
         nb, nk = modes.nbranch, modes.nkpoint
         paths = np.zeros(nb * nk, dtype=np.int64)
         trans = np.zeros(nb * nk, dtype=np.int64)
         stats = _lib.PtStats()
         sc = None if branch_scale is None else np.ascontiguousarray(branch_scale, dtype=np.float64)
-        _lib.check(_lib.load().pt_trace_alias(
+        L = _lib.load()
+        fn = L.pt_trace_alias_noalloc if noalloc else L.pt_trace_alias
+        _lib.check(fn(
             self.handle, geom.handle, modes.handle, int(npaths), int(precision),
             sc.ctypes.data if sc is not None else None, paths.ctypes.data, trans.ctypes.data,
             ctypes.byref(stats)))
