@@ -26,6 +26,13 @@
 !       the Philox stream (seed, 2^62); prints "T i2 i1 kind value" for every material
 !       pair (GammaDeterm's gamma for TBC rows, RT:858-870) and "S ig is p" for every
 !       surface of every grain.
+!   replay dump  <file.mcp>
+!       loadMCP only; prints the parameters the trace reads ("PARAM ...", "MFP s value",
+!       "CELLSPEC ..."), then per grain "G ig incident nsurface generatrix(3)" and per
+!       surface "S ig is kindShape internal incident grain_inside grain_outside
+!       imaterial1 imaterial2 kindTrans nvertex", "R specularity radius law", "C center",
+!       "N normal", "T tangent" and "V vertex" lines (law: param%transmissivity of the
+!       surface's material pair, 0 for the box grain).
 program replay_driver
   use RTMC
   use iso_c_binding
@@ -69,6 +76,8 @@ program replay_driver
     call run_post()
   case ('laws')
     call run_laws()
+  case ('dump')
+    call run_dump()
   case default
     write(*,*) 'usage: replay grid|paths|geom ...'
     stop 2
@@ -170,6 +179,42 @@ contains
       end do
     end do
   end subroutine run_laws
+
+  subroutine run_dump()
+    integer :: ig, is, k
+    real(8) :: law
+    type(Surface_type), pointer :: sf
+    call load(status)
+    write(*,'(a,5es27.17e3,es27.17e3,3i12,2i6,es27.17e3)') 'PARAM ', param%boxx, param%boxy, &
+      param%boxz, param%scopey, param%scopez, param%temperature, param%nphonon, param%nangle, &
+      param%nMeanFreePath, param%kindGrain, param%nmaterial, param%specfluc
+    do is = 1, param%nMeanFreePath
+      write(*,'(a,i6,es27.17e3)') 'MFP ', is, param%vMeanFreePath(is)
+    end do
+    write(*,'(a,6es27.17e3)') 'CELLSPEC ', param%cellspec
+    write(*,'(a,4i8)') 'GRAINS ', lbound(vgrain, 1), ubound(vgrain, 1), igrain_box, igrain_fill
+    do ig = lbound(vgrain, 1), ubound(vgrain, 1)
+      write(*,'(a,i8,i3,i8,3es27.17e3)') 'G ', ig, merge(1, 0, vgrain(ig)%incident), &
+        vgrain(ig)%nsurface, vgrain(ig)%generatrix
+      do is = 1, vgrain(ig)%nsurface
+        sf => vgrain(ig)%vsurface(is)
+        law = 0.0d0
+        if (ig /= igrain_box) law = param%transmissivity(sf%imaterial2, sf%imaterial1)
+        write(*,'(a,11(1x,i0))') 'S ', ig, is, sf%kindShape, merge(1, 0, sf%internal), &
+          merge(1, 0, sf%incident), sf%grain_inside, sf%grain_outside, sf%imaterial1, &
+          sf%imaterial2, sf%kindTrans, sf%nvertex
+        write(*,'(a,3es27.17e3)') 'R ', sf%specularity, sf%radius, law
+        write(*,'(a,3es27.17e3)') 'C ', sf%center
+        write(*,'(a,3es27.17e3)') 'N ', sf%normal
+        write(*,'(a,9es27.17e3)') 'T ', sf%tangent
+        if (sf%kindShape == ID_SHAPE_POLYGON) then
+          do k = 1, sf%nvertex
+            write(*,'(a,3es27.17e3)') 'V ', sf%vertex(:, k)
+          end do
+        end if
+      end do
+    end do
+  end subroutine run_dump
 
   subroutine run_post()
     character(len=256) :: tfile, mcp

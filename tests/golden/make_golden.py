@@ -14,7 +14,10 @@ what the reference computed:
   geom_cases.txt, geom_ref.txt         reference geometry routines on random cases
 
 Only runs where /root/reference and oracle/_ref exist (the development container):
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [laws | examples | case NAME...]
+
+  examples/<name>.json                 loadMCP of the reference's own Examples/01-05: the
+                                       parameters and every surface record (numbers only)
 """
 from __future__ import annotations
 
@@ -35,6 +38,19 @@ sys.path.insert(0, os.path.join(ROOT, "p-trans_amd"))
 from ptrans import anphon, gen, mcp  # noqa: E402
 
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+def _two_materials():
+    """Bricks with Ge spheres and Si z-cylinders in Si (Nm = 2): the reference then keeps
+    no phonon table (RT:637-646) and its Gamma law draws an extra number per decision,
+    t = 1/(gamma + 1) + u (1 - 1/(gamma + 1)) (RT:1423-1425, 2396-2402)."""
+    from ptrans.mcp import TransRow
+    m = gen.inclusions(seed=3, nsphere=4, ncyl=2, kind="Value", value=0.3, p=0.5, nphonon=20,
+                       nangle=5, mfp=(500.0,))
+    grains = [dataclasses.replace(g, material="Ge") if g.shape == "sphere" else g for g in m.grains]
+    rows = [TransRow("Si", "Si", "Value", 0.3, 0.5), TransRow("Si", "Ge", "Gamma", 1.5, 0.4),
+            TransRow("Ge", "Si", "Gamma", 0.5, 0.6), TransRow("Ge", "Ge", "Value", 0.0, 0.0)]
+    return dataclasses.replace(m, materials=["Si", "Ge"], nt=(2, 2), trans_rows=rows, grains=grains)
+
 
 # name -> (model factory, (nbranch, nkpoint, table seed), mfp_nm, paths seed, npaths, grid seed)
 CASES = {
@@ -66,6 +82,8 @@ CASES = {
                                                   nphonon=20, nangle=5, mfp=(400.0,)),
                                  specfluc=0.1),
                              (2, 3, 19), 400.0, 1009, 240, 1009),
+    # two materials: a 1 x 1 table (the reference loads none for Nm > 1)
+    "c3_two_materials_gamma": (_two_materials, (1, 1, 20), 500.0, 1010, 240, 2010),
 }
 
 
@@ -307,9 +325,81 @@ def run_laws(name, spec):
     return len(lines)
 
 
+EXAMPLES = "/root/reference/Examples"
+
+
+def _f(x):
+    return float(x.replace("D", "E"))
+
+
+def run_examples():
+    """loadMCP of the reference on its own Examples/01-05 (with a synthetic Si table; the
+    examples use the Value law, which reads none of it): every parameter the trace reads
+    and every surface record, as tests/golden/examples/<name>.json."""
+    outdir = os.path.join(HERE, "examples")
+    os.makedirs(outdir, exist_ok=True)
+    for fn in sorted(os.listdir(EXAMPLES)):
+        if not fn.endswith(".mcp"):
+            continue
+        with tempfile.TemporaryDirectory() as tmp:
+            os.makedirs(os.path.join(tmp, "Materials"))
+            anphon.write_anphon(os.path.join(tmp, "Materials", "Si.anphon"),
+                                anphon.synthetic_table(2, 3, 11))
+            out = subprocess.run([os.path.join(REF_DIR, "replay_libm"), "dump",
+                                  os.path.join(EXAMPLES, fn)], cwd=tmp, capture_output=True,
+                                 text=True, timeout=120, check=True)
+        rec = dict(source="Examples/" + fn, mfp=[], grains=[])
+        for ln in out.stdout.splitlines():
+            t = ln.split()
+            if not t:
+                continue
+            if t[0] == "PARAM":
+                v = t[1:]
+                rec["param"] = dict(box=[_f(x) for x in v[0:3]], scope=[_f(x) for x in v[3:5]],
+                                    temperature=_f(v[5]), nphonon=int(v[6]), nangle=int(v[7]),
+                                    nmfp=int(v[8]), kind_grain=int(v[9]), nmaterial=int(v[10]),
+                                    specfluc=_f(v[11]))
+            elif t[0] == "MFP":
+                rec["mfp"].append(_f(t[2]))
+            elif t[0] == "CELLSPEC":
+                rec["cellspec"] = [_f(x) for x in t[1:]]
+            elif t[0] == "GRAINS":
+                rec["grain_range"] = [int(x) for x in t[1:]]
+            elif t[0] == "G":
+                rec["grains"].append(dict(index=int(t[1]), incident=int(t[2]), nsurface=int(t[3]),
+                                          generatrix=[_f(x) for x in t[4:7]], surfaces=[]))
+            elif t[0] == "S":
+                k = [int(x) for x in t[3:]]
+                rec["grains"][-1]["surfaces"].append(dict(
+                    kind=k[0], internal=k[1], incident=k[2], grain_inside=k[3], grain_outside=k[4],
+                    imaterial1=k[5], imaterial2=k[6], trans_kind=k[7], nvertex=k[8], vertices=[]))
+            elif t[0] in ("R", "C", "N", "T", "V"):
+                sf = rec["grains"][-1]["surfaces"][-1]
+                v = [_f(x) for x in t[1:]]
+                if t[0] == "R":
+                    sf["specularity"], sf["radius"], sf["law"] = v
+                elif t[0] == "V":
+                    sf["vertices"].append(v)
+                else:
+                    sf[{"C": "center", "N": "normal", "T": "tangent"}[t[0]]] = v
+        with open(os.path.join(outdir, fn[:-4] + ".json"), "w") as f:
+            json.dump(rec, f, indent=0)
+        print("%-32s %d grains, %d surfaces" % (fn, len(rec["grains"]),
+                                                sum(len(g["surfaces"]) for g in rec["grains"])))
+
+
 def main():
     if not os.path.exists(os.path.join(REF_DIR, "replay_own")):
         sys.exit("oracle/_ref is not built: make -C oracle ref (needs /root/reference)")
+    if len(sys.argv) > 1 and sys.argv[1] == "examples":
+        run_examples()
+        return
+    if len(sys.argv) > 2 and sys.argv[1] == "case":
+        for name in sys.argv[2:]:
+            for tag in ("own", "libm"):
+                t, n, g = run_case(name, CASES[name], tag)
+                print("%-22s %-4s paths %4d transmitted %4d grid total %d" % (name, tag, n, t, g))
+        return
     for name, spec in LAW_CASES.items():
         print("%-22s %d lines" % (name, run_laws(name, spec)))
     if len(sys.argv) > 1 and sys.argv[1] == "laws":
