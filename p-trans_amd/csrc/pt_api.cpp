@@ -111,6 +111,10 @@ struct pt_ctx {
     double *scale = nullptr;                   // per-branch scale of the current call
     int scale_n = 0;
     unsigned long long *noalloc = nullptr;     // this device's `scratch` (device address)
+    // grow-only work space of the alias, adjoint and grain-tally calls (tallies, groups,
+    // sums), kept between calls so a repeated call allocates nothing
+    void *work = nullptr;
+    size_t work_bytes = 0;
 };
 
 struct pt_geom {
@@ -194,6 +198,7 @@ int pt_ctx_destroy(pt_ctx *c)
     if (c->host_stage) (void)hipHostFree(c->host_stage);
     if (c->counters) (void)hipFree(c->counters);
     if (c->scale) (void)hipFree(c->scale);
+    if (c->work) (void)hipFree(c->work);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -692,6 +697,24 @@ int set_scale(pt_ctx *c, const pt_modes *M, const double *scale, TraceParams &P)
     return PT_OK;
 }
 
+// The context's work space, grown to at least `bytes` (256-byte aligned pieces are carved
+// from it by the caller).
+int ctx_work(pt_ctx *c, size_t bytes, char **out)
+{
+    if (bytes > c->work_bytes) {
+        PT_HIP(hipStreamSynchronize(c->stream));
+        if (c->work) PT_HIP(hipFree(c->work));
+        c->work = nullptr;
+        c->work_bytes = 0;
+        PT_HIP(hipMalloc(&c->work, bytes));
+        c->work_bytes = bytes;
+    }
+    *out = (char *)c->work;
+    return PT_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 void fill_stats(pt_stats *stats, const unsigned long long *h, float ms, float ms_red)
 {
     if (!stats) return;
@@ -803,13 +826,17 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         return fail(PT_E_UNSUPPORTED, "adjoint needs %zu bytes of LDS per workgroup (limit %zu): "
                                       "fewer groups or batches", lds + kTraceStaticLds,
                     (size_t)prop.sharedMemPerBlock);
-    unsigned long long *d_tal = nullptr;
-    int32_t *d_grp = nullptr;
-    double *d_sum = nullptr, *d_gt = nullptr;
-    PT_HIP(hipMalloc((void **)&d_tal, 2 * nm * sizeof(unsigned long long)));
-    hipError_t e = hipMalloc((void **)&d_grp, (size_t)ns * sizeof(int32_t));
-    if (e == hipSuccess) e = hipMalloc((void **)&d_sum, nsum * sizeof(double));
-    if (e == hipSuccess && ntal) e = hipMalloc((void **)&d_gt, ntal * sizeof(double));
+    const size_t b_tal = align256(2 * nm * sizeof(unsigned long long));
+    const size_t b_grp = align256((size_t)ns * sizeof(int32_t));
+    const size_t b_sum = align256(nsum * sizeof(double));
+    char *wk = nullptr;
+    rc = ctx_work(c, b_tal + b_grp + b_sum + align256(ntal * sizeof(double)), &wk);
+    if (rc) return rc;
+    unsigned long long *d_tal = (unsigned long long *)wk;
+    int32_t *d_grp = (int32_t *)(wk + b_tal);
+    double *d_sum = (double *)(wk + b_tal + b_grp);
+    double *d_gt = ntal ? (double *)(wk + b_tal + b_grp + b_sum) : nullptr;
+    hipError_t e = hipSuccess;
     std::vector<double> h_gt(ntal);
     std::vector<unsigned long long> h_tal(2 * nm);
     std::vector<double> h_sum(nsum);
@@ -840,8 +867,6 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         if ((e = hipStreamSynchronize(c->stream))) break;
         e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
     } while (0);
-    for (void *p : {(void *)d_tal, (void *)d_grp, (void *)d_sum, (void *)d_gt})
-        if (p) (void)hipFree(p);
     if (e != hipSuccess) return fail(PT_E_HIP, "%s: %s", fn, hipGetErrorString(e));
     for (size_t i = 0; i < nm; ++i) {
         mode_paths[i] += (int64_t)h_tal[i];
@@ -1033,8 +1058,10 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     int rc = set_scale(c, M, branch_scale, P);
     if (rc) return rc;
     P.total = npaths;
-    unsigned long long *d_tal = nullptr;
-    PT_HIP(hipMalloc((void **)&d_tal, 2 * nm * sizeof(unsigned long long)));
+    char *wk = nullptr;
+    rc = ctx_work(c, align256(2 * nm * sizeof(unsigned long long)), &wk);
+    if (rc) return rc;
+    unsigned long long *d_tal = (unsigned long long *)wk;
     P.mode_paths = d_tal;
     P.mode_trans = d_tal + nm;
     std::vector<unsigned long long> h_tal(2 * nm);
@@ -1055,7 +1082,6 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
         if ((e = hipStreamSynchronize(c->stream))) break;
         e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
     } while (0);
-    (void)hipFree(d_tal);
     if (e != hipSuccess) return fail(PT_E_HIP, "pt_trace_alias: %s", hipGetErrorString(e));
     for (size_t i = 0; i < nm; ++i) {
         mode_paths[i] += (int64_t)h_tal[i];

@@ -4,6 +4,7 @@
     python -m ptrans.profiling counters <run_results.db> [kernel-substring]
     python -m ptrans.profiling traffic  <kt.db> <fetch.db> <write.db> <copy_fetch.db>
                                         <copy_write.db> <sq.db> <copy_mib> <paths> <workload>
+                                        [key=kernel-substring ...]
 
 'kernels' prints per-kernel call count and average/total duration; 'counters' prints,
 per kernel and counter, the mean value per dispatch.  'traffic' prints the JSON summary
@@ -48,7 +49,7 @@ TRACE_KERNELS = {"fp64": "trace_kernel<0, false, 0>", "fp32_filter": "trace_kern
 
 
 def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy_mib: float,
-                    paths: int, workload: str):
+                    paths: int, workload: str, kernels=None):
     copy_bytes = copy_mib * 2 ** 20
 
     def one(db, counter, sub):
@@ -63,7 +64,7 @@ def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy
     kt = {k["name"]: k for k in kernel_stats(kt_db)}
     out = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale,
            "peak_valu_wave_instr_per_s": PEAK_VALU_WAVE_INSTR_PER_S, "kernels": {}}
-    for key, sub in TRACE_KERNELS.items():
+    for key, sub in (kernels or TRACE_KERNELS).items():
         name = next(n for n in kt if sub in n)
         avg_us = kt[name]["average_ns"] / 1e3
         _, f = one(fetch_db, "FETCH_SIZE", sub)
@@ -81,8 +82,9 @@ def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy
 def main(argv):
     if len(argv) >= 11 and argv[1] == "traffic":
         import json
-        print(json.dumps(traffic_summary(*argv[2:8], float(argv[8]), int(argv[9]), argv[10]),
-                         indent=1))
+        kmap = dict(a.split("=", 1) for a in argv[11:]) or None
+        print(json.dumps(traffic_summary(*argv[2:8], float(argv[8]), int(argv[9]), argv[10],
+                                         kmap), indent=1))
         return 0
     if len(argv) < 3 or argv[1] not in ("kernels", "counters"):
         print(__doc__)

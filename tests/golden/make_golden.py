@@ -44,7 +44,7 @@ def _two_materials():
     no phonon table (RT:637-646) and its Gamma law draws an extra number per decision,
     t = 1/(gamma + 1) + u (1 - 1/(gamma + 1)) (RT:1423-1425, 2396-2402)."""
     from ptrans.mcp import TransRow
-    m = gen.inclusions(seed=3, nsphere=4, ncyl=2, kind="Value", value=0.3, p=0.5, nphonon=20,
+    m = gen.inclusions(seed=3, nsphere=4, ncyl=2, kind="Value", value=0.3, p=0.5, nphonon=60,
                        nangle=5, mfp=(500.0,))
     grains = [dataclasses.replace(g, material="Ge") if g.shape == "sphere" else g for g in m.grains]
     rows = [TransRow("Si", "Si", "Value", 0.3, 0.5), TransRow("Si", "Ge", "Gamma", 1.5, 0.4),

@@ -196,3 +196,35 @@ def test_gpu_batch_diffusive_paths_equal_oracle(gpu_ctx):
         assert sums[s, 0] == n
         assert abs(sums[s, 1] - sx) <= 1e-12 * max(sx, 1e-300), (s, sums[s, 1], sx)
     b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batch_at_c5_sizes_equals_brute_force_oracle(gpu_ctx):
+    """Structures of the screening batch's own generator (4 x 4 x 4 jittered grains) at
+    1,920, 12,000 and 94,080 triangles -- the low end, middle and high end of its
+    2e3 - 1e5 range, 160 - 2,352 BVH-held candidates per grain -- traced together, 2,000
+    paths each: per-structure path counts and sum x equal the oracle's brute-force scan of
+    each structure alone (findCollisionSurface's candidate set and first-index tie-break,
+    RT:1989-2096), in FP64 and in the bit-identical FP32 filter, at leaf sizes 2 and 8."""
+    from ptrans.trace import Batch
+    geos = [gen.sc_lattice_flat(seed=400 + m, k=4, m=m, jitter=5.0, trans_kind=2,
+                                trans_param=2.0, p=0.3) for m in (2, 5, 14)]
+    mods = _modes()
+    omega, vel, mfp, weight, prob, alias = mods
+    npaths = [2000, 2000, 2000]
+    ref = []
+    for s, geo in enumerate(geos):
+        ref.append(_oracle_struct_sums(geo, mods, 41, 2000 * s, 2000))
+    m = gpu_ctx.modes(omega=omega, vel=vel, mfp=mfp, weight=weight)
+    gpu_ctx.set_seed(41)
+    for leaf in (2, 8):
+        b = Batch(gpu_ctx, geos, npaths, leaf_size=leaf, grid=32)
+        print("C5-size batch, leaf %d:" % leaf, b.info(), flush=True)
+        for prec in (0, 2):
+            sums, fin, st = b.trace(m, precision=prec)
+            assert st["paths"] == 6000
+            for s in range(3):
+                n, sx, nt = ref[s]
+                assert sums[s, 0] == n, (leaf, prec, s, sums[s, 0], n)
+                assert abs(sums[s, 1] - sx) <= 1e-12 * max(sx, 1e-300), (leaf, prec, s, sums[s, 1], sx)
+        b.close()
