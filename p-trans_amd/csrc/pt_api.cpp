@@ -386,6 +386,25 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
         cand_off[g + 1] = (int32_t)cand_idx.size();
         inc_off[g + 1] = (int32_t)inc_idx.size();
     }
+    // per-lane scans (Voronoi-type grains: only polygon candidates, in an aligned box): a
+    // plane record per candidate
+    bool lane_scan = box_is_axis_aligned(d) && d->kind_grain == PT_GRAIN_VORONOI;
+    for (size_t j = (size_t)cand_off[1]; j < cand_idx.size() && lane_scan; ++j)
+        lane_scan = d->surfaces[cand_idx[j]].kind == PT_SHAPE_POLYGON;
+    lane_scan = lane_scan && cand_idx.size() > (size_t)cand_off[1];
+    std::vector<CandPlane> cplane;
+    if (lane_scan) {
+        cplane.resize(cand_idx.size());
+        for (size_t j = 0; j < cand_idx.size(); ++j) {
+            const pt_surface &s = d->surfaces[cand_idx[j]];
+            CandPlane &q = cplane[j];
+            memset(&q, 0, sizeof q);
+            for (int k = 0; k < 3; ++k) q.n[k] = s.normal[k];
+            if (s.kind == PT_SHAPE_POLYGON)
+                for (int k = 0; k < 3; ++k) q.v1[k] = d->vertices[3 * s.vertex_offset + k];
+            q.idx = cand_idx[j];
+        }
+    }
     std::vector<int32_t> ginc(d->grain_incident, d->grain_incident + ng);
     std::vector<double> gen(d->generatrix, d->generatrix + 3 * (size_t)ng);
     std::vector<double> vtx(d->vertices, d->vertices + 3 * (size_t)d->nvertex);
@@ -397,18 +416,19 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     DSurf *d_surf = nullptr;
     double *d_tang = nullptr, *d_vtx = nullptr, *d_gen = nullptr;
     int32_t *d_co = nullptr, *d_ci = nullptr, *d_io = nullptr, *d_ii = nullptr, *d_ginc = nullptr;
+    CandPlane *d_cp = nullptr;
     if ((rc = upload(surf, &d_surf, c->stream)) || (rc = upload(tang, &d_tang, c->stream)) ||
         (rc = upload(vtx, &d_vtx, c->stream)) || (rc = upload(gen, &d_gen, c->stream)) ||
         (rc = upload(cand_off, &d_co, c->stream)) || (rc = upload(cand_idx, &d_ci, c->stream)) ||
         (rc = upload(inc_off, &d_io, c->stream)) || (rc = upload(inc_idx, &d_ii, c->stream)) ||
-        (rc = upload(ginc, &d_ginc, c->stream))) {
+        (rc = upload(ginc, &d_ginc, c->stream)) || (rc = upload(cplane, &d_cp, c->stream))) {
         for (void *p : {(void *)d_surf, (void *)d_tang, (void *)d_vtx, (void *)d_gen, (void *)d_co,
-                        (void *)d_ci, (void *)d_io, (void *)d_ii, (void *)d_ginc})
+                        (void *)d_ci, (void *)d_io, (void *)d_ii, (void *)d_ginc, (void *)d_cp})
             if (p) (void)hipFree(p);
         delete G;
         return rc;
     }
-    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc};
+    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc, d_cp};
     G->surf = surf;
     DevGeom &g = G->dg;
     g.surf = d_surf;
@@ -421,6 +441,8 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     g.grain_incident = d_ginc;
     g.gen = d_gen;
     g.surf_base = nullptr;
+    g.cand_plane = d_cp;
+    g.lane_scan = lane_scan ? 1 : 0;
     for (int k = 0; k < 3; ++k) g.box[k] = d->box[k];
     g.scope[0] = d->scope[0];
     g.scope[1] = d->scope[1];

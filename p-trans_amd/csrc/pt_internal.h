@@ -26,6 +26,17 @@ struct alignas(16) DSurf {
 };
 static_assert(sizeof(DSurf) == 112, "DSurf layout");
 
+// One polygon candidate in scan order (the grain's candidate list): the plane data the
+// filter and the exact plane distance read, so one 64-byte load per candidate needs no
+// index chain.  idx: the global surface index.
+struct alignas(16) CandPlane {
+    double n[3];        // the surface's normal (as in DSurf)
+    double v1[3];       // its first vertex
+    int32_t idx;
+    int32_t pad[3];
+};
+static_assert(sizeof(CandPlane) == 64, "CandPlane layout");
+
 // Geometry in HBM.  Candidate lists hold, per grain, the global indices of its
 // internal surfaces in reference order (the scan of RT:2012-2028 skips the others).
 struct DevGeom {
@@ -39,6 +50,9 @@ struct DevGeom {
     const int32_t *grain_incident;
     const double *gen;         // generatrix xyz per grain
     const int32_t *surf_base;  // global index of each grain's first surface
+    const CandPlane *cand_plane;  // [cand_off[ngrain_total]] when lane_scan
+    int32_t lane_scan;         // every grain candidate is a polygon and the box is aligned:
+                               // the accelerated scan runs each lane's own candidate list
     double box[3];
     double scope[2];
     int32_t ngrain;

@@ -90,23 +90,47 @@ typedef const __attribute__((address_space(4))) double cdouble;
 typedef const __attribute__((address_space(4))) DSurf CSurf;
 typedef const __attribute__((address_space(4))) int32_t cint;
 
+// DSurf record through a scalar (constant address space) load.
+__device__ inline DSurf load_surf_c(const DSurf *base, int idx)
+{
+    CSurf *p = (CSurf *)base + idx;
+    DSurf o;
+    for (int k = 0; k < 3; ++k) { o.n[k] = p->n[k]; o.c[k] = p->c[k]; }
+    o.r = p->r;
+    o.p = p->p;
+    o.tparam = p->tparam;
+    o.kind = p->kind;
+    o.nv = p->nv;
+    o.voff = p->voff;
+    o.gout = p->gout;
+    o.tkind = p->tkind;
+    return o;
+}
+
+
 // calcSurfaceDistancePolygon, RT:2907-2959, in two parts: the one-sided plane hit
 // (cosang > 0, l >= 0), then the convex point-in-polygon test of the hit point with
 // on-edge counted inside.
+template <class NT, class VT>
+__device__ inline bool plane_hit(NT *n, VT *v, const double *rs, const double *nd, double *lhit,
+                                 OpCount &oc)
+{
+    const double cosang = nd[0] * n[0] + nd[1] * n[1] + nd[2] * n[2];
+    oc.flops += 5;
+    if (cosang <= 0.0) return false;
+    oc.flops += 9;
+    double d[3] = {v[0] - rs[0], v[1] - rs[1], v[2] - rs[2]};
+    const double l = (d[0] * n[0] + d[1] * n[1] + d[2] * n[2]) / cosang;
+    if (l < 0.0) return false;
+    *lhit = l;
+    return true;
+}
+
 template <class VT>
 __device__ inline bool polygon_plane(VT *vtx, const DSurf &s, const double *rs, const double *nd,
                                      double *lhit, OpCount &oc)
 {
-    const double cosang = nd[0] * s.n[0] + nd[1] * s.n[1] + nd[2] * s.n[2];
-    oc.flops += 5;
-    if (cosang <= 0.0) return false;
-    oc.flops += 9;
-    VT *v = vtx + 3 * (int64_t)s.voff;
-    double d[3] = {v[0] - rs[0], v[1] - rs[1], v[2] - rs[2]};
-    const double l = (d[0] * s.n[0] + d[1] * s.n[1] + d[2] * s.n[2]) / cosang;
-    if (l < 0.0) return false;
-    *lhit = l;
-    return true;
+    return plane_hit(s.n, vtx + 3 * (int64_t)s.voff, rs, nd, lhit, oc);
 }
 
 template <class VT>
@@ -152,12 +176,13 @@ __device__ inline bool dist_polygon(const DevGeom &g, const DSurf &s, const doub
 }
 
 // isInsidePolygon, RT:2961-2983
-__device__ bool inside_polygon(const double *v, int nv, const double *p, OpCount &oc)
+template <class VT>
+__device__ bool inside_polygon(VT *v, int nv, const double *p, OpCount &oc)
 {
     double c1[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < nv; ++i) {
-        const double *vi = v + 3 * i;
-        const double *vn = v + 3 * (i + 1 == nv ? 0 : i + 1);
+        VT *vi = v + 3 * i;
+        VT *vn = v + 3 * (i + 1 == nv ? 0 : i + 1);
         double ra[3], rb[3], ci[3];
         for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = p[k] - vi[k]; }
         cross3(ra, rb, ci);
@@ -305,6 +330,9 @@ __device__ bool inside_surface(const DSurf &s, const double *p, OpCount &oc)
     return false;
 }
 
+// The grain and candidate loops below run in step on every active lane (a lane that leaves
+// an inner loop early is masked), so their indices are wave-uniform and the geometry comes
+// in through scalar loads.
 __device__ int find_inside_which_grain(const DevGeom &g, const double *p, OpCount &oc)
 {
     int count = 0, which = -1;
@@ -312,8 +340,11 @@ __device__ int find_inside_which_grain(const DevGeom &g, const double *p, OpCoun
         bool in = true;
         // the candidate list of a grain is its internal surfaces; incident ones are
         // never internal (RT:1108-1111), so this is exactly isInsideGrain's set
-        for (int j = g.cand_off[ig]; j < g.cand_off[ig + 1]; ++j) {
-            if (!inside_surface(g.surf[g.cand_idx[j]], p, oc)) { in = false; break; }
+        const int j0 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig]);
+        const int j1 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig + 1]);
+        for (int j = j0; j < j1; ++j) {
+            const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
+            if (!inside_surface(load_surf_c(g.surf, idx), p, oc)) { in = false; break; }
         }
         if (in) { which = ig; ++count; }
     }
@@ -329,8 +360,8 @@ __device__ int nearest_incident_grain(const DevGeom &g, const double *p, OpCount
     double d2min = 1.0e10;
     int best = 0;
     for (int ig = 1; ig <= g.ngrain; ++ig) {
-        if (!g.grain_incident[ig]) continue;
-        const double *q = g.gen + 3 * ig;
+        if (!((cint *)g.grain_incident)[ig]) continue;
+        cdouble *q = (cdouble *)g.gen + 3 * ig;
         const double d0 = p[0] - q[0], d1 = p[1] - q[1], d2 = p[2] - q[2];
         const double dd = d0 * d0 + d1 * d1 + d2 * d2;
         oc.flops += 8;
@@ -344,10 +375,12 @@ __device__ int incident_collision_grain(const DevGeom &g, const double *p, OpCou
 {
     int found = -1;
     for (int ig = 1; ig <= g.ngrain; ++ig) {
-        if (!g.grain_incident[ig]) continue;
-        for (int j = g.inc_off[ig]; j < g.inc_off[ig + 1]; ++j) {
-            const DSurf &s = g.surf[g.inc_idx[j]];
-            if (inside_polygon(g.vtx + 3 * (int64_t)s.voff, s.nv, p, oc)) { found = ig; break; }
+        if (!((cint *)g.grain_incident)[ig]) continue;
+        const int j0 = __builtin_amdgcn_readfirstlane(((cint *)g.inc_off)[ig]);
+        const int j1 = __builtin_amdgcn_readfirstlane(((cint *)g.inc_off)[ig + 1]);
+        for (int j = j0; j < j1; ++j) {
+            const DSurf s = load_surf_c(g.surf, __builtin_amdgcn_readfirstlane(((cint *)g.inc_idx)[j]));
+            if (inside_polygon((cdouble *)g.vtx + 3 * (int64_t)s.voff, s.nv, p, oc)) { found = ig; break; }
         }
     }
     return found >= 0 ? found : g.igrain_fill;
@@ -567,6 +600,45 @@ __device__ int filt_sphere(const DSurf &s, const double *rs, const double *nd, f
     return F_HIT;
 }
 
+// Plane of a convex polygon (calcSurfaceDistancePolygon, RT:2907-2959): F_MISS when the
+// polygon certainly faces away (the reference's cosang <= 0) or its plane certainly lies
+// behind the start point (L < 0); F_HIT with an interval [lo, hi] around the reference's
+// FP64 plane distance L = ((v1 - p) . n) / cosang; F_UNSURE near cosang = 0 (grazing) or
+// L = 0 (the facet just left).  The terms v1 - p are the reference's FP64 differences, so
+// each FP32 product and sum adds a rounding of at most a few 2^-24 of the magnitudes
+// involved, far inside kMarg.  Whether the hit point lies inside the polygon is settled
+// for the event's winner only.
+template <class NT, class VT>
+__device__ inline int filt_plane(NT *n, VT *v, const double *rs, const double *nd, float *lo,
+                                 float *hi)
+{
+    const float n0 = (float)n[0], n1 = (float)n[1], n2 = (float)n[2];
+    const float c0 = (float)nd[0] * n0, c1 = (float)nd[1] * n1, c2 = (float)nd[2] * n2;
+    const float c = c0 + c1 + c2;
+    const float ec = kMarg * (fabsf(c0) + fabsf(c1) + fabsf(c2)) + 1e-30f;
+    if (c < -ec) return F_MISS;
+    if (c <= ec) return F_UNSURE;
+    const float a0 = (float)(v[0] - rs[0]) * n0, a1 = (float)(v[1] - rs[1]) * n1,
+                a2 = (float)(v[2] - rs[2]) * n2;
+    const float a = a0 + a1 + a2;
+    const float ea = kMarg * (fabsf(a0) + fabsf(a1) + fabsf(a2)) + 1e-30f;
+    if (a < -ea) return F_MISS;
+    if (a <= ea) return F_UNSURE;
+    const float r = ea / a + ec / c + kMarg;     // relative error bound of the quotient
+    if (!(r < 1.0e-3f)) return F_UNSURE;
+    const float L = a * frcp(c);
+    *lo = L * (1.0f - r);
+    *hi = L * (1.0f + r) + 1e-30f;
+    return F_HIT;
+}
+
+template <class VT>
+__device__ inline int filt_polygon(VT *vtx, const DSurf &s, const double *rs, const double *nd,
+                                   float *lo, float *hi)
+{
+    return filt_plane(s.n, vtx + 3 * (int64_t)s.voff, rs, nd, lo, hi);
+}
+
 // Running selection over candidate intervals in scan order.
 // Candidate index flag: a polygon entered with its FP64 plane hit, whose in-polygon test
 // is still due (surface indices stay below 2^30).
@@ -585,23 +657,6 @@ struct Pick {
         else if (lo < lo2) lo2 = lo;
     }
 };
-
-// DSurf record through a scalar (constant address space) load.
-__device__ inline DSurf load_surf_c(const DSurf *base, int idx)
-{
-    CSurf *p = (CSurf *)base + idx;
-    DSurf o;
-    for (int k = 0; k < 3; ++k) { o.n[k] = p->n[k]; o.c[k] = p->c[k]; }
-    o.r = p->r;
-    o.p = p->p;
-    o.tparam = p->tparam;
-    o.kind = p->kind;
-    o.nv = p->nv;
-    o.voff = p->voff;
-    o.gout = p->gout;
-    o.tkind = p->tkind;
-    return o;
-}
 
 // FP64 distance to box wall w of an axis-aligned box.  With n = +-e_a and v1[a] = P
 // (both checked when the geometry is created) every other term of the reference's
@@ -650,6 +705,24 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
         }
     }
     int q0 = -1, q1 = -1;                                 // queued unsure curved candidates
+    if (g.lane_scan) {
+        // every grain candidate is a polygon: each lane runs its own grain's list (one plane
+        // record per candidate), instead of the wave taking the grains one after another
+        todo0 = todo1 = false;
+        const int j1 = g.cand_off[ph.which + 1];
+        for (int j = g.cand_off[ph.which]; __any(j < j1); ++j) {
+            if (j < j1) {
+                const CandPlane c = g.cand_plane[j];
+                float lo = 0.f, hi = 0.f;
+                const int r = filt_plane(c.n, c.v1, rs, nd, &lo, &hi);
+                oc.flops += 12;
+                double l;
+                if (r == F_HIT) pk.add(c.idx | kPlaneBit, lo, hi, false);
+                else if (r == F_UNSURE && plane_hit(c.n, c.v1, rs, nd, &l, oc))
+                    pk.add(c.idx | kPlaneBit, l, l, true);
+            }
+        }
+    }
     while (__any(todo0 || todo1)) {
         if (todo0 || todo1) {
             const int mine = todo0 ? 0 : ph.which;
@@ -664,9 +737,11 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                     int r;
                     if (s.kind == PT_SHAPE_CYLINDER) { r = filt_cylinder(s, rs, nd, g.box[2], &lo, &hi); oc.flops += 30; }
                     else if (s.kind == PT_SHAPE_SPHERE) { r = filt_sphere(s, rs, nd, &lo, &hi); oc.flops += 30; }
-                    else r = F_UNSURE;                        // polygons: FP64 plane hit
+                    else { r = filt_polygon((cdouble *)g.vtx, s, rs, nd, &lo, &hi); oc.flops += 12; }
                     if (r == F_HIT) {
-                        pk.add(idx, lo, hi, false);
+                        // a polygon's plane interval bounds its distance from below (the hit
+                        // point may still lie outside the polygon: the winner's test below)
+                        pk.add(s.kind == PT_SHAPE_POLYGON ? idx | kPlaneBit : idx, lo, hi, false);
                     } else if (r == F_UNSURE) {
                         if (s.kind != PT_SHAPE_POLYGON && q0 < 0) {
                             q0 = idx;
@@ -702,10 +777,13 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
     }
     if (pk.idx_hi < 0 || pk.idx_lo1 != pk.idx_hi || !(pk.lo2 > pk.hi_min)) return false;
     const int win = pk.idx_hi & ~kPlaneBit;
-    if ((pk.idx_hi & kPlaneBit) && !polygon_contains(g.vtx, g.surf[win], rs, nd, pk.exact_l, oc))
-        return false;
     double l = pk.exact_l;
-    if (!pk.exact) {
+    if (pk.idx_hi & kPlaneBit) {
+        // the winning plane: its exact FP64 distance, then the in-polygon test of its hit point
+        const DSurf sw = g.surf[win];
+        if (!pk.exact && !polygon_plane(g.vtx, sw, rs, nd, &l, oc)) return false;
+        if (!polygon_contains(g.vtx, sw, rs, nd, l, oc)) return false;
+    } else if (!pk.exact) {
         if (g.box_aligned && win < 6) {
             l = box_wall_distance(g, win, rs, nd);
             oc.flops += 2;
