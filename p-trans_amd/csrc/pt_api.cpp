@@ -1261,16 +1261,19 @@ namespace {
 
 struct BvhPrim {
     double lo[3], hi[3], c[3];
+    double n[3];        // polygon normal; curved prims face every way
+    bool curved;
     int idx;
 };
 
 float round_down(double v) { float f = (float)v; if ((double)f > v) f = nextafterf(f, -3.0e38f); return f; }
 float round_up(double v) { float f = (float)v; if ((double)f < v) f = nextafterf(f, 3.0e38f); return f; }
 
-// A child slot of a BvhNode: its reference and padded bounds.
+// A child slot of a BvhNode: its reference, padded bounds and normal bounds.
 struct BvhChild {
     int32_t ref, count;
     float lo[3], hi[3];
+    float nlo[3], nhi[3];
 };
 
 // Median-split BVH over prims[b, e); returns the child slot of the subtree (a leaf or an
@@ -1294,6 +1297,13 @@ BvhChild bvh_build(std::vector<BvhPrim> &P, int b, int e, int depth, int max_dep
         const float l = round_down(lo[k]), h = round_up(hi[k]);
         me.lo[k] = l - (1.0e-3f + 1.0e-6f * fabsf(l));
         me.hi[k] = h + (1.0e-3f + 1.0e-6f * fabsf(h));
+        double nl = 1e300, nh = -1e300;
+        for (int i = b; i < e; ++i) {
+            nl = std::min(nl, P[i].curved ? -1.0e30 : P[i].n[k]);
+            nh = std::max(nh, P[i].curved ? 1.0e30 : P[i].n[k]);
+        }
+        me.nlo[k] = round_down(std::max(nl, -1.0e30));
+        me.nhi[k] = round_up(std::min(nh, 1.0e30));
     }
     *depth_out = std::max(*depth_out, depth);
     int axis = 0;
@@ -1318,6 +1328,8 @@ BvhChild bvh_build(std::vector<BvhPrim> &P, int b, int e, int depth, int max_dep
         for (int k = 0; k < 3; ++k) {
             nodes[ni].lo[c][k] = kid[c].lo[k];
             nodes[ni].hi[c][k] = kid[c].hi[k];
+            nodes[ni].nlo[c][k] = kid[c].nlo[k];
+            nodes[ni].nhi[c][k] = kid[c].nhi[k];
         }
     }
     me.ref = ni;
@@ -1340,8 +1352,12 @@ int bvh_root(std::vector<BvhPrim> &P, int max_depth, int leaf, std::vector<BvhNo
     for (int k = 0; k < 3; ++k) {
         nd.lo[0][k] = top.lo[k];
         nd.hi[0][k] = top.hi[k];
+        nd.nlo[0][k] = top.nlo[k];
+        nd.nhi[0][k] = top.nhi[k];
         nd.lo[1][k] = 3.0e38f;
         nd.hi[1][k] = -3.0e38f;
+        nd.nlo[1][k] = -1.0e30f;
+        nd.nhi[1][k] = 1.0e30f;
     }
     nodes.push_back(nd);
     return (int)nodes.size() - 1;
@@ -1349,6 +1365,8 @@ int bvh_root(std::vector<BvhPrim> &P, int max_depth, int leaf, std::vector<BvhNo
 
 void prim_bounds(const pt_geom_desc &d, const pt_surface &s, BvhPrim &p)
 {
+    p.curved = s.kind != PT_SHAPE_POLYGON;
+    for (int k = 0; k < 3; ++k) p.n[k] = s.normal[k];
     if (s.kind == PT_SHAPE_POLYGON) {
         for (int k = 0; k < 3; ++k) { p.lo[k] = 1e300; p.hi[k] = -1e300; }
         for (int v = 0; v < s.nvertex; ++v)

@@ -96,18 +96,21 @@ constexpr bool is_grid_mode(int mode) { return mode == TRACE_GRID || mode == TRA
 // this fraction of the smallest box side.
 constexpr double kBallMfpFrac = 0.05;
 
-// Bounding volume hierarchy node: FP32 bounds padded outward on the host (so FP32 slab
-// tests never drop a true hit); a >= 0: children a and b; a < 0: leaf of prims
-// [-a - 1, -a - 1 + b).  32 bytes.
-// Binary BVH node holding both children's bounds (outward padded, FP32), so one 64-byte
-// load decides which children a ray or a ball query enters and in which order.
+// Binary BVH node holding both children's bounds (outward padded, FP32), so one load
+// decides which children a ray or a ball query enters and in which order, and, per
+// child, FP32 bounds of its prims' normals (outward rounded; [-1, 1] for curved prims): a
+// child whose every polygon certainly faces away from the ray (the reference's cosang <= 0,
+// a miss) is not entered.  128 bytes: one cache line.
 struct BvhNode {
     float lo[2][3];
     float hi[2][3];
     int32_t child[2];   // >= 0: internal node index; < 0: leaf, prims from -(child + 1)
     int32_t count[2];   // leaf prim count (0 for an internal child)
+    float nlo[2][3];    // per child: componentwise bounds of the prims' normals
+    float nhi[2][3];
+    int32_t pad[4];
 };
-static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
 // per-structure sums: n, sum x, sum x^2, sum (x - b) score, sum ((x - b) score)^2
 constexpr int kBatchSums = 5;

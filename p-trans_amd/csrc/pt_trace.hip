@@ -831,7 +831,7 @@ __device__ void scan_full(const DevGeom &g, const Path &ph, double *dmin, int *b
 // i.e. less than 1e-10 nm over 1e10 nm) is taken as parallel: the origin must lie in
 // the slab, which the 1e-3 nm box padding makes safe.
 struct RayF32 {
-    float o[3], inv[3];
+    float o[3], inv[3], d[3];
     bool mv[3];
 };
 
@@ -842,6 +842,7 @@ __device__ inline RayF32 ray_f32(const double *pos, const double *dir)
     for (int k = 0; k < 3; ++k) {
         r.o[k] = (float)pos[k];
         const float d = (float)dir[k];
+        r.d[k] = d;
         r.mv[k] = fabsf(d) > 1.0e-20f;
         r.inv[k] = r.mv[k] ? 1.0f / d : 0.0f;
     }
@@ -866,6 +867,18 @@ __device__ inline bool slab_enter(const RayF32 &r, const float *lo, const float 
     }
     *tin = t0;
     return in && !(t1 < -1.0e-5f * fabsf(t1) || t0 > t1 + 1.0e-5f * fabsf(t1));
+}
+
+// True unless every normal within the bounds [nlo, nhi] certainly faces away from the
+// ray: the largest d . n over the bounds (each product at its larger end) below -1e-5,
+// far beyond the FP32 rounding of d and of the products, so every polygon below has the
+// reference's FP64 cosang < 0 (RT:2928-2930) and cannot be hit.
+__device__ inline bool may_face(const RayF32 &r, const float *nlo, const float *nhi)
+{
+    float ub = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ub += fmaxf(r.d[k] * nlo[k], r.d[k] * nhi[k]);
+    return !(ub < -1.0e-5f);
 }
 
 // The FP64 surface tests of one BVH leaf; keeps the smallest (distance, surface index).
@@ -938,8 +951,8 @@ __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, d
             const BvhNode nd = bt.nodes[stk[--sp * 256]];
             ++oc.nodes;
             float t0, t1;
-            bool g0 = slab_enter(rf, nd.lo[0], nd.hi[0], &t0);
-            bool g1 = slab_enter(rf, nd.lo[1], nd.hi[1], &t1);
+            bool g0 = slab_enter(rf, nd.lo[0], nd.hi[0], &t0) && may_face(rf, nd.nlo[0], nd.nhi[0]);
+            bool g1 = slab_enter(rf, nd.lo[1], nd.hi[1], &t1) && may_face(rf, nd.nlo[1], nd.nhi[1]);
             // margin 1e-5 relative on the entry parameter (the padded bounds cover the rest)
             g0 = g0 && !(hit && (double)t0 * (1.0 - 1.0e-5) > bl);
             g1 = g1 && !(hit && (double)t1 * (1.0 - 1.0e-5) > bl);
