@@ -458,7 +458,8 @@ def adjoint_leg(ctx, npaths: int, cpu, table_seed: int = 20):
 
 # ------------------------------------------------------------------------------ batch leg
 
-def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table_seed: int = 20):
+def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table_seed: int = 20,
+              leaf: int = 0):
     """C5 screening batch: `replays` graph replays of paths_per_struct paths per structure
     (each replay a fresh Philox key), FP32 trace, and the same replays in the bit-identical
     precision (the FP64 results) for the per-structure kappa error."""
@@ -471,10 +472,14 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     t = synthetic_table(3, 1000, seed=table_seed)
     omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
     m = ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
-    tune = autotune.tune_batch(ctx, geos, m, probe_paths=max(1000, paths_per_struct // 20),
-                               leaf_sizes=(2, 4, 8), chunks=(256, 1024), refills=(16,),
-                               repeats=1, precision=PREC_FP32_RETEST,
-                               log=lambda r: say("autotune", json.dumps(r)))
+    if leaf > 0:            # fixed setting (profiling runs): no tuning launches
+        ctx.set_tuning(256, 16, 0)
+        tune = {"leaf_size": leaf, "chunk": 256, "refill_min": 16, "blocks_per_cu": 0, "table": []}
+    else:
+        tune = autotune.tune_batch(ctx, geos, m, probe_paths=max(1000, paths_per_struct // 20),
+                                   leaf_sizes=(2, 4, 8), chunks=(256, 1024), refills=(16,),
+                                   repeats=1, precision=PREC_FP32_RETEST,
+                                   log=lambda r: say("autotune", json.dumps(r)))
     t0 = time.perf_counter()
     b = Batch(ctx, geos, paths_per_struct, leaf_size=tune["leaf_size"], grid=64)
     t_build = time.perf_counter() - t0
@@ -500,11 +505,16 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
             acc[prec] += sums
             kern[prec].append(st["kernel_ms"])
         say("batch: precision %d, %d replays, %.1f ms per replay" % (prec, replays, np.mean(kern[prec])))
-    # the FP64 kernel itself on the first replay: its sums equal the filter's (same paths)
+    # the FP64 kernel itself on every 16th structure (1e4 paths each): its sums equal the
+    # filter's on the same paths, and its rate
+    sub = geos[::16]
+    bs = Batch(ctx, sub, 10000, leaf_size=tune["leaf_size"], grid=64)
+    bs.set_specularity(np.full(len(sub), 0.3))
     ctx.set_seed(5000)
-    s64, _, st64 = b.trace(m, precision=PREC_FP64, baseline=base, graph=True)
+    s64, _, st64 = bs.trace(m, precision=PREC_FP64)
     ctx.set_seed(5000)
-    s_f1, _, _ = b.trace(m, precision=PREC_FP32_FILTER, baseline=base, graph=True)
+    s_f1, _, _ = bs.trace(m, precision=PREC_FP32_FILTER)
+    bs.close()
     ctx.set_op_counting(True)
     ctx.set_seed(5000)
     _, _, cst = b.trace(m, precision=PREC_FP32_RETEST)
@@ -550,7 +560,8 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
         "completion_ms_p50": float(np.percentile(fin, 50)),
         "completion_ms_p99": float(np.percentile(fin, 99)),
         "fp32_filter_paths_per_s": paths_launch / (np.mean(kern[PREC_FP32_FILTER]) * 1e-3),
-        "fp64_paths_per_s": paths_launch / (st64["kernel_ms"] * 1e-3),
+        "fp64_paths_per_s": st64["paths"] / (st64["kernel_ms"] * 1e-3),
+        "fp64_sample": "%d structures (every 16th) x 1e4 paths" % len(sub),
         "fp64_sums_equal_filter_sums": bool(np.array_equal(s64[:, 0], s_f1[:, 0])
                                             and np.allclose(s64[:, 1], s_f1[:, 1], rtol=1e-12, atol=0)),
         "kappa_rel_err_vs_fp64_median": float(np.median(rel)),
@@ -624,6 +635,10 @@ def main():
     ap.add_argument("--batch-replays", type=int, default=1,
                     help="graph replays of the batch (fresh keys): 100 x 1e5 = the configured 1e7")
     ap.add_argument("--headline-only", action="store_true", help="skip every leg")
+    ap.add_argument("--only", choices=("film", "adjoint", "batch"), default=None,
+                    help="run one leg alone, without the headline (profiling runs)")
+    ap.add_argument("--batch-leaf", type=int, default=0,
+                    help="BVH leaf size of the batch leg (0: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
     args = ap.parse_args()
@@ -650,6 +665,16 @@ def main():
     cpu_leg = None if args.no_cpu else (max(1.0, args.cpu_seconds / 2), threads)
     npaths = int(args.paths)
     with Context(seed=2024, device=0) as ctx:
+        if args.only:
+            if args.only == "film":
+                res = film_leg(ctx, args.film_np, args.film_steps or args.steps, args.warmup, cpu_leg)
+            elif args.only == "adjoint":
+                res = adjoint_leg(ctx, int(args.adjoint_paths), cpu_leg)
+            else:
+                res = batch_leg(ctx, args.batch_structures, int(args.batch_paths),
+                                args.batch_replays, cpu_leg, leaf=args.batch_leaf)
+            print(json.dumps({"leg": args.only, "result": res}))
+            return
         head = headline(ctx, npaths, args.steps, args.warmup, int(args.count_paths),
                         int(args.compare_paths), cpu_head)
         film = adj = bat = None
@@ -660,7 +685,7 @@ def main():
                 say("adjoint: %.1f ms" % adj["adjoint_kernel_ms"])
             if args.batch_structures > 0:
                 bat = batch_leg(ctx, args.batch_structures, int(args.batch_paths),
-                                args.batch_replays, cpu_leg)
+                                args.batch_replays, cpu_leg, leaf=args.batch_leaf)
     dt = head.pop("dt")
     value = npaths * args.steps / dt
     cpu = head.pop("cpu_baseline")

@@ -4,14 +4,15 @@
     python -m ptrans.profiling counters <run_results.db> [kernel-substring]
     python -m ptrans.profiling traffic  <kt.db> <fetch.db> <write.db> <copy_fetch.db>
                                         <copy_write.db> <sq.db> <copy_mib> <paths> <workload>
-                                        [key=kernel-substring ...]
+                                        [key=kernel-substring ...] [tcc=<tcc.db>]
 
 'kernels' prints per-kernel call count and average/total duration; 'counters' prints,
 per kernel and counter, the mean value per dispatch.  'traffic' prints the JSON summary
 the bench's roofline reads: per trace-kernel precision its average duration (kernel
 trace), HBM bytes per launch (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the
 same counter reads for a device copy of copy_mib MiB) and VALU wave-instructions per
-launch and per path with the issue fraction over 256 CUs x 2.4 GHz.
+launch and per path with the issue fraction over 256 CUs x 2.4 GHz; with a TCC_HIT_sum /
+TCC_MISS_sum run (tcc=...) also the L2 hit rate.
 """
 from __future__ import annotations
 
@@ -49,7 +50,7 @@ TRACE_KERNELS = {"fp64": "trace_kernel<0, false, 0>", "fp32_filter": "trace_kern
 
 
 def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy_mib: float,
-                    paths: int, workload: str, kernels=None):
+                    paths: int, workload: str, kernels=None, tcc_db=None):
     copy_bytes = copy_mib * 2 ** 20
 
     def one(db, counter, sub):
@@ -76,15 +77,20 @@ def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy
             "hbm_fetch_bytes": fb, "hbm_write_bytes": wb, "hbm_bytes_per_launch": fb + wb,
             "valu_wave_instr_per_launch": v, "valu_wave_instr_per_path": v / paths,
             "valu_issue_fraction": v / (avg_us * 1e-6) / PEAK_VALU_WAVE_INSTR_PER_S}
+        if tcc_db:
+            _, hit = one(tcc_db, "TCC_HIT_sum", sub)
+            _, miss = one(tcc_db, "TCC_MISS_sum", sub)
+            out["kernels"][key]["l2_hit_rate"] = hit / max(1.0, hit + miss)
     return out
 
 
 def main(argv):
     if len(argv) >= 11 and argv[1] == "traffic":
         import json
-        kmap = dict(a.split("=", 1) for a in argv[11:]) or None
+        kv = dict(a.split("=", 1) for a in argv[11:])
+        tcc = kv.pop("tcc", None)
         print(json.dumps(traffic_summary(*argv[2:8], float(argv[8]), int(argv[9]), argv[10],
-                                         kmap), indent=1))
+                                         kv or None, tcc), indent=1))
         return 0
     if len(argv) < 3 or argv[1] not in ("kernels", "counters"):
         print(__doc__)
