@@ -1534,13 +1534,21 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
             loc_off.push_back((int32_t)loc_idx.size());
         }
     }
+    // per-XCD queues: contiguous structure ranges of about equal path counts
+    std::vector<int64_t> qoff(kXcdQueues + 1, 0);
+    qoff[kXcdQueues] = ntot_paths;
+    for (int q = 1, s = 0; q < kXcdQueues; ++q) {
+        const int64_t target = (int64_t)((double)ntot_paths * q / kXcdQueues);
+        while (s < nstruct && B->path_off[s] < target) ++s;
+        qoff[q] = std::max(qoff[q - 1], B->path_off[s]);
+    }
     std::vector<int32_t> group(ns, -1);
     for (size_t i = 6; i < ns; ++i) group[i] = 0;
     B->nm = 0;
     const size_t n = (size_t)nstruct;
     B->scratch_words = 16 + kBatchSums * n + n + 1 + (n + 1) / 2;
     hipStream_t st = c->stream;
-    int64_t *d_poff = nullptr;
+    int64_t *d_poff = nullptr, *d_qoff = nullptr;
     int32_t *d_root = nullptr, *d_prims = nullptr, *d_loff = nullptr, *d_lidx = nullptr;
     BvhNode *d_nodes = nullptr;
     double *d_base = nullptr;
@@ -1548,12 +1556,13 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     if ((rc = upload(B->path_off, &d_poff, st)) || (rc = upload(root, &d_root, st)) ||
         (rc = upload(nodes, &d_nodes, st)) || (rc = upload(prims, &d_prims, st)) ||
         (rc = upload(loc_off, &d_loff, st)) || (rc = upload(loc_idx, &d_lidx, st)) ||
-        (rc = upload(group, &B->d_group, st)) || (rc = upload(zeros, &d_base, st))) {
-        B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base};
+        (rc = upload(group, &B->d_group, st)) || (rc = upload(zeros, &d_base, st)) ||
+        (rc = upload(qoff, &d_qoff, st))) {
+        B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base, d_qoff};
         pt_batch_destroy(B);
         return rc;
     }
-    B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base};
+    B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base, d_qoff};
     DevBatch &bt = B->bt;
     bt.nstruct = nstruct;
     bt.grid = grid;
@@ -1564,6 +1573,7 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     bt.loc_off = d_loff;
     bt.loc_idx = d_lidx;
     bt.baseline = d_base;
+    bt.qoff = d_qoff;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         pt_batch_destroy(B);
@@ -1628,7 +1638,7 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     const size_t nm = (size_t)M->nb * M->nk;
     const size_t S = (size_t)kBatchSums * B->nstruct;
-    const size_t need = 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm;
+    const size_t need = 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm + kXcdQueues;
     if (B->scratch_words < need || !B->scratch) {
         if (B->scratch) PT_HIP(hipFree(B->scratch));
         B->scratch = nullptr;
@@ -1657,6 +1667,7 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
     P.bt = bt;
     P.mode_paths = w + 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2;
     P.mode_trans = P.mode_paths + nm;
+    P.bt.qctr = P.mode_trans + nm;
     P.adj = nullptr;
     *lds = adj_lds_bytes(P, c->threads);
     hipDeviceProp_t prop;

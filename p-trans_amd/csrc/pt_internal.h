@@ -114,6 +114,10 @@ static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
 // per-structure sums: n, sum x, sum x^2, sum (x - b) score, sum ((x - b) score)^2
 constexpr int kBatchSums = 5;
+// Batch work queues, one per XCD: each covers a contiguous range of structures, so the
+// L2 of an XCD serves the BVHs and facets of the few structures its waves trace at a
+// time; a wave whose own queue is empty takes chunks from the others.
+constexpr int kXcdQueues = 8;
 
 // A batch of structures merged into one mesh geometry (shared box grain, disjoint grain
 // sets): per structure its path range, start locator and tallies; per grain the BVH of
@@ -132,6 +136,8 @@ struct DevBatch {
     unsigned int *done;                // [nstruct] paths finished
     unsigned long long *finish;        // [nstruct] wall clock when the last path ended
     unsigned long long *start;         // wall clock at kernel start (one word)
+    const int64_t *qoff;               // [kXcdQueues + 1]: path range of each queue
+    unsigned long long *qctr;          // [kXcdQueues]: paths taken from each queue
 };
 
 // Largest number of specularity groups of one adjoint launch (each lane keeps one FP64

@@ -1142,8 +1142,12 @@ __device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, i
             const int nc = sw ? nd.child[1] : nd.child[0], fc = sw ? nd.child[0] : nd.child[1];
             const int nn = sw ? nd.count[1] : nd.count[0], fn = sw ? nd.count[0] : nd.count[1];
             const float dn = sw ? d1 : d0, df = sw ? d0 : d1;
-            if (dn < r * r && nc < 0) r = fminf(r, bvh_leaf_bound(g, bt, -nc - 1, nn, px, py, pz));
-            if (df < r * r && fc < 0) r = fminf(r, bvh_leaf_bound(g, bt, -fc - 1, fn, px, py, pz));
+            // a leaf's surfaces lie in its box: the distance to any of them is at least the
+            // larger of the box distance and the nearest of their planes
+            if (dn < r * r && nc < 0)
+                r = fminf(r, fmaxf(sqrtf(dn), bvh_leaf_bound(g, bt, -nc - 1, nn, px, py, pz)));
+            if (df < r * r && fc < 0)
+                r = fminf(r, fmaxf(sqrtf(df), bvh_leaf_bound(g, bt, -fc - 1, fn, px, py, pz)));
             if (df < r * r && fc >= 0 && sp < kBvhStack) stk[sp++ * 256] = fc;
             if (dn < r * r && nc >= 0 && sp < kBvhStack) stk[sp++ * 256] = nc;
         }
@@ -1660,6 +1664,10 @@ __device__ inline int64_t readlane_i64(int64_t v, int l)
 // Paths left in a wave after its queue ran dry: at most this many are served one at a
 // time by the whole wave (ball set-up, in-ball events, scan), the rest step together.
 constexpr int kFewPaths = 4;
+// A wave with a path beyond this many events stops taking new paths: its other lanes run
+// out, and the long random walk gets the few-path service (the whole wave on one path)
+// while the other waves go on with the queue, instead of waiting for the end of the launch.
+constexpr int64_t kDrainEvents = 20000;
 
 // One path of a wave down to a few, whose next free flight stays inside its safety ball
 // (the tail of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An
@@ -1942,11 +1950,36 @@ trace_kernel(TraceParams P)
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
 
+    constexpr bool kService = kPrec != kPrecFP64 && kMode != TRACE_GRID;
     while (true) {
         const unsigned long long need = __ballot(!active);
+        const bool drain = kService && __any(active && ph.events >= kDrainEvents);
         // refill in batches: start_path is long, so issue it for many lanes at once
-        if (need != 0ull && !exhausted && (__popcll(need) >= P.refill_min || need == ~0ull)) {
-            if (chunk_next >= chunk_end) {
+        if (need != 0ull && !exhausted && !drain &&
+            (__popcll(need) >= P.refill_min || need == ~0ull)) {
+            if (chunk_next >= chunk_end && kBatch) {
+                // the queue of this wave's XCD first (hardware register XCC_ID), then the others
+                long long b0 = -1, e0 = -1;
+                if (lane == 0) {
+                    const int home = __builtin_amdgcn_s_getreg(20 | (3 << 11)) & (kXcdQueues - 1);
+                    for (int t = 0; t < kXcdQueues; ++t) {
+                        const int q = (home + t) & (kXcdQueues - 1);
+                        const long long qs = P.bt.qoff[q], qe = P.bt.qoff[q + 1];
+                        if (qs >= qe) continue;
+                        const unsigned long long b = atomicAdd(P.bt.qctr + q, (unsigned long long)P.chunk);
+                        if ((long long)b < qe - qs) {
+                            b0 = qs + (long long)b;
+                            e0 = b0 + P.chunk < qe ? b0 + P.chunk : qe;
+                            break;
+                        }
+                    }
+                }
+                b0 = __shfl(b0, 0, 64);
+                e0 = __shfl(e0, 0, 64);
+                chunk_next = uniform64(b0 < 0 ? P.total : b0);
+                chunk_end = uniform64(e0 < 0 ? P.total : e0);
+                if (b0 < 0) exhausted = true;
+            } else if (chunk_next >= chunk_end) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(P.queue, (unsigned long long)P.chunk);
                 base = __shfl(base, 0, 64);
@@ -1981,10 +2014,11 @@ trace_kernel(TraceParams P)
         }
         int pre_best = -2;
         double pre_dmin = 0.0;
-        if (kPrec != kPrecFP64 && kMode != TRACE_GRID && exhausted) {
+        if (kService && (exhausted || drain)) {
             // at most kFewPaths paths left in the wave after the queue ran dry (the launch's
-            // tail): for each in turn, the whole wave sets up its ball, runs its in-ball
-            // events, and scans its next event; the event itself then runs in step_path
+            // tail) or while it drains for a long walk: for each in turn, the whole wave sets
+            // up its ball, runs its in-ball events, and scans its next event; the event itself
+            // then runs in step_path
             unsigned long long am = __ballot(active && ph.end_side == 0);
             if (am != 0ull && __popcll(am) <= kFewPaths) {
                 while (am != 0ull) {
