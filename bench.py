@@ -545,6 +545,11 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     alg_bytes = 32.0 + 32.0 * nodes + 40.0 * tests
     achieved_gbs = alg_bytes * paths_launch / (kms * 1e-3) / 1e9
     prof = load_profile("trace_kernel_batch_traffic.json").get("kernels", {}).get("fp32_trace", {})
+    traffic = None
+    if prof.get("hbm_bytes_per_launch") and prof.get("valu_wave_instr_per_path"):
+        # the profile's launch may hold another path count: scale its bytes per path
+        prof_paths = prof["valu_wave_instr_per_launch"] / prof["valu_wave_instr_per_path"]
+        traffic = prof["hbm_bytes_per_launch"] / prof_paths * paths_launch
     out = {
         "workload": "%d jittered 4x4x4 simple-cubic grain structures (1000 nm cube, +-5 nm "
                     "jitter, Gamma law, p = 0.3), %d - %d triangles each, 3x1000 modes, "
@@ -580,7 +585,8 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
         "kappa_gradient_evals_per_s": rate / n2s if n2s else None,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": prof.get("hbm_bytes_per_launch"),
+                     "traffic": traffic,
+                     "traffic_bytes_per_path_profiled": traffic / paths_launch if traffic else None,
                      "algorithmic_bytes_per_path": alg_bytes, "bvh_nodes_per_path": nodes,
                      "leaf_tests_per_path": tests,
                      "valu_issue_fraction_profiled": prof.get("valu_issue_fraction"),
