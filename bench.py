@@ -474,7 +474,8 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     m = ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
     if leaf > 0:            # fixed setting (profiling runs): no tuning launches
         ctx.set_tuning(256, 16, 0)
-        tune = {"leaf_size": leaf, "chunk": 256, "refill_min": 16, "blocks_per_cu": 0, "table": []}
+        tune = {"leaf_size": leaf, "top_levels": 1, "chunk": 256, "refill_min": 16,
+                "blocks_per_cu": 0, "table": []}
     else:
         tune = autotune.tune_batch(ctx, geos, m, probe_paths=max(1000, paths_per_struct // 20),
                                    leaf_sizes=(2, 4, 8), chunks=(256, 1024), refills=(16,),
@@ -482,6 +483,7 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
                                    log=lambda r: say("autotune", json.dumps(r)))
     t0 = time.perf_counter()
     b = Batch(ctx, geos, paths_per_struct, leaf_size=tune["leaf_size"], grid=64)
+    b.set_top_levels(tune.get("top_levels", 0))
     t_build = time.perf_counter() - t0
     info = b.info()
     say("batch: built", json.dumps(info), "in %.1f s" % t_build)
@@ -509,6 +511,7 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     # filter's on the same paths, and its rate
     sub = geos[::16]
     bs = Batch(ctx, sub, 10000, leaf_size=tune["leaf_size"], grid=64)
+    bs.set_top_levels(tune.get("top_levels", 0))
     bs.set_specularity(np.full(len(sub), 0.3))
     ctx.set_seed(5000)
     s64, _, st64 = bs.trace(m, precision=PREC_FP64)

@@ -401,6 +401,12 @@ int pt_batch_info(const pt_batch *batch, int64_t *info);
 /* every facet of structure s gets specularity mean[s] (0 < mean[s] < 1); box walls keep
  * theirs.  The gradient of pt_batch_trace is with respect to these values. */
 int pt_batch_set_specularity(pt_batch *batch, const double *mean);
+/* Top BVH levels staged in each workgroup's LDS (0: none, at most 3; a tuning knob, no
+ * reference counterpart): a workgroup copies the top `levels` levels of the grain BVHs of
+ * the structure its waves are tracing into LDS, and its scans of that structure read those
+ * nodes there.  Results do not depend on it; the LDS it takes (2^levels - 1 nodes of 144
+ * bytes per grain of the largest structure) can lower the workgroups per CU. */
+int pt_batch_set_top_levels(pt_batch *batch, int32_t levels);
 /*
  * One launch over the whole batch with the context's seed.  Per structure s, adds to
  * sums[5 s .. 5 s + 4]: paths, sum x, sum x^2, sum (x - baseline[s]) * score_s and the sum

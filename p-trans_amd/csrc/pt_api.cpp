@@ -1233,6 +1233,7 @@ struct pt_batch {
     pt_ctx *ctx = nullptr;
     pt_geom *geom = nullptr;               // the merged geometry
     int32_t nstruct = 0, grid = 0, leaf = 0, depth = 0;
+    int32_t top_cap = 0;                   // most grains of one structure
     int64_t npaths = 0;
     std::vector<int64_t> path_off;
     std::vector<int32_t> struct_surf_begin; // merged surface range of each structure
@@ -1513,6 +1514,46 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     }
     B->nodes = (int64_t)nodes.size();
     B->prims = (int64_t)prims.size();
+    if (B->nodes >= kLdsRef) {
+        pt_batch_destroy(B);
+        return fail(PT_E_UNSUPPORTED, "pt_batch_create: %lld BVH nodes (at most %d)",
+                    (long long)nodes.size(), kLdsRef - 1);
+    }
+    // image of the top BVH levels for LDS staging: per grain the first kTopImage heap
+    // slots (root, its children, theirs) as stored (global child references); a grain
+    // without candidates gets a root with two empty leaves
+    std::vector<BvhNode> top((size_t)(ng + 1) * kTopImage, BvhNode{});
+    for (int g = 1; g <= ng; ++g) {
+        BvhNode *h = &top[(size_t)g * kTopImage];
+        if (root[g] < 0) {
+            for (int c = 0; c < 2; ++c) {
+                h[0].child[c] = -1;
+                h[0].count[c] = 0;
+                for (int k = 0; k < 3; ++k) {
+                    h[0].lo[c][k] = 3.0e38f;
+                    h[0].hi[c][k] = -3.0e38f;
+                    h[0].nlo[c][k] = -1.0e30f;
+                    h[0].nhi[c][k] = 1.0e30f;
+                }
+            }
+            continue;
+        }
+        std::vector<int32_t> ref(kTopImage, -1);
+        ref[0] = root[g];
+        for (int j = 0; j < kTopImage; ++j) {
+            if (ref[j] < 0) continue;
+            h[j] = nodes[(size_t)ref[j]];
+            for (int c = 0; c < 2; ++c)
+                if (2 * j + 1 + c < kTopImage && h[j].child[c] >= 0) ref[2 * j + 1 + c] = h[j].child[c];
+        }
+    }
+    std::vector<int32_t> sgrain(nstruct + 1, 1);
+    int max_ng = 0;
+    for (int s = 0; s < nstruct; ++s) {
+        sgrain[s + 1] = sgrain[s] + descs[s]->ngrain;
+        max_ng = std::max(max_ng, descs[s]->ngrain);
+    }
+    B->top_cap = max_ng;
     // start locator: per structure, incident polygons overlapping each (y, z) cell
     std::vector<int32_t> loc_off(1, 0), loc_idx;
     for (int s = 0; s < nstruct; ++s) {
@@ -1550,19 +1591,23 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     hipStream_t st = c->stream;
     int64_t *d_poff = nullptr, *d_qoff = nullptr;
     int32_t *d_root = nullptr, *d_prims = nullptr, *d_loff = nullptr, *d_lidx = nullptr;
-    BvhNode *d_nodes = nullptr;
+    int32_t *d_sgrain = nullptr;
+    BvhNode *d_nodes = nullptr, *d_top = nullptr;
     double *d_base = nullptr;
     std::vector<double> zeros(n, 0.0);
     if ((rc = upload(B->path_off, &d_poff, st)) || (rc = upload(root, &d_root, st)) ||
         (rc = upload(nodes, &d_nodes, st)) || (rc = upload(prims, &d_prims, st)) ||
         (rc = upload(loc_off, &d_loff, st)) || (rc = upload(loc_idx, &d_lidx, st)) ||
         (rc = upload(group, &B->d_group, st)) || (rc = upload(zeros, &d_base, st)) ||
-        (rc = upload(qoff, &d_qoff, st))) {
-        B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base, d_qoff};
+        (rc = upload(qoff, &d_qoff, st)) || (rc = upload(top, &d_top, st)) ||
+        (rc = upload(sgrain, &d_sgrain, st))) {
+        B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base, d_qoff,
+                    d_top, d_sgrain};
         pt_batch_destroy(B);
         return rc;
     }
-    B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base, d_qoff};
+    B->owned = {d_poff, d_root, d_nodes, d_prims, d_loff, d_lidx, B->d_group, d_base, d_qoff,
+                d_top, d_sgrain};
     DevBatch &bt = B->bt;
     bt.nstruct = nstruct;
     bt.grid = grid;
@@ -1574,6 +1619,10 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     bt.loc_idx = d_lidx;
     bt.baseline = d_base;
     bt.qoff = d_qoff;
+    bt.top = d_top;
+    bt.sgrain = d_sgrain;
+    bt.top_k = 0;
+    bt.top_cap = B->top_cap;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         pt_batch_destroy(B);
@@ -1621,6 +1670,15 @@ int pt_batch_set_specularity(pt_batch *B, const double *p)
         for (int i = B->struct_surf_begin[s]; i < B->struct_surf_begin[s + 1]; ++i) all[i] = p[s];
     }
     return pt_geom_set_specularity(B->geom, all.data());
+}
+
+int pt_batch_set_top_levels(pt_batch *B, int32_t levels)
+{
+    if (!B) return fail(PT_E_INVALID, "pt_batch_set_top_levels: NULL batch");
+    if (levels < 0 || levels > kMaxTopLevels)
+        return fail(PT_E_INVALID, "pt_batch_set_top_levels: levels must be in 0..%d", kMaxTopLevels);
+    B->bt.top_k = (1 << levels) - 1;
+    return PT_OK;
 }
 
 }  // extern "C"
@@ -1672,9 +1730,9 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
     *lds = adj_lds_bytes(P, c->threads);
     hipDeviceProp_t prop;
     PT_HIP(hipGetDeviceProperties(&prop, c->device));
-    if (*lds + kTraceStaticLds > prop.sharedMemPerBlock)
-        return fail(PT_E_UNSUPPORTED, "batch of %d structures needs %zu bytes of LDS per workgroup",
-                    B->nstruct, *lds + kTraceStaticLds);
+    if (*lds + trace_static_lds(TRACE_BATCH) > prop.sharedMemPerBlock)
+        return fail(PT_E_UNSUPPORTED, "batch of %d structures needs %zu bytes of LDS per workgroup "
+                    "(fewer top BVH levels in LDS)", B->nstruct, *lds + trace_static_lds(TRACE_BATCH));
     PT_HIP(launch_blocks(c, TRACE_BATCH, precision, *lds, blocks));
     return PT_OK;
 }

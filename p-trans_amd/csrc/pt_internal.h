@@ -112,6 +112,33 @@ struct BvhNode {
 };
 static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
+
+// Top levels of the grain BVHs of one structure staged in a workgroup's LDS.  A node
+// reference r >= kLdsRef names LDS slot r - kLdsRef (grain i of the structure, heap
+// position j of its top levels: slot i * K + j, K = 2^levels - 1); below it, a global
+// node index.  Slots are 144 bytes apart, so that lanes reading the roots of different
+// grains spread over the LDS banks.
+constexpr int32_t kLdsRef = 1 << 30;
+constexpr int kMaxTopLevels = 3;
+// heap slots per grain in the device image the LDS copies are made from
+constexpr int kTopImage = (1 << kMaxTopLevels) - 1;
+// path flag (internal, never reported): the path reads the LDS image of its structure
+constexpr int kPathTop = 1 << 30;
+// a path bound to the LDS image leaves it after this many events (a long random walk)
+constexpr int64_t kTopUnbindEvents = 2000;
+struct TopSlot {
+    BvhNode n;
+    int32_t pad[4];
+};
+static_assert(sizeof(TopSlot) == 144, "TopSlot layout");
+// Header of the LDS image: word = bound lanes (bits 0-15) | structure (bits 16-30; kTopNone:
+// none) | being staged (bit 31); the structure's first global grain and its grains staged.
+struct TopHdr {
+    unsigned int word;
+    int32_t g0, ng, pad;
+};
+constexpr unsigned int kTopNone = 0x7fffu;
+constexpr unsigned int kTopBusy = 0x80000000u;
 // per-structure sums: n, sum x, sum x^2, sum (x - b) score, sum ((x - b) score)^2
 constexpr int kBatchSums = 5;
 // Batch work queues, one per XCD: each covers a contiguous range of structures, so the
@@ -138,6 +165,13 @@ struct DevBatch {
     unsigned long long *start;         // wall clock at kernel start (one word)
     const int64_t *qoff;               // [kXcdQueues + 1]: path range of each queue
     unsigned long long *qctr;          // [kXcdQueues]: paths taken from each queue
+    // LDS staging of the top BVH levels (top_k = 0: off): per global grain its top_k
+    // nodes in heap order with LDS child references, [ngrain_total * top_k]; the first
+    // global grain of each structure, [nstruct + 1]; grains staged per structure
+    const BvhNode *top;
+    const int32_t *sgrain;
+    int32_t top_k;
+    int32_t top_cap;
 };
 
 // Largest number of specularity groups of one adjoint launch (each lane keeps one FP64
@@ -180,18 +214,35 @@ struct TraceParams {
 };
 
 __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
-// Static LDS of trace_kernel: eight per-lane event counters of a 256-lane workgroup.
-constexpr size_t kTraceStaticLds = 8 * 256 * sizeof(unsigned long long);
+// Static LDS of trace_kernel: per-lane event counters of a 256-lane workgroup, eight of
+// them (the batch kernel, which has no FP64 retests or scan fallbacks: six).
+constexpr int trace_lane_stats(int mode) { return mode == TRACE_BATCH ? 6 : 8; }
+constexpr size_t trace_static_lds(int mode)
+{
+    return (size_t)trace_lane_stats(mode) * 256 * sizeof(unsigned long long);
+}
+constexpr size_t kTraceStaticLds = trace_static_lds(TRACE_GRID);
 // Largest event limit: a path draws at most 7 numbers before its first event and 6 per
 // event (specular decision, two direction draws, the transmission draw, the nmaterial > 1
 // draw of RT:2399 and a new free path), and its draw counter is 32 bits wide.
 constexpr int64_t kMaxEventLimit = (((int64_t)1 << 32) - 8) / 6;
 // Dynamic LDS of trace_kernel (the static kTraceStaticLds comes on top)
+// Layout: [ngroup * threads scores | nbatch * adj_stride sums | 2 * ngt grain tallies]
+// (doubles), then for batches, from a 16-byte boundary, [kBvhStack * threads stack
+// entries | TopHdr | top_k * top_cap TopSlots].
+__host__ __device__ inline size_t adj_lds_doubles(const TraceParams &p, int threads)
+{
+    return (size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt;
+}
+__host__ __device__ inline size_t batch_lds_offset(const TraceParams &p, int threads)
+{
+    return (adj_lds_doubles(p, threads) * sizeof(double) + 15) & ~(size_t)15;
+}
 inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 {
-    return ((size_t)p.ngroup * threads + (size_t)p.nbatch * adj_stride(p) + 2 * (size_t)p.ngt) *
-               sizeof(double) +
-           (p.bt.nstruct ? (size_t)kBvhStack * threads * sizeof(int32_t) : 0);
+    if (!p.bt.nstruct) return adj_lds_doubles(p, threads) * sizeof(double);
+    return batch_lds_offset(p, threads) + (size_t)kBvhStack * threads * sizeof(int32_t) +
+           (p.bt.top_k ? sizeof(TopHdr) + (size_t)p.bt.top_k * p.bt.top_cap * sizeof(TopSlot) : 0);
 }
 
 // kernels' launch wrappers (pt_trace.hip)

@@ -919,6 +919,105 @@ __device__ inline float box_dist2(const float *lo, const float *hi, float px, fl
     return ex * ex + ey * ey + ez * ez;
 }
 
+// The LDS image of the top BVH levels sits after the lanes' stacks (stk: this lane's stack
+// base, entries 256 apart).
+__device__ inline TopHdr *top_hdr(int *stk)
+{
+    return (TopHdr *)(stk - threadIdx.x + kBvhStack * 256);
+}
+
+// Root reference of the path's grain BVH: the LDS copy when the path is bound to the
+// structure staged in its workgroup's LDS (kPathTop), else the global root (-1: none).
+__device__ inline int bvh_root_ref(const DevBatch &bt, const Path &ph, int *stk)
+{
+    if (ph.flags & kPathTop) {
+        const TopHdr *h = top_hdr(stk);
+        const int li = ph.which - h->g0;
+        if (li >= 0 && li < h->ng) return kLdsRef + li * bt.top_k;
+    }
+    return ph.which >= 0 ? bt.grain_root[ph.which] : -1;
+}
+
+// A node by reference: an LDS slot or a global node (one generic load either way).
+__device__ inline BvhNode bvh_node(const DevBatch &bt, int r, int *stk)
+{
+    const BvhNode *p = r >= kLdsRef ? &((const TopSlot *)(top_hdr(stk) + 1))[r - kLdsRef].n
+                                    : bt.nodes + r;
+    return *p;
+}
+
+// Wave-uniform, all lanes active: binds the lanes that have just started a path (fresh) to
+// the workgroup's LDS image of the top BVH levels of their structure.  The image is
+// restaged for the structure of the wave's first fresh path when no lane is bound to the
+// structure it holds (the header word's count is zero): the wave claims the word (busy
+// bit), copies the grains' first top_k heap slots with the child references inside the
+// staged levels turned into LDS references, and publishes the new structure.  Fresh lanes
+// of the staged structure then add themselves to the count; the others, and every lane
+// when the count of another structure is not zero, read global nodes.  A bound lane's
+// image cannot change: staging needs a zero count.
+__device__ void top_bind(const DevBatch &bt, int *stk, Path &ph, bool fresh)
+{
+    const unsigned long long fm = __ballot(fresh);
+    if (fm == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    TopHdr *h = top_hdr(stk);
+    const int s = __builtin_amdgcn_readlane(ph.sid, (int)__builtin_ctzll(fm));
+    const unsigned int tag = (unsigned int)s << 16;
+    int claimed = 0;
+    if (lane == 0) {
+        unsigned int w = __hip_atomic_load(&h->word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((w & ~0xffffu) != tag && (w & 0xffffu) == 0u && !(w & kTopBusy))
+            claimed = atomicCAS(&h->word, w, kTopBusy | tag) == w;
+    }
+    claimed = __shfl(claimed, 0, 64);
+    if (claimed) {
+        const int K = bt.top_k;
+        const int g0 = bt.sgrain[s];
+        const int ng = min(bt.sgrain[s + 1] - g0, bt.top_cap);
+        TopSlot *ls = (TopSlot *)(h + 1);
+        for (int i = lane; i < ng * K * 8; i += 64) {
+            const int slot = i >> 3, part = i & 7;
+            const int gi = slot / K, j = slot - gi * K;
+            int4 v = ((const int4 *)(bt.top + ((size_t)(g0 + gi) * kTopImage + j)))[part];
+            if (part == 3) {
+                // child[0], child[1]: internal children inside the staged levels -> LDS
+                if (v.x >= 0 && 2 * j + 1 < K) v.x = kLdsRef + gi * K + 2 * j + 1;
+                if (v.y >= 0 && 2 * j + 2 < K) v.y = kLdsRef + gi * K + 2 * j + 2;
+            }
+            ((int4 *)&ls[slot])[part] = v;
+        }
+        if (lane == 0) {
+            h->g0 = g0;
+            h->ng = ng;
+            __hip_atomic_store(&h->word, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    const bool mine = fresh && ph.sid == s;
+    const unsigned long long bm = __ballot(mine);
+    int got = 0;
+    if (lane == 0 && bm != 0ull) {
+        unsigned int w = __hip_atomic_load(&h->word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        while (!(w & kTopBusy) && (w & ~0xffffu) == tag) {
+            const unsigned int prev = atomicCAS(&h->word, w, w + (unsigned int)__popcll(bm));
+            if (prev == w) {
+                got = 1;
+                break;
+            }
+            w = prev;
+        }
+    }
+    got = __shfl(got, 0, 64);
+    if (got && mine) ph.flags |= kPathTop;
+}
+
+// Leaves the LDS image (a path that ends, or a long random walk that would keep the
+// image from being restaged).
+__device__ inline void top_unbind(int *stk, Path &ph)
+{
+    atomicSub(&top_hdr(stk)->word, 1u);
+    ph.flags &= ~kPathTop;
+}
+
 // findCollisionSurface (RT:1989-2096) for batch structures: the three facing box walls,
 // then the current grain's candidates through its BVH.  Node tests are FP32 slab tests
 // against host-padded bounds (they only prune); surfaces are tested by the reference's
@@ -942,13 +1041,13 @@ __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, d
             hit = true;
         }
     }
-    const int root = ph.which >= 0 ? bt.grain_root[ph.which] : -1;
+    const int root = bvh_root_ref(bt, ph, stk);
     if (root >= 0) {
         const RayF32 rf = ray_f32(ph.pos, ph.dir);
         int sp = 1;
         stk[0] = root;
         while (sp > 0) {
-            const BvhNode nd = bt.nodes[stk[--sp * 256]];
+            const BvhNode nd = bvh_node(bt, stk[--sp * 256], stk);
             ++oc.nodes;
             float t0, t1;
             bool g0 = slab_enter(rf, nd.lo[0], nd.hi[0], &t0) && may_face(rf, nd.nlo[0], nd.nhi[0]);
@@ -1129,12 +1228,12 @@ __device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, i
     float r = fminf(fminf(fminf(px, (float)g.box[0] - px), fminf(py, (float)g.box[1] - py)),
                     fminf(pz, (float)g.box[2] - pz));
     r = fmaxf(r, 0.0f);
-    const int root = ph.which >= 0 ? bt.grain_root[ph.which] : -1;
+    const int root = bvh_root_ref(bt, ph, stk);
     if (root >= 0) {
         int sp = 1;
         stk[0] = root;
         while (sp > 0) {
-            const BvhNode nd = bt.nodes[stk[--sp * 256]];
+            const BvhNode nd = bvh_node(bt, stk[--sp * 256], stk);
             ++oc.nodes;
             const float d0 = box_dist2(nd.lo[0], nd.hi[0], px, py, pz);
             const float d1 = box_dist2(nd.lo[1], nd.hi[1], px, py, pz);
@@ -1913,9 +2012,10 @@ trace_kernel(TraceParams P)
     extern __shared__ double adj_lds[];
     // per-lane statistics live in LDS, not in registers held through the whole trace:
     // paths, events, draws, lost, start mismatch, event limit, FP64 retests, fallbacks
-    __shared__ unsigned long long lane_stats[8][256];
-    static_assert(sizeof(lane_stats) == kTraceStaticLds, "static LDS of the trace kernel");
-    for (int k = 0; k < 8; ++k) lane_stats[k][threadIdx.x] = 0ull;
+    constexpr int kStats = trace_lane_stats(kMode);
+    __shared__ unsigned long long lane_stats[kStats][256];
+    static_assert(sizeof(lane_stats) == trace_static_lds(kMode), "static LDS of the trace kernel");
+    for (int k = 0; k < kStats; ++k) lane_stats[k][threadIdx.x] = 0ull;
     const int lane = threadIdx.x & 63;
     AdjLane al;
     al.stride = blockDim.x;
@@ -1931,11 +2031,14 @@ trace_kernel(TraceParams P)
     // batch: the lane's BVH stack (per-structure sums go straight to global atomics: a
     // path end is rare next to its events, and the LDS they would take costs a workgroup
     // per CU)
-    int *stk = (int *)(adj_acc + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0)) + threadIdx.x;
+    int *stk = (int *)((char *)adj_lds + batch_lds_offset(P, blockDim.x)) + threadIdx.x;
+    // batch with LDS staging: the image is empty until a wave stages a structure
+    const bool top_on = kBatch && P.bt.top_k > 0;
     if (kAdj) {
         const int nlds = P.ngroup * blockDim.x + P.nbatch * adj_k + (P.grain_tally ? 2 * P.ngt : 0);
         for (int i = threadIdx.x; i < nlds; i += blockDim.x) adj_lds[i] = 0.0;
         if (kBatch && blockIdx.x == 0 && threadIdx.x == 0) *P.bt.start = wall_clock64();
+        if (top_on && threadIdx.x == 0) top_hdr(stk)->word = kTopNone << 16;
         __syncthreads();
     }
     unsigned long long *st = &lane_stats[0][threadIdx.x];   // counter k at st[256 * k]
@@ -1991,12 +2094,14 @@ trace_kernel(TraceParams P)
                 const int64_t avail = chunk_end - chunk_next;
                 const int rank = __popcll(need & ((1ull << lane) - 1ull));
                 const int nneed = __popcll(need);
+                bool fresh = false;
                 if (!active && rank < avail) {
                     pid = chunk_next + rank;
                     al.nscat = 0;
                     al.len = 0.0;
                     if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
                         active = true;
+                        fresh = true;
                         if (kAdj & 1) al.fw = 1.0 / P.m.mfp[mode];
                     } else {
                         if (is_grid_mode(kMode)) P.outcome[pid] = OUT_MISMATCH;
@@ -2006,12 +2111,15 @@ trace_kernel(TraceParams P)
                     }
                 }
                 chunk_next += (avail < nneed ? avail : nneed);
+                if (top_on) top_bind(P.bt, stk, ph, fresh);
             }
         }
         if (!__any(active)) {
             if (exhausted) break;
             continue;
         }
+        if (top_on && active && (ph.flags & kPathTop) && ph.events >= kTopUnbindEvents)
+            top_unbind(stk, ph);
         int pre_best = -2;
         double pre_dmin = 0.0;
         if (kService && (exhausted || drain)) {
@@ -2106,6 +2214,7 @@ trace_kernel(TraceParams P)
                 st[256 * 2] += rng.draw;
                 if (ph.flags & PT_PATH_LOST) st[256 * 3] += 1ull;
                 if (ph.flags & PT_PATH_EVENT_LIMIT) st[256 * 5] += 1ull;
+                if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
                 active = false;
             }
         }
@@ -2120,7 +2229,7 @@ trace_kernel(TraceParams P)
     }
     // stats slots: paths, events, draws, lost, mismatch, limit -> stats[0..5];
     // retests, fallbacks -> stats[10], stats[11]
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < kStats; ++k) {
         const unsigned long long v = wave_sum(st[256 * k]);
         if (lane == 0) atomicAdd(P.stats + (k < 6 ? k : k + 4), v);
     }

@@ -24,6 +24,7 @@ EXPORTS = (
     "pt_trace_alias_noalloc", "pt_replay_alias",
     "pt_adjoint_alias", "pt_trace_alias_grains",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_info", "pt_batch_set_specularity",
+    "pt_batch_set_top_levels",
     "pt_batch_trace", "pt_batch_trace_graph",
 )
 
@@ -93,6 +94,7 @@ def load():
     L.pt_batch_destroy.argtypes = [vp]
     L.pt_batch_info.argtypes = [vp, vp]
     L.pt_batch_set_specularity.argtypes = [vp, vp]
+    L.pt_batch_set_top_levels.argtypes = [vp, i32]
     L.pt_batch_trace.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     L.pt_batch_trace_graph.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp]
     L.pt_philox_fill.argtypes = [u64, u64, u64, i64, vp]

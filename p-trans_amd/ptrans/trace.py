@@ -285,6 +285,11 @@ class Batch:
         m = np.ascontiguousarray(mean, dtype=np.float64)
         _lib.check(_lib.load().pt_batch_set_specularity(self.handle, m.ctypes.data))
 
+    def set_top_levels(self, levels: int):
+        """Top BVH levels each workgroup stages in LDS for the structure it traces (0: none,
+        at most 3).  Results do not depend on it."""
+        _lib.check(_lib.load().pt_batch_set_top_levels(self.handle, int(levels)))
+
     def trace(self, modes: Modes, precision: int = PREC_FP64, baseline=None, graph: bool = False):
         """One launch: per-structure sums (nstruct, 5) of [paths, sum x, sum x^2,
         sum (x - baseline) score, sum ((x - baseline) score)^2], completion times (ms) and

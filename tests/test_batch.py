@@ -228,3 +228,61 @@ def test_gpu_batch_at_c5_sizes_equals_brute_force_oracle(gpu_ctx):
                 assert sums[s, 0] == n, (leaf, prec, s, sums[s, 0], n)
                 assert abs(sums[s, 1] - sx) <= 1e-12 * max(sx, 1e-300), (leaf, prec, s, sums[s, 1], sx)
         b.close()
+
+
+_TOP_REF = {}        # oracle sums of the LDS-levels cases, shared by the parameters
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("levels", [1, 2, 3])
+def test_gpu_batch_lds_top_levels_equal_brute_force_oracle(gpu_ctx, levels):
+    """Top BVH levels staged in LDS (restaged as the waves move from structure to
+    structure): C5-size structures (1,920 / 12,000 / 94,080 triangles) and twelve small
+    lattices of 400 paths each (many restagings), ballistic-leaning and diffusive (safety
+    balls found through the BVH) mode tables, in FP64, the FP32 filter and the FP32 trace:
+    per-structure path counts and sum x equal the oracle's brute-force scan of each
+    structure alone."""
+    from test_alias import _weights
+    from ptrans.anphon import synthetic_table
+    from ptrans.trace import Batch
+    t = synthetic_table(2, 4, 9, mfp_min_nm=10.0, mfp_max_nm=40.0)
+    omega_d = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    prob_d, alias_d = O.build_alias(_weights(omega_d, t.vel, t.mfp_nm, t.weight))
+    tables = {"ballistic": _modes(),
+              "diffusive": (omega_d, t.vel, t.mfp_nm, t.weight, prob_d, alias_d)}
+    c5 = [gen.sc_lattice_flat(seed=400 + m, k=4, m=m, jitter=5.0, trans_kind=2,
+                              trans_param=2.0, p=0.3) for m in (2, 5, 14)]
+    cases = [("c5", c5, [1500, 1500, 1500], ("ballistic",)),
+             ("small", _lattices() * 4, [400] * 12, ("ballistic", "diffusive"))]
+    for name, geos, npaths, tnames in cases:
+        b = Batch(gpu_ctx, geos, npaths, leaf_size=2, grid=32)
+        b.set_top_levels(levels)
+        for tname in tnames:
+            mods = tables[tname]
+            omega, vel, mfp, weight, prob, alias = mods
+            m = gpu_ctx.modes(omega=omega, vel=vel, mfp=mfp, weight=weight)
+            key = (name, tname)
+            if key not in _TOP_REF:
+                _TOP_REF[key] = [_oracle_struct_sums(geo, mods, 43, b.path_off[s], npaths[s])
+                                 for s, geo in enumerate(geos)]
+            ref = _TOP_REF[key]
+            gpu_ctx.set_seed(43)
+            for prec in (0, 2):
+                print("levels %d, %s, %s modes, precision %d" % (levels, name, tname, prec),
+                      flush=True)
+                sums, fin, st = b.trace(m, precision=prec, graph=prec == 2)
+                assert st["paths"] == sum(npaths)
+                for s in range(len(geos)):
+                    n, sx, nt = ref[s]
+                    assert sums[s, 0] == n, (name, tname, prec, s, sums[s, 0], n)
+                    assert abs(sums[s, 1] - sx) <= 1e-12 * max(sx, 1e-300), (name, tname, prec, s)
+            # the FP32 trace (not bit-identical to FP64) gives the same paths with the image
+            # as without it
+            b.set_top_levels(0)
+            plain, _, _ = b.trace(m, precision=1)
+            b.set_top_levels(levels)
+            staged, _, _ = b.trace(m, precision=1, graph=True)
+            np.testing.assert_array_equal(staged[:, 0], plain[:, 0])
+            np.testing.assert_allclose(staged[:, :3], plain[:, :3], rtol=1e-12, atol=1e-300)
+            m.close()
+        b.close()
