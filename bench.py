@@ -406,6 +406,12 @@ def adjoint_leg(ctx, npaths: int, cpu, table_seed: int = 20):
                                         baseline=base, nbatch=64, precision=FAST)
     t_adj = time.perf_counter() - t0
     gr = post.alias_gradients(sums, length, ngroup)
+    # the same adjoint at ten times the paths per launch: the tail of the few longest random
+    # walks (which sets a 1e8-path launch) amortised over more paths
+    big = 10 * npaths
+    ctx.set_seed(779)
+    _, _, _, bst = ctx.adjoint_alias(g, m, big, surface_group=group, ngroup=ngroup,
+                                     baseline=base, nbatch=64, precision=FAST)
     ctx.set_seed(777)
     mp64, mt64, st64 = ctx.trace_alias(g, m, npaths, precision=PREC_FP64)
     k64 = post.alias_kappa(mp64, mt64, t.mfp_nm, length)
@@ -422,6 +428,8 @@ def adjoint_leg(ctx, npaths: int, cpu, table_seed: int = 20):
         "adjoint_paths_per_s": adj_rate,
         "adjoint_kernel_ms": ast["kernel_ms"],
         "adjoint_cost_vs_forward": ast["kernel_ms"] / fst["kernel_ms"],
+        "adjoint_paths_per_s_at_10x_paths": big / (bst["kernel_ms"] * 1e-3),
+        "adjoint_kernel_ms_at_10x_paths": bst["kernel_ms"],
         "adjoint_wall_vs_forward_wall": t_adj / t_fwd,
         "components": ngroup + 6,
         "kappa_ratio": gr.ratio, "kappa_ratio_stderr": gr.stderr,
@@ -451,6 +459,7 @@ def adjoint_leg(ctx, npaths: int, cpu, table_seed: int = 20):
         out["cpu_baseline"] = cpu_leg_baseline(work, list(range(0, npaths, 2000)), cpu,
                                                "adjoint paths of the same workload, FP64, 2000 per task")
         out["adjoint_vs_cpu"] = adj_rate / out["cpu_baseline"]["value"]
+        out["adjoint_vs_cpu_at_10x_paths"] = out["adjoint_paths_per_s_at_10x_paths"] / out["cpu_baseline"]["value"]
     g.close()
     m.close()
     return out
