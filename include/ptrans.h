@@ -229,6 +229,15 @@ int pt_ctx_set_op_counting(pt_ctx *ctx, int32_t on);
  * 1..64) and workgroups per CU (0: the occupancy maximum).  Results do not depend on
  * these. */
 int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t blocks_per_cu);
+/* Long random walks (accelerated precisions; every trace but the ballistic grid): a path
+ * past `events` events, or one of the last few of a wave once the work queue is empty,
+ * leaves the persistent kernel for a second kernel that runs it on a whole wave (its ball
+ * set-up and scans spread over the lanes, its in-ball events 64 at a time).  At most
+ * max_records such paths per launch (the buffer holds total / 256 records, at least 65536,
+ * at most max_records; 0: none, every path stays); the rest stay in the persistent kernel.
+ * Defaults: 500 events, 2^20 records.  Results do not depend on these.  No counterpart in
+ * the reference, whose OpenMP threads each run one cell's particles to the end (RT:1573). */
+int pt_ctx_set_tail(pt_ctx *ctx, int64_t events, int64_t max_records);
 
 /* ---- geometry ------------------------------------------------------------------ */
 /* Uploads vgrain (RT:122) as flattened by the host.  Validates indices; at most

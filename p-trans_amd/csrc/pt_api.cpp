@@ -60,7 +60,10 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr size_t kNoAllocMaxModes = 65536;
 constexpr int kNoAllocMaxBranches = 64;
 constexpr size_t kNoAllocCounters = 16;
-constexpr size_t kNoAllocWords = kNoAllocCounters + 2 * kNoAllocMaxModes + kNoAllocMaxBranches;
+// then the records of the paths handed to the tail kernel (16-byte aligned)
+constexpr size_t kNoAllocTailWord = (kNoAllocCounters + 2 * kNoAllocMaxModes + kNoAllocMaxBranches + 1) & ~(size_t)1;
+constexpr int64_t kNoAllocTailRecs = 8192;
+constexpr size_t kNoAllocWords = kNoAllocTailWord + kNoAllocTailRecs * sizeof(TailRec) / 8;
 static_assert(kNoAllocMaxModes == PT_NOALLOC_MAX_MODES && kNoAllocMaxBranches == PT_NOALLOC_MAX_BRANCHES,
               "limits of the allocation-free trace");
 static __device__ __align__(256) unsigned long long scratch[kNoAllocWords];
@@ -115,6 +118,13 @@ struct pt_ctx {
     // sums), kept between calls so a repeated call allocates nothing
     void *work = nullptr;
     size_t work_bytes = 0;
+    // grow-only buffer of the paths handed to the tail kernel, and their adjoint scores
+    TailRec *tail = nullptr;
+    int64_t tail_alloc = 0;
+    double *tail_score = nullptr;
+    int64_t tail_score_alloc = 0;
+    int64_t tail_events = 500;                 // a path past this many events is handed over
+    int64_t tail_max = 1 << 20;                // at most this many records per launch
 };
 
 struct pt_geom {
@@ -199,6 +209,8 @@ int pt_ctx_destroy(pt_ctx *c)
     if (c->counters) (void)hipFree(c->counters);
     if (c->scale) (void)hipFree(c->scale);
     if (c->work) (void)hipFree(c->work);
+    if (c->tail) (void)hipFree(c->tail);
+    if (c->tail_score) (void)hipFree(c->tail_score);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -234,6 +246,16 @@ int pt_ctx_set_tuning(pt_ctx *c, int32_t chunk, int32_t refill_min, int32_t bloc
     c->chunk = chunk;
     c->refill_min = refill_min;
     c->blocks_per_cu = blocks_per_cu;
+    return PT_OK;
+}
+
+int pt_ctx_set_tail(pt_ctx *c, int64_t events, int64_t max_records)
+{
+    if (!c) return fail(PT_E_INVALID, "pt_ctx_set_tail: NULL context");
+    if (events < 1 || max_records < 0 || max_records > ((int64_t)1 << 26))
+        return fail(PT_E_INVALID, "pt_ctx_set_tail: events >= 1, 0 <= max_records <= 2^26");
+    c->tail_events = events;
+    c->tail_max = max_records;
     return PT_OK;
 }
 
@@ -405,6 +427,38 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
             q.idx = cand_idx[j];
         }
     }
+    // regions of the tail kernel (lane_scan geometries): per grain the planes of its scan
+    // order, normalised and oriented towards the grain's generatrix
+    std::vector<float4> region;
+    std::vector<int32_t> region_off(lane_scan ? ng + 1 : 0, 0);
+    if (lane_scan) {
+        for (int gi = 0; gi < ng; ++gi) {
+            region_off[gi] = (int32_t)region.size();
+            if (gi == 0) continue;
+            const double *gp = d->generatrix + 3 * (size_t)gi;
+            const size_t mark = region.size();
+            bool ok = true;
+            for (int part = 0; part < 2 && ok; ++part) {
+                const int g2 = part == 0 ? 0 : gi;
+                for (int j = cand_off[g2]; j < cand_off[g2 + 1] && ok; ++j) {
+                    const pt_surface &s = d->surfaces[cand_idx[j]];
+                    const double *v = d->vertices + 3 * s.vertex_offset;
+                    const double nn = std::sqrt(s.normal[0] * s.normal[0] + s.normal[1] * s.normal[1] +
+                                                s.normal[2] * s.normal[2]);
+                    if (!(nn > 0.0)) { ok = false; break; }
+                    double u[3], off = 0.0, side = 0.0;
+                    for (int k = 0; k < 3; ++k) u[k] = s.normal[k] / nn;
+                    for (int k = 0; k < 3; ++k) { off += u[k] * v[k]; side += u[k] * (gp[k] - v[k]); }
+                    if (!(std::fabs(side) > 1.0e-3)) { ok = false; break; }
+                    const double sg = side > 0.0 ? 1.0 : -1.0;
+                    region.push_back(make_float4((float)(sg * u[0]), (float)(sg * u[1]), (float)(sg * u[2]),
+                                                 (float)(sg * off)));
+                }
+            }
+            if (!ok) region.resize(mark);
+        }
+        region_off[ng] = (int32_t)region.size();
+    }
     std::vector<int32_t> ginc(d->grain_incident, d->grain_incident + ng);
     std::vector<double> gen(d->generatrix, d->generatrix + 3 * (size_t)ng);
     std::vector<double> vtx(d->vertices, d->vertices + 3 * (size_t)d->nvertex);
@@ -417,18 +471,22 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     double *d_tang = nullptr, *d_vtx = nullptr, *d_gen = nullptr;
     int32_t *d_co = nullptr, *d_ci = nullptr, *d_io = nullptr, *d_ii = nullptr, *d_ginc = nullptr;
     CandPlane *d_cp = nullptr;
+    float4 *d_rg = nullptr;
+    int32_t *d_ro = nullptr;
     if ((rc = upload(surf, &d_surf, c->stream)) || (rc = upload(tang, &d_tang, c->stream)) ||
         (rc = upload(vtx, &d_vtx, c->stream)) || (rc = upload(gen, &d_gen, c->stream)) ||
         (rc = upload(cand_off, &d_co, c->stream)) || (rc = upload(cand_idx, &d_ci, c->stream)) ||
         (rc = upload(inc_off, &d_io, c->stream)) || (rc = upload(inc_idx, &d_ii, c->stream)) ||
-        (rc = upload(ginc, &d_ginc, c->stream)) || (rc = upload(cplane, &d_cp, c->stream))) {
+        (rc = upload(ginc, &d_ginc, c->stream)) || (rc = upload(cplane, &d_cp, c->stream)) ||
+        (rc = upload(region, &d_rg, c->stream)) || (rc = upload(region_off, &d_ro, c->stream))) {
         for (void *p : {(void *)d_surf, (void *)d_tang, (void *)d_vtx, (void *)d_gen, (void *)d_co,
-                        (void *)d_ci, (void *)d_io, (void *)d_ii, (void *)d_ginc, (void *)d_cp})
+                        (void *)d_ci, (void *)d_io, (void *)d_ii, (void *)d_ginc, (void *)d_cp,
+                        (void *)d_rg, (void *)d_ro})
             if (p) (void)hipFree(p);
         delete G;
         return rc;
     }
-    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc, d_cp};
+    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc, d_cp, d_rg, d_ro};
     G->surf = surf;
     DevGeom &g = G->dg;
     g.surf = d_surf;
@@ -443,6 +501,11 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     g.surf_base = nullptr;
     g.cand_plane = d_cp;
     g.lane_scan = lane_scan ? 1 : 0;
+    g.region = d_rg;
+    g.region_off = lane_scan ? d_ro : nullptr;
+    // FP32 region tests: coordinates below the box's largest side, a few roundings of
+    // 2^-24 of it each; the margin is far above them and the reference's FP64 rounding
+    g.region_margin = (float)(1.0e-2 + 2.0e-5 * std::max(d->box[0], std::max(d->box[1], d->box[2])));
     for (int k = 0; k < 3; ++k) g.box[k] = d->box[k];
     g.scope[0] = d->scope[0];
     g.scope[1] = d->scope[1];
@@ -677,6 +740,38 @@ TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
     return P;
 }
 
+// The tail buffer of a launch of `total` paths of (mode, precision) with ngroup adjoint
+// scores per path: room for total / 256 records (at least 65536, at most the context's
+// tail_max), grown as needed and kept.  No records (paths stay in the persistent kernel)
+// for traces that hand nothing over or when tail_max is 0.
+int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, TraceParams &P)
+{
+    P.tail = nullptr;
+    P.tail_score = nullptr;
+    P.tail_cap = 0;
+    P.tail_events = c->tail_events;
+    if (!tail_mode(mode, precision) || c->count_ops || c->tail_max <= 0) return PT_OK;
+    const int64_t cap = std::min<int64_t>(c->tail_max, std::max<int64_t>(65536, total / 256));
+    if (c->tail_alloc < cap) {
+        if (c->tail) PT_HIP(hipFree(c->tail));
+        c->tail = nullptr;
+        c->tail_alloc = 0;
+        PT_HIP(hipMalloc((void **)&c->tail, (size_t)cap * sizeof(TailRec)));
+        c->tail_alloc = cap;
+    }
+    if (ngroup > 0 && c->tail_score_alloc < cap * ngroup) {
+        if (c->tail_score) PT_HIP(hipFree(c->tail_score));
+        c->tail_score = nullptr;
+        c->tail_score_alloc = 0;
+        PT_HIP(hipMalloc((void **)&c->tail_score, (size_t)cap * ngroup * sizeof(double)));
+        c->tail_score_alloc = cap * ngroup;
+    }
+    P.tail = c->tail;
+    P.tail_score = ngroup > 0 ? c->tail_score : nullptr;
+    P.tail_cap = cap;
+    return PT_OK;
+}
+
 // Workgroups of a persistent trace launch: CUs x (the tuned count, or the occupancy
 // maximum of this kernel variant with `lds` bytes of dynamic LDS).
 hipError_t launch_blocks(const pt_ctx *c, int mode, int precision, size_t lds, int *blocks)
@@ -874,6 +969,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         P.grain_tally = d_gt;
         if ((e = launch_blocks(c, amode, precision, lds, &blocks))) break;
         P.chunk = wave_chunk(c, P.total, blocks);
+        if ((rc = tail_prepare(c, amode, precision, npaths, amode == TRACE_ADJOINT ? ngroup : 0, P))) return rc;
         if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
@@ -977,6 +1073,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     do {
         if ((e = launch_blocks(c, gmode, precision, 0, &blocks))) break;
         P.chunk = wave_chunk(c, P.total, blocks);
+        if ((rc = tail_prepare(c, gmode, precision, total, 0, P))) return rc;
         if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
@@ -1094,6 +1191,7 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     do {
         if ((e = launch_blocks(c, TRACE_ALIAS, precision, 0, &blocks))) break;
         P.chunk = wave_chunk(c, P.total, blocks);
+        if ((rc = tail_prepare(c, TRACE_ALIAS, precision, npaths, 0, P))) return rc;
         if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
@@ -1141,7 +1239,7 @@ int pt_trace_alias_noalloc(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64
     if (stats) memset(stats, 0, sizeof *stats);
     if (npaths == 0) return PT_OK;
     PT_HIP(hipSetDevice(c->device));
-    // scratch: counters (queue, statistics) | mode_paths | mode_trans | branch scales
+    // scratch: counters (queue, statistics) | mode_paths | mode_trans | branch scales | tail
     unsigned long long *w = c->noalloc;
     TraceParams P = base_params(c, G, M);
     P.precision = precision;
@@ -1152,6 +1250,12 @@ int pt_trace_alias_noalloc(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64
     P.mode_trans = P.mode_paths + nm;
     double *d_scale = (double *)(w + kNoAllocCounters + 2 * kNoAllocMaxModes);
     P.m.scale = branch_scale ? d_scale : nullptr;
+    // long walks go to the tail records at the end of the scratch
+    P.tail_events = c->tail_events;
+    if (tail_mode(TRACE_ALIAS, precision) && !c->count_ops && c->tail_max > 0) {
+        P.tail = (TailRec *)(w + kNoAllocTailWord);
+        P.tail_cap = std::min<int64_t>(c->tail_max, kNoAllocTailRecs);
+    }
     unsigned long long host_ctr[16] = {0};
     float ms = 0.f;
     int blocks = 0;
@@ -1249,6 +1353,8 @@ struct pt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraphNode_t trace_node = nullptr;
+    hipGraphNode_t tail_node = nullptr;        // the tail kernel after the trace (or none)
+    const TailRec *graph_tail = nullptr;       // tail buffer the graph was built with
     const pt_modes *graph_modes = nullptr;
     int graph_prec = -1;
     int graph_blocks = 0;                  // launch shape the graph was built with
@@ -1734,7 +1840,7 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
         return fail(PT_E_UNSUPPORTED, "batch of %d structures needs %zu bytes of LDS per workgroup "
                     "(fewer top BVH levels in LDS)", B->nstruct, *lds + trace_static_lds(TRACE_BATCH));
     PT_HIP(launch_blocks(c, TRACE_BATCH, precision, *lds, blocks));
-    return PT_OK;
+    return tail_prepare(c, TRACE_BATCH, precision, B->npaths, 1, P);
 }
 
 int batch_finish(pt_batch *B, double *sums, double *finish_ms, pt_stats *stats, float ms)
@@ -1810,7 +1916,7 @@ int pt_batch_trace_graph(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t prec
     if (rc) return rc;
     if (stats) memset(stats, 0, sizeof *stats);
     if (!B->exec || B->graph_modes != M || B->graph_prec != precision ||
-        B->graph_blocks != blocks || B->graph_lds != lds) {
+        B->graph_blocks != blocks || B->graph_lds != lds || B->graph_tail != P.tail) {
         if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
         if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
         PT_HIP(hipGraphCreate(&B->graph, 0));
@@ -1836,15 +1942,30 @@ int pt_batch_trace_graph(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t prec
         k.kernelParams = B->graph_args;
         k.extra = nullptr;
         PT_HIP(hipGraphAddKernelNode(&B->trace_node, B->graph, &fnode, 1, &k));
+        B->tail_node = nullptr;
+        if (P.tail_cap > 0) {
+            // the long walks the trace handed over, in the same graph
+            hipKernelNodeParams t = k;
+            t.func = (void *)tail_kernel_fn(TRACE_BATCH, precision);
+            t.sharedMemBytes = 0;
+            PT_HIP(hipGraphAddKernelNode(&B->tail_node, B->graph, &B->trace_node, 1, &t));
+        }
         PT_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
         B->graph_modes = M;
         B->graph_prec = precision;
         B->graph_blocks = blocks;
         B->graph_lds = lds;
+        B->graph_tail = P.tail;
     }
     B->graph_params = P;                       // the current seed and tuning
     B->graph_args[0] = (void *)&B->graph_params;
     PT_HIP(hipGraphExecKernelNodeSetParams(B->exec, B->trace_node, &B->graph_knp));
+    if (B->tail_node) {
+        hipKernelNodeParams t = B->graph_knp;
+        t.func = (void *)tail_kernel_fn(TRACE_BATCH, precision);
+        t.sharedMemBytes = 0;
+        PT_HIP(hipGraphExecKernelNodeSetParams(B->exec, B->tail_node, &t));
+    }
     std::vector<double> base(B->nstruct, 0.0);
     if (baseline) base.assign(baseline, baseline + B->nstruct);
     float ms = 0.f;

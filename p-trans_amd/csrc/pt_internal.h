@@ -53,6 +53,13 @@ struct DevGeom {
     const CandPlane *cand_plane;  // [cand_off[ngrain_total]] when lane_scan
     int32_t lane_scan;         // every grain candidate is a polygon and the box is aligned:
                                // the accelerated scan runs each lane's own candidate list
+    // lane_scan geometries: per grain g the planes of its scan order (the box grain's
+    // candidates, then g's), unit normal and offset (n, n.v) in FP32, oriented so that the
+    // grain's generatrix lies on the positive side: [region_off[g], region_off[g + 1]);
+    // a grain whose generatrix lies on one of them gets none
+    const float4 *region;
+    const int32_t *region_off;
+    float region_margin;       // distance to every plane that a region test requires (nm)
     double box[3];
     double scope[2];
     int32_t ngrain;
@@ -178,6 +185,30 @@ struct DevBatch {
 // score per group in LDS).
 constexpr int kAdjMaxGroups = 32;
 
+// A long random walk handed from the persistent trace kernel to the tail kernel: the
+// whole state of the path between two events (the per-group likelihood-ratio scores go
+// to a separate array, ngroup doubles per record).  192 bytes.
+struct alignas(16) TailRec {
+    double pos[3];
+    double dir[3];
+    double fp, mfp, omega;
+    double spare;               // the Rng's second half-block, when draw is odd
+    double len;                 // adjoint: the path's flight length so far
+    double fdx, fl, fw;         // grain tallies: the open grain's sums and 1 / mfp_m
+    int64_t pid, events;
+    float bc[3], br;
+    int32_t which, cur, flags, start_grain;
+    int32_t mode, sid, nscat, fgrain;
+    uint32_t draw;
+    int32_t pad[3];
+};
+static_assert(sizeof(TailRec) == 192, "TailRec layout");
+// Slots of the 16-word counter block (queue at 0, statistics at 1-9 and 11-14): paths handed
+// to the tail kernel (may count past the capacity: those paths stayed), and the tail
+// kernel's own queue.
+constexpr int kCtrTailN = 10;
+constexpr int kCtrTailNext = 15;
+
 struct TraceParams {
     DevGeom g;
     DevModes m;
@@ -211,7 +242,22 @@ struct TraceParams {
     double *grain_tally;
     int32_t ngt;               // grains tallied (ngrain_total) or 0
     DevBatch bt;               // TRACE_BATCH
+    // long random walks (accelerated precisions, every mode but the ballistic grid): a path
+    // past tail_events events, or one of the last few of a wave once the queue is empty, is
+    // handed to the tail kernel, which runs it on a whole wave; at most tail_cap records
+    // (0: none, paths stay in the persistent kernel)
+    TailRec *tail;
+    double *tail_score;        // [tail_cap * ngroup] (adjoint and batch modes)
+    int64_t tail_cap;
+    int64_t tail_events;
 };
+
+// Modes and precisions whose long walks go to the tail kernel (the op-counting kernels
+// keep them, so that every operation is counted in one place).
+constexpr bool tail_mode(int mode, int precision)
+{
+    return mode != TRACE_GRID && precision != PT_PREC_FP64;
+}
 
 __host__ __device__ inline int adj_stride(const TraceParams &p) { return 3 + p.ngroup + p.m.nb; }
 // Static LDS of trace_kernel: per-lane event counters of a 256-lane workgroup, eight of
@@ -259,6 +305,8 @@ hipError_t launch_replay(const TraceParams &p, int mode, int64_t first, int64_t 
 hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int *blocks_per_cu);
 // the trace kernel of (mode, precision) and the zero-fill kernel, for graph nodes
 const void *trace_kernel_fn(int mode, int precision);
+// the tail kernel of (mode, precision) (nullptr: that trace hands nothing over)
+const void *tail_kernel_fn(int mode, int precision);
 const void *fill_kernel_fn();
 hipError_t launch_fill(unsigned long long *p, size_t n, hipStream_t s);
 

@@ -1207,6 +1207,74 @@ __device__ int lone_scan(const DevGeom &g, int which, const double *pos, const d
     return bi;
 }
 
+// lone_scan for geometries whose candidates are all polygons (lane_scan): lane k takes
+// the plane record of position k of the scan order and the reference's FP64 plane hit
+// (cosang > 0, l >= 0); the wave keeps the smallest plane distance below 1e10 at the
+// earliest position.  A polygon's distance is its plane's when the hit point lies inside
+// it, so if the winning plane's polygon contains its hit point, no polygon is hit nearer
+// and none at the same distance earlier: that is the reference's winner.  Otherwise the
+// full lone_scan decides.  One 64-byte load per lane instead of the record, vertex and
+// index chain of every candidate.
+__device__ int lone_scan_planes(const DevGeom &g, int which, const double *pos, const double *dir,
+                                double *dmin, OpCount &oc)
+{
+    const int lane = threadIdx.x & 63;
+    const ScanOrder so(g, which);
+    double bl = 1.0e10;
+    int bk = -1;
+    for (int c = 0; c < so.n; c += 64) {
+        const int k = c + lane;
+        double l = __builtin_huge_val();
+        int key = 0x7fffffff;
+        if (k < so.n) {
+            const CandPlane cp = g.cand_plane[k < so.n0 ? so.a0 + k : so.b0 + k - so.n0];
+            double d;
+            if (plane_hit(cp.n, cp.v1, pos, dir, &d, oc) && d < bl) {
+                l = d;
+                key = k;
+            }
+        }
+        for (int m = 1; m < 64; m <<= 1) {
+            const double ol = __shfl_xor(l, m, 64);
+            const int ok = __shfl_xor(key, m, 64);
+            if (ol < l || (ol == l && ok < key)) {
+                l = ol;
+                key = ok;
+            }
+        }
+        if (l < bl) {
+            bl = l;
+            bk = key;
+        }
+    }
+    if (bk < 0) {
+        *dmin = 1.0e10;
+        return -1;
+    }
+    const int idx = so.cand(g, bk);
+    if (!polygon_contains(g.vtx, g.surf[idx], pos, dir, bl, oc)) return lone_scan(g, which, pos, dir, dmin, oc);
+    *dmin = bl;
+    return idx;
+}
+
+// True when both ends of the flight from p to e lie on the grain's side of every plane of
+// its region by the margin: the flight then crosses no candidate's plane, and the scan
+// would find no surface within it (the reference's FP64 plane distances, whose rounding
+// is far below the margin, are then negative, beyond the flight, or of planes facing away).
+__device__ inline bool in_region(const float4 *rp, int nr, float m, const double *p, const double *e)
+{
+    const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
+    const float ex = (float)e[0], ey = (float)e[1], ez = (float)e[2];
+    bool ok = true;
+    for (int j = 0; j < nr; ++j) {
+        const float4 q = rp[j];
+        const float a = q.x * px + q.y * py + q.z * pz - q.w;
+        const float b = q.x * ex + q.y * ey + q.z * ez - q.w;
+        ok = ok && a > m && b > m;
+    }
+    return ok;
+}
+
 // Lower bound on the distance from a point to the surfaces of one BVH leaf, as in
 // ball_setup.
 __device__ inline float bvh_leaf_bound(const DevGeom &g, const DevBatch &bt, int first, int count,
@@ -1740,11 +1808,6 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     return ph.end_side != 0;
 }
 
-__device__ inline float readlane_f32(float v, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
 __device__ inline double readlane_f64(double v, int l)
 {
     const long long b = __double_as_longlong(v);
@@ -1753,98 +1816,138 @@ __device__ inline double readlane_f64(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-__device__ inline int64_t readlane_i64(int64_t v, int l)
+// Once the queue is empty, a wave down to this many paths hands them to the tail kernel.
+constexpr int kFewPaths = 4;
+
+// A wave's LDS buffer of the tail kernel's in-ball batches: per coordinate k and event i
+// of a batch the flight fp(i) dir(i)_k and the nudge of its new direction (xn, 16 bytes,
+// read by one ds_read_b128), and the positions at which the events begin (pb, 65 used).
+constexpr int kChainXn = 3 * 64 * 2;
+constexpr int kChainPb = 68;
+// then the planes of the path's region (at most 64, one float4 each)
+constexpr int kChainWords = kChainXn + 3 * kChainPb + 2 * 64;
+
+// Makes this wave's LDS stores visible to its own later loads by other lanes.
+__device__ inline void wave_lds_sync()
 {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)v & 0xffffffffull), l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((uint64_t)v >> 32), l);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Paths left in a wave after its queue ran dry: at most this many are served one at a
-// time by the whole wave (ball set-up, in-ball events, scan), the rest step together.
-constexpr int kFewPaths = 4;
-// A wave with a path beyond this many events stops taking new paths: its other lanes run
-// out, and the long random walk gets the few-path service (the whole wave on one path)
-// while the other waves go on with the queue, instead of waiting for the end of the launch.
-constexpr int64_t kDrainEvents = 20000;
+__device__ inline double pick3(const double *v, int k)
+{
+    return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]);
+}
 
-// One path of a wave down to a few, whose next free flight stays inside its safety ball
-// (the tail of a launch, where one diffusive path can run for 1e5-1e6 events alone).  An
-// in-ball event is an internal scattering whose new direction and next free path depend
-// only on the path's three draws for it (direction: 2, free path: 1), so lane i computes
-// those of event i of a batch of 64.  Everything runs in uniform control flow on a copy of
-// the path broadcast to every lane, so every lane-indexed read reads a lane that computed
-// its values.  The positions at which the events begin come from one sequential chain of
-// FP64 additions (the flight, then the nudge, as step_path rounds them); then each lane
-// applies step_path's checks (inside the box, below the event limit, flight inside the
-// ball) to its event, and the batch keeps the events before the first that fails.  The
-// path's lane takes the copy back.  Every value of the path's state is the one step_path
+// In-ball events of a path that the whole wave holds (the tail kernel: every lane keeps
+// the same copy of the path and computes the same values).  An in-ball event is an
+// internal scattering whose new direction and next free path depend only on the path's
+// three draws for it (direction: 2, free path: 1), so lane i computes those of event i of
+// a batch of 64.  The positions at which the events begin come from the exact sequential
+// chain p(i+1) = (p(i) + fp(i) dir(i)) + nudge dir(i+1), rounded in step_path's order:
+// lane k runs coordinate k's FP64 additions over the displacements the lanes left in LDS
+// (two dependent additions and one 16-byte LDS load per event), for events 0-15, then 16-63.
+// Each lane then applies step_path's checks (inside the box, below the event limit,
+// flight inside the ball, or kRegion: both ends of the flight inside the grain's region,
+// in_region) to its event's start state, and the batch keeps the events before the first
+// that fails.  Every value of the path's state is the one step_path
 // would make: the path is unchanged, bit for bit, and its Rng continues at the right draw.
-template <int kPrec, int kAdj>
-__device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, Path &ph,
-                              AdjLane &al, int src)
+template <int kPrec, int kAdj, bool kRegion>
+__device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, Path &ph,
+                              AdjLane &al, double *cb, const float4 *rp = nullptr, int nr = 0,
+                              float rm = 0.0f)
 {
     const int lane = threadIdx.x & 63;
-    const uint64_t path = (uint64_t)readlane_i64((int64_t)rng.path, src);
-    const double mfp = readlane_f64(ph.mfp, src);
-    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)rng.draw, src);
-    Path u;                                  // the fields in_ball and the event use
-    for (int k = 0; k < 3; ++k) {
-        u.pos[k] = readlane_f64(ph.pos[k], src);
-        u.dir[k] = readlane_f64(ph.dir[k], src);
-        u.bc[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ph.bc[k]), src));
+    double *xn = cb, *pb = cb + kChainXn;
+    // regions (nr <= 64): lane j holds plane j, and a copy in LDS serves the full tests
+    float4 *rl = (float4 *)(cb + kChainXn + 3 * kChainPb);
+    float4 pl = make_float4(0.0f, 0.0f, 0.0f, -3.0e38f);    // a plane no point is near
+    if (kRegion) {
+        if (lane < nr) pl = rp[lane];
+        rl[lane] = pl;
     }
-    u.br = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ph.br), src));
-    u.fp = readlane_f64(ph.fp, src);
-    u.events = readlane_i64(ph.events, src);
-    const int which = __builtin_amdgcn_readlane(ph.which, src);
+    const uint32_t d0 = rng.draw;
     uint32_t d = d0;
     double spare = 0.0;
+    int64_t nev = 0;
     while (true) {
+        // draws di, di + 1, di + 2 lie in the two blocks from di / 2 on
         const uint64_t di = (uint64_t)d + 3ull * (uint64_t)lane;
-        const double u0 = pt_philox_draw(rng.seed, path, di);
-        const double u1 = pt_philox_draw(rng.seed, path, di + 1);
-        const double u2 = pt_philox_draw(rng.seed, path, di + 2);
+        double b0[2], b1[2];
+        pt_philox_block(rng.seed, rng.path, di >> 1, b0);
+        pt_philox_block(rng.seed, rng.path, (di >> 1) + 1, b1);
+        const bool odd = di & 1;
+        const double u0 = odd ? b0[1] : b0[0];
+        const double u1 = odd ? b1[0] : b0[1];
+        const double u2 = odd ? b1[1] : b1[0];
         double nd[3];
         scatter_dir<kPrec>(u0, u1, nd);
-        const double nfp = free_path_of<kPrec>(mfp, u2);
-        // lane i: the state in which event i of the batch begins.  Its free path and
-        // direction come from event i - 1's draws (lane 0: the path's own); its position
-        // from the exact chain p(i+1) = (p(i) + fp(i) dir(i)) + nudge nd(i), rounded in
-        // step_path's order, run once for the wave with the displacements broadcast.
+        const double nfp = free_path_of<kPrec>(ph.mfp, u2);
+        // lane i: the free path and direction with which event i begins (event i - 1's
+        // draws; lane 0: the path's own)
         double fi = __shfl_up(nfp, 1, 64), di3[3];
         for (int k = 0; k < 3; ++k) di3[k] = __shfl_up(nd[k], 1, 64);
         if (lane == 0) {
-            fi = u.fp;
-            for (int k = 0; k < 3; ++k) di3[k] = u.dir[k];
+            fi = ph.fp;
+            for (int k = 0; k < 3; ++k) di3[k] = ph.dir[k];
         }
-        double fly[3], nud[3], pi[3], p[3];
         for (int k = 0; k < 3; ++k) {
-            fly[k] = fi * di3[k];
-            nud[k] = PT_NUDGE * nd[k];
-            pi[k] = p[k] = u.pos[k];
+            xn[(k * 64 + lane) * 2] = fi * di3[k];
+            xn[(k * 64 + lane) * 2 + 1] = PT_NUDGE * nd[k];
         }
-        // in steps of 16 events: the chain, then step_path's checks of those events on
-        // their start states, all lanes at once; the batch runs events 0 .. n-1, and event
-        // n (if any) is left to step_path
-        Path e;
-        for (int k = 0; k < 3; ++k) {
-            e.dir[k] = di3[k];
-            e.bc[k] = u.bc[k];
+        double p = pick3(ph.pos, lane < 3 ? lane : 2);
+        if (lane < 3) pb[lane * kChainPb] = p;
+        if (kRegion) {
+            // the batch's safety ball: around its start point, up to the region's nearest
+            // plane (the region's planes are all the scan's candidates); flights that leave
+            // it take the full region test
+            const float cx = (float)ph.pos[0], cy = (float)ph.pos[1], cz = (float)ph.pos[2];
+            float r = pl.x * cx + pl.y * cy + pl.z * cz - pl.w;
+            for (int m = 1; m < 64; m <<= 1) r = fminf(r, __shfl_xor(r, m, 64));
+            ph.bc[0] = cx;
+            ph.bc[1] = cy;
+            ph.bc[2] = cz;
+            ph.br = ball_radius_of(r);
         }
-        e.fp = fi;
-        e.br = u.br;
+        wave_lds_sync();
         int n = 64;
-        for (int s0 = 0; s0 < 64; s0 += 16) {
-            for (int i = s0; i < s0 + 16; ++i) {
-                for (int k = 0; k < 3; ++k)
-                    p[k] = (p[k] + readlane_f64(fly[k], i)) + readlane_f64(nud[k], i);
-                if (lane == i + 1)
-                    for (int k = 0; k < 3; ++k) pi[k] = p[k];
+        // the chain of the first 16 events, then of the other 48: a batch that ends early
+        // (a flight leaving the ball and the region) pays for few additions it does not use
+        for (int s0 = 0; s0 < 64; s0 = s0 == 0 ? 16 : 64) {
+            const int s1 = s0 == 0 ? 16 : 64;
+            if (lane < 3) {
+                // eight events' displacements in flight at once, then their additions
+                const double2 *x2 = (const double2 *)xn + lane * 64;
+                double *po = pb + lane * kChainPb + 1;
+                for (int h = s0; h < s1; h += 8) {
+                    double2 v[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) v[i] = x2[h + i];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        p = (p + v[i].x) + v[i].y;
+                        po[h + i] = p;
+                    }
+                }
             }
-            for (int k = 0; k < 3; ++k) e.pos[k] = pi[k];
-            const bool ok = inside_box(g, pi) && u.events + lane < event_limit && in_ball(e);
-            const unsigned long long okm = __ballot(ok) | ~(0xffffull << s0);
+            wave_lds_sync();
+            Path e;
+            for (int k = 0; k < 3; ++k) {
+                e.pos[k] = pb[k * kChainPb + lane];
+                e.dir[k] = di3[k];
+                e.bc[k] = ph.bc[k];
+            }
+            e.fp = fi;
+            e.br = ph.br;
+            bool safe = in_ball(e);
+            if (kRegion && !safe) {
+                double q[3];
+                for (int k = 0; k < 3; ++k) q[k] = e.pos[k] + fi * di3[k];
+                safe = in_region(rl, nr, rm, e.pos, q);
+            }
+            const bool ok = inside_box(g, e.pos) && ph.events + lane < event_limit && safe;
+            const unsigned long long okm = __ballot(ok) | (s0 == 0 ? ~0xffffull : 0xffffull);
             if (okm != ~0ull) {
                 n = (int)__builtin_ctzll(~okm);
                 break;
@@ -1852,50 +1955,40 @@ __device__ void lone_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
         }
         if (kAdj && n > 0) {
             // the batch's n scatterings and flights for the likelihood-ratio score and the
-            // grain tally, summed over the lanes (FP64 sums of a path's flights, whose
-            // order the batch sums do not keep anyway)
+            // grain tally, summed over the lanes (an xor butterfly: every lane ends with the
+            // same sum)
             double sl = lane < n ? fi : 0.0, sx = lane < n ? fi * di3[0] : 0.0;
             for (int m = 1; m < 64; m <<= 1) {
                 sl += __shfl_xor(sl, m, 64);
                 sx += __shfl_xor(sx, m, 64);
             }
-            if (lane == src) {
-                if (kAdj & 1) al.flights(which, sl, sx);
-                if (kAdj & 2) {
-                    al.nscat += n;
-                    al.len += sl;
-                }
+            if (kAdj & 1) al.flights(ph.which, sl, sx);
+            if (kAdj & 2) {
+                al.nscat += n;
+                al.len += sl;
             }
         }
+        for (int k = 0; k < 3; ++k) ph.pos[k] = pb[k * kChainPb + n];
         if (n == 64) {
-            for (int k = 0; k < 3; ++k) {
-                u.pos[k] = p[k];
-                u.dir[k] = readlane_f64(nd[k], 63);
-            }
-            u.fp = readlane_f64(nfp, 63);                    // fp - distance = 0: resampled
+            for (int k = 0; k < 3; ++k) ph.dir[k] = readlane_f64(nd[k], 63);
+            ph.fp = readlane_f64(nfp, 63);                  // fp - distance = 0: resampled
         } else {
-            for (int k = 0; k < 3; ++k) {
-                u.pos[k] = readlane_f64(pi[k], n);
-                u.dir[k] = readlane_f64(di3[k], n);
-            }
-            u.fp = readlane_f64(fi, n);
+            for (int k = 0; k < 3; ++k) ph.dir[k] = readlane_f64(di3[k], n);
+            ph.fp = readlane_f64(fi, n);
         }
-        u.events += n;
+        ph.events += n;
+        nev += n;
         d += 3u * (uint32_t)n;
         if (n < 64) {
             // an odd next draw is the second half of a block: the Rng keeps it as its spare
             spare = readlane_f64(u0, n);
             break;
         }
+        // the next batch overwrites the buffer: every lane has read its positions
+        wave_lds_sync();
     }
-    if (lane == src && d != d0) {
-        const uint32_t nev = (d - d0) / 3u;
-        for (int k = 0; k < 3; ++k) {
-            ph.pos[k] = u.pos[k];
-            ph.dir[k] = u.dir[k];
-        }
-        ph.fp = u.fp;
-        ph.events = u.events;
+    if (kRegion) ph.br = -1.0f;
+    if (d != d0) {
         ph.cur = -1;
         rng.draw = d;
         if (d & 1u) rng.spare = spare;
@@ -1984,6 +2077,123 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
     }
 }
 
+// The end of a path: its outcome byte (grid), per-mode tallies (alias), structure sums
+// (batch), likelihood-ratio batch sums (adjoint) or grain flux flush (grains), and the
+// lane's statistics.  adj_acc: the per-batch sums (the workgroup's LDS accumulators).
+template <int kMode>
+__device__ inline void end_path(const TraceParams &P, const Path &ph, const Rng &rng, AdjLane &al,
+                                double *adj_acc, int adj_k, int mode, int64_t pid,
+                                unsigned long long *st)
+{
+    constexpr bool kBatch = kMode == TRACE_BATCH;
+    constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
+    // StartSide /= EndSide (RT:1833): lost paths count, as in the reference; a path stopped
+    // at the event limit has not ended and does not
+    const bool transmitted = ph.end_side != 1 && !(ph.flags & PT_PATH_EVENT_LIMIT);
+    if (is_grid_mode(kMode)) {
+        P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
+    } else {
+        atomicAdd(P.mode_paths + mode, 1ull);
+        if (transmitted) atomicAdd(P.mode_trans + mode, 1ull);
+    }
+    if (kBatch) {
+        const double x = transmitted ? 1.0 / P.m.mfp[mode] : 0.0;
+        const int sid = ph.sid;
+        double *acc = P.bt.sums + kBatchSums * sid;
+        atomicAdd(acc + 0, 1.0);
+        atomicAdd(acc + 1, x);
+        atomicAdd(acc + 2, x * x);
+        const double v = al.score[0];
+        if (v != 0.0) {
+            const double gs = (x - P.bt.baseline[sid]) * v;
+            atomicAdd(acc + 3, gs);
+            atomicAdd(acc + 4, gs * gs);
+        }
+        al.score[0] = 0.0;
+        const unsigned n_s = (unsigned)(P.bt.path_off[sid + 1] - P.bt.path_off[sid]);
+        if (atomicAdd(P.bt.done + sid, 1u) + 1u == n_s) P.bt.finish[sid] = wall_clock64();
+    } else if (kAdj == 1) {
+        al.flush();
+    } else if (kAdj) {
+        const double lam = P.m.mfp[mode];
+        const int ib = mode % P.m.nb;
+        const double sb = P.m.scale ? P.m.scale[ib] : 1.0;
+        const double x = transmitted ? 1.0 / lam : 0.0;
+        const double w = x - P.baseline;
+        double *acc = adj_acc + (int)(pid % P.nbatch) * adj_k;
+        atomicAdd(acc + 0, 1.0);
+        atomicAdd(acc + 1, x);
+        atomicAdd(acc + 2, x * x);
+        for (int gi = 0; gi < P.ngroup; ++gi) {
+            const double v = al.score[gi * al.stride];
+            if (v != 0.0) atomicAdd(acc + 3 + gi, w * v);
+            al.score[gi * al.stride] = 0.0;
+        }
+        atomicAdd(acc + 3 + P.ngroup + ib, w * ((double)al.nscat / sb - al.len / lam));
+    }
+    st[0] += 1ull;
+    st[256 * 1] += (unsigned long long)ph.events;
+    st[256 * 2] += rng.draw;
+    if (ph.flags & PT_PATH_LOST) st[256 * 3] += 1ull;
+    if (ph.flags & PT_PATH_EVENT_LIMIT) st[256 * 5] += 1ull;
+}
+
+// path flag (internal): the tail buffer was full when this path asked for a record, so it
+// stays in the persistent kernel
+constexpr int kPathStay = 1 << 29;
+
+// Hands the lane's path to the tail kernel: its state, Rng position and adjoint or grain
+// sums go to the next tail record, and the lane's own sums start over for its next path.
+// False when the buffer is full (the path then stays).
+template <int kMode>
+__device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, AdjLane &al,
+                        int mode, int64_t pid)
+{
+    constexpr bool kBatch = kMode == TRACE_BATCH;
+    constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
+    const unsigned long long slot = atomicAdd(P.queue + kCtrTailN, 1ull);
+    if (slot >= (unsigned long long)P.tail_cap) return false;
+    TailRec r;
+    for (int k = 0; k < 3; ++k) {
+        r.pos[k] = ph.pos[k];
+        r.dir[k] = ph.dir[k];
+        r.bc[k] = ph.bc[k];
+    }
+    r.fp = ph.fp;
+    r.mfp = ph.mfp;
+    r.omega = ph.omega;
+    r.spare = rng.spare;
+    r.len = al.len;
+    r.fdx = al.fdx;
+    r.fl = al.fl;
+    r.fw = al.fw;
+    r.pid = pid;
+    r.events = ph.events;
+    r.br = ph.br;
+    r.which = ph.which;
+    r.cur = ph.cur;
+    r.flags = ph.flags & ~kPathTop;
+    r.start_grain = ph.start_grain;
+    r.mode = mode;
+    r.sid = ph.sid;
+    r.nscat = al.nscat;
+    r.fgrain = (kAdj & 1) ? al.fgrain : -1;
+    r.draw = rng.draw;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    P.tail[slot] = r;
+    if (kAdj & 2)
+        for (int gi = 0; gi < P.ngroup; ++gi) {
+            P.tail_score[slot * P.ngroup + gi] = al.score[gi * al.stride];
+            al.score[gi * al.stride] = 0.0;
+        }
+    if (kAdj & 1) {
+        al.fgrain = -1;
+        al.fdx = 0.0;
+        al.fl = 0.0;
+    }
+    return true;
+}
+
 }  // namespace
 
 // Persistent trace kernel.  Each wave takes chunks of path ids from one global queue and
@@ -2053,13 +2263,13 @@ trace_kernel(TraceParams P)
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
 
-    constexpr bool kService = kPrec != kPrecFP64 && kMode != TRACE_GRID;
+    // long random walks go to the tail kernel (accelerated precisions; the op-counting
+    // kernels keep them)
+    constexpr bool kHand = tail_mode(kMode, kPrec) && !kCount;
     while (true) {
         const unsigned long long need = __ballot(!active);
-        const bool drain = kService && __any(active && ph.events >= kDrainEvents);
         // refill in batches: start_path is long, so issue it for many lanes at once
-        if (need != 0ull && !exhausted && !drain &&
-            (__popcll(need) >= P.refill_min || need == ~0ull)) {
+        if (need != 0ull && !exhausted && (__popcll(need) >= P.refill_min || need == ~0ull)) {
             if (chunk_next >= chunk_end && kBatch) {
                 // the queue of this wave's XCD first (hardware register XCC_ID), then the others
                 long long b0 = -1, e0 = -1;
@@ -2120,100 +2330,25 @@ trace_kernel(TraceParams P)
         }
         if (top_on && active && (ph.flags & kPathTop) && ph.events >= kTopUnbindEvents)
             top_unbind(stk, ph);
-        int pre_best = -2;
-        double pre_dmin = 0.0;
-        if (kService && (exhausted || drain)) {
-            // at most kFewPaths paths left in the wave after the queue ran dry (the launch's
-            // tail) or while it drains for a long walk: for each in turn, the whole wave sets
-            // up its ball, runs its in-ball events, and scans its next event; the event itself
-            // then runs in step_path
-            unsigned long long am = __ballot(active && ph.end_side == 0);
-            if (am != 0ull && __popcll(am) <= kFewPaths) {
-                while (am != 0ull) {
-                    const int src = (int)__builtin_ctzll(am);
-                    am &= am - 1ull;
-                    const int which = __builtin_amdgcn_readlane(ph.which, src);
-                    if (!kBatch && __builtin_isnan(readlane_f32(ph.br, src))) {
-                        float c[3];
-                        for (int k = 0; k < 3; ++k) c[k] = readlane_f32((float)ph.pos[k], src);
-                        const float r = lone_ball_radius(P.g, which, c[0], c[1], c[2]);
-                        if (lane == src) {
-                            for (int k = 0; k < 3; ++k) ph.bc[k] = c[k];
-                            ph.br = r;
-                        }
-                    }
-                    const int go = __builtin_amdgcn_readlane((lane == src && in_ball(ph)) ? 1 : 0, src);
-                    if (go) lone_ball_run<kPrec, kAdj>(P.g, P.event_limit, rng, ph, al, src);
-                    const int scan = __builtin_amdgcn_readlane((lane == src && !in_ball(ph)) ? 1 : 0, src);
-                    if (!kBatch && scan) {
-                        double p[3], d[3];
-                        for (int k = 0; k < 3; ++k) {
-                            p[k] = readlane_f64(ph.pos[k], src);
-                            d[k] = readlane_f64(ph.dir[k], src);
-                        }
-                        double dl;
-                        const int bi = lone_scan(P.g, which, p, d, &dl, oc);
-                        if (lane == src) {
-                            pre_best = bi;
-                            pre_dmin = dl;
-                        }
-                    }
+        if (kHand) {
+            // a path past tail_events events, or one of the last few of the wave once the queue
+            // is empty, goes to the tail kernel
+            const bool few = exhausted && __popcll(__ballot(active)) <= kFewPaths;
+            if (active && ph.end_side == 0 && !(ph.flags & kPathStay) &&
+                (few || ph.events >= P.tail_events)) {
+                if (handoff<kMode>(P, ph, rng, al, mode, pid)) {
+                    if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
+                    active = false;
+                } else {
+                    ph.flags |= kPathStay;
                 }
             }
         }
         if (active) {
             if (ph.end_side != 0 ||
                 step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
-                                               &P.bt, stk, pre_best, pre_dmin)) {
-                // StartSide /= EndSide (RT:1833): lost paths count, as in the reference;
-                // a path stopped at the event limit has not ended and does not
-                const bool transmitted = ph.end_side != 1 && !(ph.flags & PT_PATH_EVENT_LIMIT);
-                if (is_grid_mode(kMode)) {
-                    P.outcome[pid] = transmitted ? OUT_TRANSMITTED : 0;
-                } else {
-                    atomicAdd(P.mode_paths + mode, 1ull);
-                    if (transmitted) atomicAdd(P.mode_trans + mode, 1ull);
-                }
-                if (kBatch) {
-                    const double x = transmitted ? 1.0 / P.m.mfp[mode] : 0.0;
-                    const int sid = ph.sid;
-                    double *acc = P.bt.sums + kBatchSums * sid;
-                    atomicAdd(acc + 0, 1.0);
-                    atomicAdd(acc + 1, x);
-                    atomicAdd(acc + 2, x * x);
-                    const double v = al.score[0];
-                    if (v != 0.0) {
-                        const double gs = (x - P.bt.baseline[sid]) * v;
-                        atomicAdd(acc + 3, gs);
-                        atomicAdd(acc + 4, gs * gs);
-                    }
-                    al.score[0] = 0.0;
-                    const unsigned n_s = (unsigned)(P.bt.path_off[sid + 1] - P.bt.path_off[sid]);
-                    if (atomicAdd(P.bt.done + sid, 1u) + 1u == n_s) P.bt.finish[sid] = wall_clock64();
-                } else if (kAdj == 1) {
-                    al.flush();
-                } else if (kAdj) {
-                    const double lam = P.m.mfp[mode];
-                    const int ib = mode % P.m.nb;
-                    const double sb = P.m.scale ? P.m.scale[ib] : 1.0;
-                    const double x = transmitted ? 1.0 / lam : 0.0;
-                    const double w = x - P.baseline;
-                    double *acc = adj_acc + (int)(pid % P.nbatch) * adj_k;
-                    atomicAdd(acc + 0, 1.0);
-                    atomicAdd(acc + 1, x);
-                    atomicAdd(acc + 2, x * x);
-                    for (int gi = 0; gi < P.ngroup; ++gi) {
-                        const double v = al.score[gi * al.stride];
-                        if (v != 0.0) atomicAdd(acc + 3 + gi, w * v);
-                        al.score[gi * al.stride] = 0.0;
-                    }
-                    atomicAdd(acc + 3 + P.ngroup + ib, w * ((double)al.nscat / sb - al.len / lam));
-                }
-                st[0] += 1ull;
-                st[256 * 1] += (unsigned long long)ph.events;
-                st[256 * 2] += rng.draw;
-                if (ph.flags & PT_PATH_LOST) st[256 * 3] += 1ull;
-                if (ph.flags & PT_PATH_EVENT_LIMIT) st[256 * 5] += 1ull;
+                                               &P.bt, stk)) {
+                end_path<kMode>(P, ph, rng, al, adj_acc, adj_k, mode, pid, st);
                 if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
                 active = false;
             }
@@ -2243,6 +2378,112 @@ trace_kernel(TraceParams P)
             atomicAdd(P.stats + 13, bs);
         }
     }
+}
+
+// Tail kernel: the long random walks the persistent kernel handed over (paths past
+// tail_events events, and the last few of each wave once its queue ran dry), one path per
+// wave.  Every lane of the wave keeps the same copy of the path and computes the same
+// values, so the control flow is uniform and the whole wave is at the path's service:
+// its ball set-up and its scans spread the candidates over the lanes (lone_ball_radius,
+// lone_scan), its in-ball events run 64 at a time (tail_ball_run), and the rest of each
+// event runs in step_path.  Lane 0 ends the path (end_path: the same tallies as the
+// persistent kernel; adjoint batch sums and grain sums go to HBM directly) and holds the
+// statistics.  The path takes the same steps as it would have in the persistent kernel.
+template <int kMode, int kPrec>
+__global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
+{
+    constexpr bool kBatch = kMode == TRACE_BATCH;
+    constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
+    constexpr int kStats = trace_lane_stats(kMode);
+    __shared__ unsigned long long lane_stats[kStats][256];
+    __shared__ __attribute__((aligned(16))) double chain[4][kChainWords];
+    __shared__ double tscore[kAdjMaxGroups * 4];          // [group * 4 + wave]
+    __shared__ int tstk[kBatch ? kBvhStack * 256 : 1];   // BVH stacks, entries 256 apart
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < kStats; ++k) lane_stats[k][threadIdx.x] = 0ull;
+    unsigned long long *st = &lane_stats[0][threadIdx.x];
+    OpCount oc = {0ull, 0ull, 0ull, 0ull, st};
+    Rng rng;
+    rng.oc = &oc;
+    rng.seed = P.seed;
+    AdjLane al;
+    al.stride = 4;
+    al.score = tscore + wave;
+    al.group = P.surf_group;
+    al.gacc = ((kAdj & 1) && P.grain_tally && lane == 0) ? P.grain_tally : nullptr;
+    const int adj_k = kAdj ? adj_stride(P) : 0;
+    int *stk = tstk + (kBatch ? threadIdx.x : 0);
+    double *cb = chain[wave];
+    const double lim = kBallMfpFrac * fmin(P.g.box[0], fmin(P.g.box[1], P.g.box[2]));
+    const unsigned long long handed = P.queue[kCtrTailN];
+    const int64_t n = (int64_t)(handed < (unsigned long long)P.tail_cap ? handed : (unsigned long long)P.tail_cap);
+    while (true) {
+        unsigned long long slot = 0;
+        if (lane == 0) slot = atomicAdd(P.queue + kCtrTailNext, 1ull);
+        slot = (unsigned long long)uniform64((int64_t)__shfl(slot, 0, 64));
+        if ((int64_t)slot >= n) break;
+        const TailRec r = P.tail[slot];
+        Path ph;
+        for (int k = 0; k < 3; ++k) {
+            ph.pos[k] = r.pos[k];
+            ph.dir[k] = r.dir[k];
+            ph.bc[k] = r.bc[k];
+        }
+        ph.fp = r.fp;
+        ph.mfp = r.mfp;
+        ph.omega = r.omega;
+        ph.which = r.which;
+        ph.cur = r.cur;
+        ph.end_side = 0;
+        ph.flags = r.flags;
+        ph.start_grain = r.start_grain;
+        ph.events = r.events;
+        ph.br = r.br;
+        ph.sid = r.sid;
+        rng.path = (uint64_t)r.pid;
+        rng.draw = r.draw;
+        rng.spare = r.spare;
+        al.nscat = r.nscat;
+        al.len = r.len;
+        al.fgrain = r.fgrain;
+        al.fdx = r.fdx;
+        al.fl = r.fl;
+        al.fw = r.fw;
+        if (kAdj & 2)
+            for (int gi = 0; gi < P.ngroup; ++gi) al.score[gi * al.stride] = P.tail_score[slot * P.ngroup + gi];
+        while (true) {
+            int pre_best = -2;
+            double pre_dmin = 0.0;
+            if (!kBatch && P.g.lane_scan && ph.mfp < lim) {
+                // every candidate is a polygon: the grain's region replaces the safety balls
+                ph.br = -1.0f;
+                const int r0 = P.g.region_off[ph.which], nr = P.g.region_off[ph.which + 1] - r0;
+                double q[3];
+                for (int k = 0; k < 3; ++k) q[k] = ph.pos[k] + ph.fp * ph.dir[k];
+                if (nr > 0 && nr <= 64 && in_region(P.g.region + r0, nr, P.g.region_margin, ph.pos, q))
+                    tail_ball_run<kPrec, kAdj, true>(P.g, P.event_limit, rng, ph, al, cb,
+                                                     P.g.region + r0, nr, P.g.region_margin);
+            } else {
+                if (!kBatch && __builtin_isnan(ph.br)) {
+                    // the ball step_path asked for at its last scattering
+                    for (int k = 0; k < 3; ++k) ph.bc[k] = (float)ph.pos[k];
+                    ph.br = lone_ball_radius(P.g, ph.which, ph.bc[0], ph.bc[1], ph.bc[2]);
+                }
+                if (in_ball(ph)) tail_ball_run<kPrec, kAdj, false>(P.g, P.event_limit, rng, ph, al, cb);
+            }
+            if (!kBatch && !in_ball(ph))
+                pre_best = P.g.lane_scan ? lone_scan_planes(P.g, ph.which, ph.pos, ph.dir, &pre_dmin, oc)
+                                         : lone_scan(P.g, ph.which, ph.pos, ph.dir, &pre_dmin, oc);
+            if (step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
+                                               &P.bt, stk, pre_best, pre_dmin))
+                break;
+        }
+        if (lane == 0) end_path<kMode>(P, ph, rng, al, P.adj, adj_k, r.mode, r.pid, st);
+        wave_lds_sync();
+    }
+    if (lane == 0)
+        for (int k = 0; k < kStats; ++k)
+            if (st[256 * k]) atomicAdd(P.stats + (k < 6 ? k : k + 4), st[256 * k]);
 }
 
 // Per (b,k,θ) cell, one wave: the cell's np outcome bytes in steps of 1 KB, 16 bytes per
@@ -2376,7 +2617,27 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
 #undef PT_LAUNCH_R
 #undef PT_LAUNCH_C
 #undef PT_LAUNCH
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && tail_mode(mode, p.precision) && !count && p.tail_cap > 0) {
+        if (threads != 256) return hipErrorInvalidValue;       // four waves per workgroup
+        hipLaunchKernelGGL(reinterpret_cast<void (*)(TraceParams)>(const_cast<void *>(tail_kernel_fn(mode, p.precision))),
+                           gr, bl, 0, s, p);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+const void *tail_kernel_fn(int mode, int precision)
+{
+#define PT_TAIL(M) (precision == kPrecFast ? (const void *)tail_kernel<M, kPrecFast> \
+                                           : (const void *)tail_kernel<M, kPrecFilter>)
+    if (!tail_mode(mode, precision)) return nullptr;
+    if (mode == TRACE_GRID_TAIL) return PT_TAIL(TRACE_GRID_TAIL);
+    if (mode == TRACE_ALIAS) return PT_TAIL(TRACE_ALIAS);
+    if (mode == TRACE_ADJOINT) return PT_TAIL(TRACE_ADJOINT);
+    if (mode == TRACE_GRAINS) return PT_TAIL(TRACE_GRAINS);
+    return PT_TAIL(TRACE_BATCH);
+#undef PT_TAIL
 }
 
 hipError_t trace_occupancy(int mode, int precision, int threads, size_t lds, int *blocks_per_cu)

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libptrans.so")
 EXPORTS = (
     "pt_version", "pt_build_info", "pt_last_error", "pt_philox_fill",
     "pt_ctx_create", "pt_ctx_destroy", "pt_ctx_set_seed", "pt_ctx_set_event_limit",
-    "pt_ctx_set_op_counting", "pt_ctx_set_tuning",
+    "pt_ctx_set_op_counting", "pt_ctx_set_tuning", "pt_ctx_set_tail",
     "pt_geom_create", "pt_geom_destroy", "pt_geom_set_specularity", "pt_surface_setup_polygon",
     "pt_modes_create", "pt_modes_destroy",
     "pt_trace_grid", "pt_replay", "pt_trace_log", "pt_alias_build", "pt_trace_alias",
@@ -76,6 +76,7 @@ def load():
     L.pt_ctx_set_event_limit.argtypes = [vp, i64]
     L.pt_ctx_set_op_counting.argtypes = [vp, i32]
     L.pt_ctx_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.pt_ctx_set_tail.argtypes = [vp, i64, i64]
     L.pt_geom_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
     L.pt_geom_destroy.argtypes = [vp]
     L.pt_surface_setup_polygon.argtypes = [i32, vp, vp, vp, vp]

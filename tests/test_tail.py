@@ -1,11 +1,15 @@
-"""The last few paths of a wave.  Once the work queue is empty, a wave with four or fewer
-paths left serves them one at a time with all 64 lanes: it sets up a pending safety ball,
-runs batches of in-ball events (their draws computed 64 at a time) and spreads the next
-candidate scan over the lanes.  Diffusive modes make such paths common (they end a launch
-alone), so these tests trace them at the FP32-filter precision, whose paths are bit-identical
-to FP64, and pin everything such a path updates against the oracle: adjoint scores
-(scattering counts and path lengths), per-grain flux tallies (flights inside one grain) and
-the per-mode tallies.
+"""Long random walks and the last few paths of a wave.  In the accelerated precisions a
+path past `tail_events` events (pt_ctx_set_tail), or one of the last four of a wave once the
+work queue is empty, leaves the persistent kernel for the tail kernel, which runs it on a
+whole wave: it sets up a pending safety ball, runs batches of 64 in-ball events (their draws
+computed 64 at a time, the positions by a sequential FP64 chain through LDS) and spreads the
+next candidate scan over the lanes.  Diffusive modes make such paths common, so these tests
+trace them at the FP32-filter precision, whose paths are bit-identical to FP64, and pin
+everything such a path updates against the oracle: adjoint scores (scattering counts and
+path lengths), per-grain flux tallies (flights inside one grain) and the per-mode tallies.
+They run with the default hand-over (500 events), with every path handed over after its
+first event, with a tail buffer smaller than the paths that ask for it (the rest stay in the
+persistent kernel) and with no hand-over at all.
 """
 import numpy as np
 import pytest
@@ -17,10 +21,22 @@ from test_adjoint import OracleSetup, _weights
 
 M_TWOPI = 3.14159265358979323846 * 2.0
 FILTER = 2
+# (events, max_records) of pt_ctx_set_tail: the default, every path after one event, a
+# buffer too small for the paths that ask (4096 records), no tail kernel
+TAILS = [(500, 1 << 20), (1, 1 << 20), (1, 4096), (500, 0)]
+TAIL_IDS = ["default", "all", "overflow", "none"]
+
+
+@pytest.fixture
+def tail_ctx(gpu_ctx, request):
+    gpu_ctx.set_tail(*request.param)
+    yield gpu_ctx
+    gpu_ctx.set_tail(*TAILS[0])
 
 
 @pytest.mark.gpu
-def test_gpu_adjoint_diffusive_tail_equals_oracle(gpu_ctx):
+@pytest.mark.parametrize("tail_ctx", TAILS, ids=TAIL_IDS, indirect=True)
+def test_gpu_adjoint_diffusive_tail_equals_oracle(tail_ctx):
     """Mean free paths of 2 - 20 nm in a 300 x 100 x 100 nm wire (safety balls below 5 nm),
     branch scales 0.8 and 1.25: adjoint batch sums equal the oracle's."""
     model = gen.nanowire(length=300.0, width=100.0, height=100.0, p_wall=0.5, nphonon=1, nangle=3)
@@ -30,11 +46,11 @@ def test_gpu_adjoint_diffusive_tail_equals_oracle(gpu_ctx):
     group[4] = group[5] = 1
     s = OracleSetup(model, 2, 3, 11, (2.0, 20.0), [0.35, 0.6], group=group)
     scale = np.array([0.8, 1.25])
-    gpu_ctx.set_seed(19)
-    g = gpu_ctx.geometry(s.geo)
+    tail_ctx.set_seed(19)
+    g = tail_ctx.geometry(s.geo)
     g.set_specularity(s.p0)
-    m = gpu_ctx.modes(omega=s.omega, vel=s.vel, mfp=s.mfp, weight=s.weight)
-    mp, mt, gs, st = gpu_ctx.adjoint_alias(g, m, 20000, surface_group=s.group, ngroup=s.ngroup,
+    m = tail_ctx.modes(omega=s.omega, vel=s.vel, mfp=s.mfp, weight=s.weight)
+    mp, mt, gs, st = tail_ctx.adjoint_alias(g, m, 20000, surface_group=s.group, ngroup=s.ngroup,
                                            baseline=0.002, nbatch=16, branch_scale=scale,
                                            precision=FILTER)
     print("events per path", st["events"] / st["paths"], flush=True)
@@ -45,7 +61,8 @@ def test_gpu_adjoint_diffusive_tail_equals_oracle(gpu_ctx):
 
 
 @pytest.mark.gpu
-def test_gpu_grain_tallies_diffusive_tail_equal_oracle(gpu_ctx):
+@pytest.mark.parametrize("tail_ctx", TAILS, ids=TAIL_IDS, indirect=True)
+def test_gpu_grain_tallies_diffusive_tail_equal_oracle(tail_ctx):
     """Mean free paths of 3 - 30 nm in a 300 nm Voronoi cube with Lorentzian boundaries
     (safety balls below 15 nm): per-mode and per-grain tallies equal the oracle's."""
     m_ = gen.voronoi_cube(seed=4, L=300.0, ngrain=5, kind="Gamma", value=1.5, p=0.3, nphonon=1,
@@ -59,10 +76,10 @@ def test_gpu_grain_tallies_diffusive_tail_equal_oracle(gpu_ctx):
     mdesc, _m = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
     mp, mt, gt = O.trace_alias_grains(gdesc, mdesc, t.mfp_nm, prob, alias, 23, 0, 20000,
                                       len(geo.grains))
-    gpu_ctx.set_seed(23)
-    g = gpu_ctx.geometry(geo)
-    m = gpu_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
-    gp, gtr, gg, st = gpu_ctx.trace_alias_grains(g, m, 20000, precision=FILTER)
+    tail_ctx.set_seed(23)
+    g = tail_ctx.geometry(geo)
+    m = tail_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    gp, gtr, gg, st = tail_ctx.trace_alias_grains(g, m, 20000, precision=FILTER)
     print("events per path", st["events"] / st["paths"], flush=True)
     assert st["events"] / st["paths"] > 30
     np.testing.assert_array_equal(gp, mp)
@@ -71,7 +88,8 @@ def test_gpu_grain_tallies_diffusive_tail_equal_oracle(gpu_ctx):
 
 
 @pytest.mark.gpu
-def test_gpu_many_candidates_tail_scan_equals_oracle(gpu_ctx):
+@pytest.mark.parametrize("tail_ctx", TAILS[:2], ids=TAIL_IDS[:2], indirect=True)
+def test_gpu_many_candidates_tail_scan_equals_oracle(tail_ctx):
     """80 spheres in a 1000 nm cube: a path in the matrix has 86 candidates, so the wave's
     scan for a path left alone takes two passes of 64 lanes (the first strict minimum kept
     across passes).  Per-mode tallies equal the oracle's."""
@@ -85,10 +103,10 @@ def test_gpu_many_candidates_tail_scan_equals_oracle(gpu_ctx):
     gdesc, _g = geo.desc()
     mdesc, _m = geometry.modes_desc(omega, t.vel, t.mfp_nm, t.weight)
     mp, mt = O.trace_alias(gdesc, mdesc, t.mfp_nm, prob, alias, 29, 0, 20000)
-    gpu_ctx.set_seed(29)
-    g = gpu_ctx.geometry(geo)
-    m = gpu_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
-    gp, gt, st = gpu_ctx.trace_alias(g, m, 20000, precision=FILTER)
+    tail_ctx.set_seed(29)
+    g = tail_ctx.geometry(geo)
+    m = tail_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    gp, gt, st = tail_ctx.trace_alias(g, m, 20000, precision=FILTER)
     print("events per path", st["events"] / st["paths"], "fallbacks", st["fallbacks"], flush=True)
     np.testing.assert_array_equal(gp, mp)
     np.testing.assert_array_equal(gt, mt)
