@@ -59,7 +59,7 @@ constexpr double kPi = 3.14159265358979323846;
 // its device address is looked up once per device when a context is created there.
 constexpr size_t kNoAllocMaxModes = 65536;
 constexpr int kNoAllocMaxBranches = 64;
-constexpr size_t kNoAllocCounters = 16;
+constexpr size_t kNoAllocCounters = kCounterWords;
 // then the records of the paths handed to the tail kernel (16-byte aligned)
 constexpr size_t kNoAllocTailWord = (kNoAllocCounters + 2 * kNoAllocMaxModes + kNoAllocMaxBranches + 1) & ~(size_t)1;
 constexpr int64_t kNoAllocTailRecs = 8192;
@@ -188,7 +188,7 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->counters, kCounterWords * sizeof(unsigned long long));
     if (e == hipSuccess) e = scratch_address(device, &c->noalloc);
     if (e != hipSuccess) {
         pt_ctx_destroy(c);
@@ -769,6 +769,7 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
     P.tail = c->tail;
     P.tail_score = ngroup > 0 ? c->tail_score : nullptr;
     P.tail_cap = cap;
+    P.tail_cap_p = cap / 4;
     return PT_OK;
 }
 
@@ -957,7 +958,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
     std::vector<double> h_gt(ntal);
     std::vector<unsigned long long> h_tal(2 * nm);
     std::vector<double> h_sum(nsum);
-    unsigned long long host_ctr[16] = {0};
+    unsigned long long host_ctr[kCounterWords] = {0};
     float ms = 0.f;
     int blocks = 0;
     do {
@@ -971,7 +972,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         P.chunk = wave_chunk(c, P.total, blocks);
         if ((rc = tail_prepare(c, amode, precision, npaths, amode == TRACE_ADJOINT ? ngroup : 0, P))) return rc;
         if ((e = hipMemcpyAsync(d_grp, grp.data(), (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream))) break;
-        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, kCounterWords * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_sum, 0, nsum * sizeof(double), c->stream))) break;
         if (ntal && (e = hipMemsetAsync(d_gt, 0, ntal * sizeof(double), c->stream))) break;
@@ -981,7 +982,7 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipMemcpyAsync(h_sum.data(), d_sum, nsum * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
         if (ntal && (e = hipMemcpyAsync(h_gt.data(), d_gt, ntal * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipStreamSynchronize(c->stream))) break;
         e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
     } while (0);
@@ -1062,7 +1063,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
     P.total = total;
     P.nangle = nangle;
     P.outcome = c->outcome;
-    unsigned long long host_ctr[16] = {0};
+    unsigned long long host_ctr[kCounterWords] = {0};
     float ms = 0.f, ms_red = 0.f;
     int blocks = 0;
     // a diffusive sampling mean free path (one that gets safety balls) gets the kernel that
@@ -1075,7 +1076,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         P.chunk = wave_chunk(c, P.total, blocks);
         if ((rc = tail_prepare(c, gmode, precision, total, 0, P))) return rc;
         if ((e = hipMemcpyAsync(d_ang, ang.data(), 2 * nangle * sizeof(double), hipMemcpyHostToDevice, c->stream))) break;
-        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, kCounterWords * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
         if ((e = launch_trace(P, gmode, blocks, c->threads, c->count_ops, 0, c->stream))) break;
@@ -1083,7 +1084,7 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
         if ((e = hipEventRecord(c->ev2, c->stream))) break;
         if ((e = hipMemcpyAsync(c->host_stage, d_counts, ncount * sizeof(long long), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipStreamSynchronize(c->stream))) break;
         if ((e = hipEventElapsedTime(&ms, c->ev0, c->ev1))) break;
         e = hipEventElapsedTime(&ms_red, c->ev1, c->ev2);
@@ -1184,7 +1185,7 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
     P.mode_paths = d_tal;
     P.mode_trans = d_tal + nm;
     std::vector<unsigned long long> h_tal(2 * nm);
-    unsigned long long host_ctr[16] = {0};
+    unsigned long long host_ctr[kCounterWords] = {0};
     float ms = 0.f;
     int blocks = 0;
     hipError_t e;
@@ -1192,13 +1193,13 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
         if ((e = launch_blocks(c, TRACE_ALIAS, precision, 0, &blocks))) break;
         P.chunk = wave_chunk(c, P.total, blocks);
         if ((rc = tail_prepare(c, TRACE_ALIAS, precision, npaths, 0, P))) return rc;
-        if ((e = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream))) break;
+        if ((e = hipMemsetAsync(c->counters, 0, kCounterWords * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
         if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, 0, c->stream))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
-        if ((e = hipMemcpyAsync(host_ctr, c->counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
+        if ((e = hipMemcpyAsync(host_ctr, c->counters, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipStreamSynchronize(c->stream))) break;
         e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
     } while (0);
@@ -1255,8 +1256,9 @@ int pt_trace_alias_noalloc(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64
     if (tail_mode(TRACE_ALIAS, precision) && !c->count_ops && c->tail_max > 0) {
         P.tail = (TailRec *)(w + kNoAllocTailWord);
         P.tail_cap = std::min<int64_t>(c->tail_max, kNoAllocTailRecs);
+        P.tail_cap_p = P.tail_cap / 4;
     }
-    unsigned long long host_ctr[16] = {0};
+    unsigned long long host_ctr[kCounterWords] = {0};
     float ms = 0.f;
     int blocks = 0;
     hipError_t e;
@@ -1344,7 +1346,7 @@ struct pt_batch {
     std::vector<void *> owned;
     DevBatch bt{};
     int32_t *d_group = nullptr;            // surface -> 0 (structure facets) or -1 (box)
-    // scratch zeroed by every launch: counters(16) | sums(kBatchSums n) | finish(n) | start(1) |
+    // scratch zeroed by every launch: counters(kCounterWords) | sums(kBatchSums n) | finish(n) | start(1) |
     // done(ceil(n/2)) | mode tallies (2 nm)
     unsigned long long *scratch = nullptr;
     size_t scratch_words = 0, nm = 0;
@@ -1693,7 +1695,7 @@ int pt_batch_create(pt_ctx *c, int32_t nstruct, const pt_geom_desc *const *descs
     for (size_t i = 6; i < ns; ++i) group[i] = 0;
     B->nm = 0;
     const size_t n = (size_t)nstruct;
-    B->scratch_words = 16 + kBatchSums * n + n + 1 + (n + 1) / 2;
+    B->scratch_words = kCounterWords + kBatchSums * n + n + 1 + (n + 1) / 2;
     hipStream_t st = c->stream;
     int64_t *d_poff = nullptr, *d_qoff = nullptr;
     int32_t *d_root = nullptr, *d_prims = nullptr, *d_loff = nullptr, *d_lidx = nullptr;
@@ -1802,7 +1804,7 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
         return fail(PT_E_UNSUPPORTED, "precision %d", precision);
     const size_t nm = (size_t)M->nb * M->nk;
     const size_t S = (size_t)kBatchSums * B->nstruct;
-    const size_t need = 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm + kXcdQueues;
+    const size_t need = kCounterWords + S + B->nstruct + 1 + (B->nstruct + 1) / 2 + 2 * nm + kXcdQueues;
     if (B->scratch_words < need || !B->scratch) {
         if (B->scratch) PT_HIP(hipFree(B->scratch));
         B->scratch = nullptr;
@@ -1824,12 +1826,12 @@ int batch_prepare(pt_batch *B, const pt_modes *M, int precision, TraceParams &P,
     P.baseline = 0.0;
     P.m.scale = nullptr;
     DevBatch bt = B->bt;
-    bt.sums = (double *)(w + 16);
-    bt.finish = w + 16 + S;
-    bt.start = w + 16 + S + B->nstruct;
-    bt.done = (unsigned int *)(w + 16 + S + B->nstruct + 1);
+    bt.sums = (double *)(w + kCounterWords);
+    bt.finish = w + kCounterWords + S;
+    bt.start = w + kCounterWords + S + B->nstruct;
+    bt.done = (unsigned int *)(w + kCounterWords + S + B->nstruct + 1);
     P.bt = bt;
-    P.mode_paths = w + 16 + S + B->nstruct + 1 + (B->nstruct + 1) / 2;
+    P.mode_paths = w + kCounterWords + S + B->nstruct + 1 + (B->nstruct + 1) / 2;
     P.mode_trans = P.mode_paths + nm;
     P.bt.qctr = P.mode_trans + nm;
     P.adj = nullptr;
@@ -1848,17 +1850,17 @@ int batch_finish(pt_batch *B, double *sums, double *finish_ms, pt_stats *stats, 
     pt_ctx *c = B->ctx;
     const size_t n = (size_t)B->nstruct;
     const size_t S = (size_t)kBatchSums * n;
-    std::vector<unsigned long long> h(16 + S + n + 1);
+    std::vector<unsigned long long> h(kCounterWords + S + n + 1);
     PT_HIP(hipMemcpyAsync(h.data(), B->scratch, h.size() * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, c->stream));
     PT_HIP(hipStreamSynchronize(c->stream));
-    const double *hs = (const double *)(h.data() + 16);
+    const double *hs = (const double *)(h.data() + kCounterWords);
     if (sums)
         for (size_t i = 0; i < S; ++i) sums[i] += hs[i];
-    const unsigned long long t0 = h[16 + S + n];
+    const unsigned long long t0 = h[kCounterWords + S + n];
     if (finish_ms)
         for (size_t s = 0; s < n; ++s) {
-            const unsigned long long t = h[16 + S + s];
+            const unsigned long long t = h[kCounterWords + S + s];
             // wall_clock64 ticks at 100 MHz on gfx9
             finish_ms[s] = t >= t0 && t ? (double)(t - t0) * 1.0e-5 : -1.0;
         }

@@ -203,11 +203,18 @@ struct alignas(16) TailRec {
     int32_t pad[3];
 };
 static_assert(sizeof(TailRec) == 192, "TailRec layout");
-// Slots of the 16-word counter block (queue at 0, statistics at 1-9 and 11-14): paths handed
-// to the tail kernel (may count past the capacity: those paths stayed), and the tail
-// kernel's own queue.
-constexpr int kCtrTailN = 10;
-constexpr int kCtrTailNext = 15;
+// The counter block of a launch: queue at 0, statistics at 1-14 (pt_stats order, see
+// fill_stats), then the tail records claimed in the general and the priority part of the
+// buffer (either may count past its capacity: those paths stayed) and the tail kernel's
+// own queue.
+constexpr int kCounterWords = 32;
+constexpr int kCtrTailN = 16;
+constexpr int kCtrTailP = 17;
+constexpr int kCtrTailNext = 18;
+// Paths whose mean free path is below this fraction of the smallest box side take records
+// in the priority part of the tail buffer, which the tail kernel serves first: their walks
+// are the longest (~(L / mfp)^2 events), so they start at once.
+constexpr double kTailPrioMfpFrac = 2.0e-3;
 
 struct TraceParams {
     DevGeom g;
@@ -248,7 +255,8 @@ struct TraceParams {
     // (0: none, paths stay in the persistent kernel)
     TailRec *tail;
     double *tail_score;        // [tail_cap * ngroup] (adjoint and batch modes)
-    int64_t tail_cap;
+    int64_t tail_cap;          // records in all; the first tail_cap_p are the priority part
+    int64_t tail_cap_p;
     int64_t tail_events;
 };
 

@@ -1860,13 +1860,9 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
 {
     const int lane = threadIdx.x & 63;
     double *xn = cb, *pb = cb + kChainXn;
-    // regions (nr <= 64): lane j holds plane j, and a copy in LDS serves the full tests
-    float4 *rl = (float4 *)(cb + kChainXn + 3 * kChainPb);
-    float4 pl = make_float4(0.0f, 0.0f, 0.0f, -3.0e38f);    // a plane no point is near
-    if (kRegion) {
-        if (lane < nr) pl = rp[lane];
-        rl[lane] = pl;
-    }
+    // regions (nr <= 64, rp: the wave's LDS copy of the planes, entries nr..63 planes no
+    // point is near): lane j holds plane j for the batch balls
+    const float4 pl = kRegion ? rp[lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint32_t d0 = rng.draw;
     uint32_t d = d0;
     double spare = 0.0;
@@ -1944,7 +1940,7 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
             if (kRegion && !safe) {
                 double q[3];
                 for (int k = 0; k < 3; ++k) q[k] = e.pos[k] + fi * di3[k];
-                safe = in_region(rl, nr, rm, e.pos, q);
+                safe = in_region(rp, nr, rm, e.pos, q);
             }
             const bool ok = inside_box(g, e.pos) && ph.events + lane < event_limit && safe;
             const unsigned long long okm = __ballot(ok) | (s0 == 0 ? ~0xffffull : 0xffffull);
@@ -2151,8 +2147,10 @@ __device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, Ad
 {
     constexpr bool kBatch = kMode == TRACE_BATCH;
     constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
-    const unsigned long long slot = atomicAdd(P.queue + kCtrTailN, 1ull);
-    if (slot >= (unsigned long long)P.tail_cap) return false;
+    const bool prio = ph.mfp < kTailPrioMfpFrac * fmin(P.g.box[0], fmin(P.g.box[1], P.g.box[2]));
+    unsigned long long slot = atomicAdd(P.queue + (prio ? kCtrTailP : kCtrTailN), 1ull);
+    if (slot >= (unsigned long long)(prio ? P.tail_cap_p : P.tail_cap - P.tail_cap_p)) return false;
+    if (!prio) slot += (unsigned long long)P.tail_cap_p;
     TailRec r;
     for (int k = 0; k < 3; ++k) {
         r.pos[k] = ph.pos[k];
@@ -2414,14 +2412,18 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
     const int adj_k = kAdj ? adj_stride(P) : 0;
     int *stk = tstk + (kBatch ? threadIdx.x : 0);
     double *cb = chain[wave];
+    float4 *rl = (float4 *)(cb + kChainXn + 3 * kChainPb);   // the staged region's planes
+    int reg_grain = -1;
     const double lim = kBallMfpFrac * fmin(P.g.box[0], fmin(P.g.box[1], P.g.box[2]));
-    const unsigned long long handed = P.queue[kCtrTailN];
-    const int64_t n = (int64_t)(handed < (unsigned long long)P.tail_cap ? handed : (unsigned long long)P.tail_cap);
+    // the priority part of the buffer first, then the general part
+    const int64_t np = (int64_t)min(P.queue[kCtrTailP], (unsigned long long)P.tail_cap_p);
+    const int64_t n = np + (int64_t)min(P.queue[kCtrTailN], (unsigned long long)(P.tail_cap - P.tail_cap_p));
     while (true) {
-        unsigned long long slot = 0;
-        if (lane == 0) slot = atomicAdd(P.queue + kCtrTailNext, 1ull);
-        slot = (unsigned long long)uniform64((int64_t)__shfl(slot, 0, 64));
-        if ((int64_t)slot >= n) break;
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(P.queue + kCtrTailNext, 1ull);
+        t = (unsigned long long)uniform64((int64_t)__shfl(t, 0, 64));
+        if ((int64_t)t >= n) break;
+        const int64_t slot = (int64_t)t < np ? (int64_t)t : P.tail_cap_p + ((int64_t)t - np);
         const TailRec r = P.tail[slot];
         Path ph;
         for (int k = 0; k < 3; ++k) {
@@ -2458,11 +2460,19 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
                 // every candidate is a polygon: the grain's region replaces the safety balls
                 ph.br = -1.0f;
                 const int r0 = P.g.region_off[ph.which], nr = P.g.region_off[ph.which + 1] - r0;
-                double q[3];
-                for (int k = 0; k < 3; ++k) q[k] = ph.pos[k] + ph.fp * ph.dir[k];
-                if (nr > 0 && nr <= 64 && in_region(P.g.region + r0, nr, P.g.region_margin, ph.pos, q))
-                    tail_ball_run<kPrec, kAdj, true>(P.g, P.event_limit, rng, ph, al, cb,
-                                                     P.g.region + r0, nr, P.g.region_margin);
+                if (nr > 0 && nr <= 64) {
+                    if (reg_grain != ph.which) {
+                        // the wave's LDS copy of the region, padded with planes no point is near
+                        rl[lane] = lane < nr ? P.g.region[r0 + lane] : make_float4(0.0f, 0.0f, 0.0f, -3.0e38f);
+                        wave_lds_sync();
+                        reg_grain = ph.which;
+                    }
+                    double q[3];
+                    for (int k = 0; k < 3; ++k) q[k] = ph.pos[k] + ph.fp * ph.dir[k];
+                    if (in_region(rl, nr, P.g.region_margin, ph.pos, q))
+                        tail_ball_run<kPrec, kAdj, true>(P.g, P.event_limit, rng, ph, al, cb, rl, nr,
+                                                         P.g.region_margin);
+                }
             } else {
                 if (!kBatch && __builtin_isnan(ph.br)) {
                     // the ball step_path asked for at its last scattering
