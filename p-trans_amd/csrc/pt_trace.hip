@@ -1261,18 +1261,38 @@ __device__ int lone_scan_planes(const DevGeom &g, int which, const double *pos, 
 // its region by the margin: the flight then crosses no candidate's plane, and the scan
 // would find no surface within it (the reference's FP64 plane distances, whose rounding
 // is far below the margin, are then negative, beyond the flight, or of planes facing away).
-__device__ inline bool in_region(const float4 *rp, int nr, float m, const double *p, const double *e)
+// Only the planes that such a flight can reach are tested: dl[j] is plane j's signed
+// distance from a point c, and both ends lie within `reach` of c, so a plane with
+// dl[j] > reach + 2m keeps both ends beyond m (the FP32 rounding of dl and reach is far
+// below m).
+__device__ inline bool in_region(const float4 *rp, const float *dl, int nr, float m, float reach,
+                                 const double *p, const double *e)
 {
     const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
     const float ex = (float)e[0], ey = (float)e[1], ez = (float)e[2];
+    const float lim = reach + 2.0f * m;
     bool ok = true;
     for (int j = 0; j < nr; ++j) {
-        const float4 q = rp[j];
-        const float a = q.x * px + q.y * py + q.z * pz - q.w;
-        const float b = q.x * ex + q.y * ey + q.z * ez - q.w;
-        ok = ok && a > m && b > m;
+        if (dl[j] <= lim) {
+            const float4 q = rp[j];
+            const float a = q.x * px + q.y * py + q.z * pz - q.w;
+            const float b = q.x * ex + q.y * ey + q.z * ez - q.w;
+            ok = ok && a > m && b > m;
+        }
     }
     return ok;
+}
+
+// Signed distances of the point c to the region's planes, lane j taking plane pl = plane
+// j (lanes past the region hold a plane no point is near), into dl; returns their minimum
+// on every lane.
+__device__ inline float region_dists(const float4 &pl, float cx, float cy, float cz, float *dl)
+{
+    const float d = pl.x * cx + pl.y * cy + pl.z * cz - pl.w;
+    dl[threadIdx.x & 63] = d;
+    float r = d;
+    for (int m = 1; m < 64; m <<= 1) r = fminf(r, __shfl_xor(r, m, 64));
+    return r;
 }
 
 // Lower bound on the distance from a point to the surfaces of one BVH leaf, as in
@@ -1824,8 +1844,9 @@ constexpr int kFewPaths = 4;
 // read by one ds_read_b128), and the positions at which the events begin (pb, 65 used).
 constexpr int kChainXn = 3 * 64 * 2;
 constexpr int kChainPb = 68;
-// then the planes of the path's region (at most 64, one float4 each)
-constexpr int kChainWords = kChainXn + 3 * kChainPb + 2 * 64;
+// then the planes of the path's region (at most 64, one float4 each) and their distances
+// from a point (64 floats)
+constexpr int kChainWords = kChainXn + 3 * kChainPb + 2 * 64 + 32;
 
 // Makes this wave's LDS stores visible to its own later loads by other lanes.
 __device__ inline void wave_lds_sync()
@@ -1861,8 +1882,10 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
     const int lane = threadIdx.x & 63;
     double *xn = cb, *pb = cb + kChainXn;
     // regions (nr <= 64, rp: the wave's LDS copy of the planes, entries nr..63 planes no
-    // point is near): lane j holds plane j for the batch balls
+    // point is near): lane j holds plane j for the batch balls; dl: their distances from
+    // the batch's start point
     const float4 pl = kRegion ? rp[lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float *dl = (float *)(rp + 64);
     const uint32_t d0 = rng.draw;
     uint32_t d = d0;
     double spare = 0.0;
@@ -1899,8 +1922,7 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
             // plane (the region's planes are all the scan's candidates); flights that leave
             // it take the full region test
             const float cx = (float)ph.pos[0], cy = (float)ph.pos[1], cz = (float)ph.pos[2];
-            float r = pl.x * cx + pl.y * cy + pl.z * cz - pl.w;
-            for (int m = 1; m < 64; m <<= 1) r = fminf(r, __shfl_xor(r, m, 64));
+            const float r = region_dists(pl, cx, cy, cz, dl);
             ph.bc[0] = cx;
             ph.bc[1] = cy;
             ph.bc[2] = cz;
@@ -1939,8 +1961,15 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
             bool safe = in_ball(e);
             if (kRegion && !safe) {
                 double q[3];
+                float reach = 0.0f;
                 for (int k = 0; k < 3; ++k) q[k] = e.pos[k] + fi * di3[k];
-                safe = in_region(rp, nr, rm, e.pos, q);
+                for (int w = 0; w < 2; ++w) {
+                    const double *x = w ? q : e.pos;
+                    const float ax = (float)x[0] - ph.bc[0], ay = (float)x[1] - ph.bc[1],
+                                az = (float)x[2] - ph.bc[2];
+                    reach = fmaxf(reach, __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az));
+                }
+                safe = in_region(rp, dl, nr, rm, reach * (1.0f + 1.0e-6f), e.pos, q);
             }
             const bool ok = inside_box(g, e.pos) && ph.events + lane < event_limit && safe;
             const unsigned long long okm = __ballot(ok) | (s0 == 0 ? ~0xffffull : 0xffffull);
@@ -2469,7 +2498,11 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
                     }
                     double q[3];
                     for (int k = 0; k < 3; ++k) q[k] = ph.pos[k] + ph.fp * ph.dir[k];
-                    if (in_region(rl, nr, P.g.region_margin, ph.pos, q))
+                    region_dists(rl[lane], (float)ph.pos[0], (float)ph.pos[1], (float)ph.pos[2],
+                                 (float *)(rl + 64));
+                    wave_lds_sync();
+                    if (in_region(rl, (float *)(rl + 64), nr, P.g.region_margin,
+                                  (float)ph.fp * (1.0f + 1.0e-6f), ph.pos, q))
                         tail_ball_run<kPrec, kAdj, true>(P.g, P.event_limit, rng, ph, al, cb, rl, nr,
                                                          P.g.region_margin);
                 }
