@@ -1261,38 +1261,37 @@ __device__ int lone_scan_planes(const DevGeom &g, int which, const double *pos, 
 // its region by the margin: the flight then crosses no candidate's plane, and the scan
 // would find no surface within it (the reference's FP64 plane distances, whose rounding
 // is far below the margin, are then negative, beyond the flight, or of planes facing away).
-// Only the planes that such a flight can reach are tested: dl[j] is plane j's signed
-// distance from a point c, and both ends lie within `reach` of c, so a plane with
-// dl[j] > reach + 2m keeps both ends beyond m (the FP32 rounding of dl and reach is far
-// below m).
-__device__ inline bool in_region(const float4 *rp, const float *dl, int nr, float m, float reach,
+// Wave-uniform control flow, every lane active.  Only the planes that the flights of the
+// asking lanes (ask) can reach are tested: lane j's dj is plane j's signed distance from a
+// point c, and both ends of a flight lie within `reach` of c, so a plane with
+// dj > reach + 2m keeps both ends beyond m (the FP32 rounding of dj and reach is far below
+// m).  The planes within the largest reach of the asking lanes are selected by a ballot
+// and read from rp (LDS).  Returns the verdict of the asking lanes (false for the others).
+__device__ inline bool in_region(const float4 *rp, float dj, bool ask, float m, float reach,
                                  const double *p, const double *e)
 {
+    float rmax = ask ? reach : 0.0f;
+    for (int k = 1; k < 64; k <<= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, k, 64));
+    unsigned long long near = __ballot(dj <= rmax + 2.0f * m);
     const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
     const float ex = (float)e[0], ey = (float)e[1], ez = (float)e[2];
-    const float lim = reach + 2.0f * m;
-    bool ok = true;
-    for (int j = 0; j < nr; ++j) {
-        if (dl[j] <= lim) {
-            const float4 q = rp[j];
-            const float a = q.x * px + q.y * py + q.z * pz - q.w;
-            const float b = q.x * ex + q.y * ey + q.z * ez - q.w;
-            ok = ok && a > m && b > m;
-        }
+    bool ok = ask;
+    while (near != 0ull) {
+        const int j = (int)__builtin_ctzll(near);
+        near &= near - 1ull;
+        const float4 q = rp[j];
+        const float a = q.x * px + q.y * py + q.z * pz - q.w;
+        const float b = q.x * ex + q.y * ey + q.z * ez - q.w;
+        ok = ok && a > m && b > m;
     }
     return ok;
 }
 
-// Signed distances of the point c to the region's planes, lane j taking plane pl = plane
-// j (lanes past the region hold a plane no point is near), into dl; returns their minimum
-// on every lane.
-__device__ inline float region_dists(const float4 &pl, float cx, float cy, float cz, float *dl)
+// Lane j's signed distance from the point c to plane pl = plane j of a region (lanes past
+// the region hold a plane no point is near).
+__device__ inline float region_dist(const float4 &pl, float cx, float cy, float cz)
 {
-    const float d = pl.x * cx + pl.y * cy + pl.z * cz - pl.w;
-    dl[threadIdx.x & 63] = d;
-    float r = d;
-    for (int m = 1; m < 64; m <<= 1) r = fminf(r, __shfl_xor(r, m, 64));
-    return r;
+    return pl.x * cx + pl.y * cy + pl.z * cz - pl.w;
 }
 
 // Lower bound on the distance from a point to the surfaces of one BVH leaf, as in
@@ -1844,9 +1843,8 @@ constexpr int kFewPaths = 4;
 // read by one ds_read_b128), and the positions at which the events begin (pb, 65 used).
 constexpr int kChainXn = 3 * 64 * 2;
 constexpr int kChainPb = 68;
-// then the planes of the path's region (at most 64, one float4 each) and their distances
-// from a point (64 floats)
-constexpr int kChainWords = kChainXn + 3 * kChainPb + 2 * 64 + 32;
+// then the planes of the path's region (at most 64, one float4 each)
+constexpr int kChainWords = kChainXn + 3 * kChainPb + 2 * 64;
 
 // Makes this wave's LDS stores visible to its own later loads by other lanes.
 __device__ inline void wave_lds_sync()
@@ -1885,7 +1883,7 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
     // point is near): lane j holds plane j for the batch balls; dl: their distances from
     // the batch's start point
     const float4 pl = kRegion ? rp[lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float *dl = (float *)(rp + 64);
+    float dj = 3.0e38f;            // this lane's plane: distance from the batch's start point
     const uint32_t d0 = rng.draw;
     uint32_t d = d0;
     double spare = 0.0;
@@ -1922,7 +1920,9 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
             // plane (the region's planes are all the scan's candidates); flights that leave
             // it take the full region test
             const float cx = (float)ph.pos[0], cy = (float)ph.pos[1], cz = (float)ph.pos[2];
-            const float r = region_dists(pl, cx, cy, cz, dl);
+            dj = region_dist(pl, cx, cy, cz);
+            float r = dj;
+            for (int m = 1; m < 64; m <<= 1) r = fminf(r, __shfl_xor(r, m, 64));
             ph.bc[0] = cx;
             ph.bc[1] = cy;
             ph.bc[2] = cz;
@@ -1959,7 +1959,7 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
             e.fp = fi;
             e.br = ph.br;
             bool safe = in_ball(e);
-            if (kRegion && !safe) {
+            if (kRegion && __any(!safe && lane >= s0 && lane < s1)) {
                 double q[3];
                 float reach = 0.0f;
                 for (int k = 0; k < 3; ++k) q[k] = e.pos[k] + fi * di3[k];
@@ -1969,7 +1969,11 @@ __device__ void tail_ball_run(const DevGeom &g, int64_t event_limit, Rng &rng, P
                                 az = (float)x[2] - ph.bc[2];
                     reach = fmaxf(reach, __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az));
                 }
-                safe = in_region(rp, dl, nr, rm, reach * (1.0f + 1.0e-6f), e.pos, q);
+                // the lanes of this step only (the others hold positions of no event yet)
+                const bool ask = !safe && lane >= s0 && lane < s1;
+                // every lane calls it (it reduces over the wave), whatever its own verdict
+                const bool inr = in_region(rp, dj, ask, rm, reach * (1.0f + 1.0e-6f), e.pos, q);
+                safe = safe || inr;
             }
             const bool ok = inside_box(g, e.pos) && ph.events + lane < event_limit && safe;
             const unsigned long long okm = __ballot(ok) | (s0 == 0 ? ~0xffffull : 0xffffull);
@@ -2498,11 +2502,10 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
                     }
                     double q[3];
                     for (int k = 0; k < 3; ++k) q[k] = ph.pos[k] + ph.fp * ph.dir[k];
-                    region_dists(rl[lane], (float)ph.pos[0], (float)ph.pos[1], (float)ph.pos[2],
-                                 (float *)(rl + 64));
-                    wave_lds_sync();
-                    if (in_region(rl, (float *)(rl + 64), nr, P.g.region_margin,
-                                  (float)ph.fp * (1.0f + 1.0e-6f), ph.pos, q))
+                    const float dj = region_dist(rl[lane], (float)ph.pos[0], (float)ph.pos[1],
+                                                 (float)ph.pos[2]);
+                    if (in_region(rl, dj, true, P.g.region_margin, (float)ph.fp * (1.0f + 1.0e-6f),
+                                  ph.pos, q))
                         tail_ball_run<kPrec, kAdj, true>(P.g, P.event_limit, rng, ph, al, cb, rl, nr,
                                                          P.g.region_margin);
                 }
