@@ -459,6 +459,70 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
         }
         region_off[ng] = (int32_t)region.size();
     }
+    // start-grain map (Voronoi grains of polygons): a cell of the start face gets grain g
+    // when, on all four corners of the cell widened by a rounding margin, g is the nearest
+    // incident generatrix by 1e-4 nm^2, every facet of g has (p - c).n < -m, and every other
+    // grain has a facet with (p - c).n > m (m = 1e-6 |n|): all of these are affine in p, so
+    // they hold on the whole cell, far beyond the FP64 rounding of the device's tests, and
+    // both start-grain searches return g there
+    std::vector<int32_t> start_map;
+    const int kStartGrid = 64;
+    double start_h[2] = {d->box[1] * d->scope[0] / kStartGrid, d->box[2] * d->scope[1] / kStartGrid};
+    bool map_ok = d->kind_grain == PT_GRAIN_VORONOI && d->ngrain > 0 && start_h[0] > 0.0 &&
+                  start_h[1] > 0.0;
+    for (int gi = 1; gi <= d->ngrain && map_ok; ++gi)
+        for (int j = cand_off[gi]; j < cand_off[gi + 1] && map_ok; ++j)
+            map_ok = d->surfaces[cand_idx[j]].kind == PT_SHAPE_POLYGON;
+    if (map_ok) {
+        start_map.assign((size_t)kStartGrid * kStartGrid, 0);
+        auto facet = [&](int j, const double *p) {
+            const pt_surface &sf = d->surfaces[cand_idx[j]];
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += (p[k] - sf.center[k]) * sf.normal[k];
+            return v;
+        };
+        auto fmarg = [&](int j) {
+            const pt_surface &sf = d->surfaces[cand_idx[j]];
+            return 1.0e-6 * std::sqrt(sf.normal[0] * sf.normal[0] + sf.normal[1] * sf.normal[1] +
+                                      sf.normal[2] * sf.normal[2]);
+        };
+        auto d2 = [&](int gi, const double *p) {
+            const double *q = d->generatrix + 3 * (size_t)gi;
+            return (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
+        };
+        for (int iy = 0; iy < kStartGrid; ++iy)
+            for (int iz = 0; iz < kStartGrid; ++iz) {
+                double cs[4][3];
+                for (int c4 = 0; c4 < 4; ++c4) {
+                    const double ey = 1.0e-6 * start_h[0] + 1.0e-9, ez = 1.0e-6 * start_h[1] + 1.0e-9;
+                    cs[c4][0] = 0.0;
+                    cs[c4][1] = (c4 & 1) ? (iy + 1) * start_h[0] + ey : iy * start_h[0] - ey;
+                    cs[c4][2] = (c4 & 2) ? (iz + 1) * start_h[1] + ez : iz * start_h[1] - ez;
+                }
+                int best = 0;
+                double bd = 1.0e10;
+                for (int gi = 1; gi <= d->ngrain; ++gi)
+                    if (d->grain_incident[gi] && bd > d2(gi, cs[0])) { bd = d2(gi, cs[0]); best = gi; }
+                bool ok = best > 0;
+                for (int gi = 1; gi <= d->ngrain && ok; ++gi) {
+                    if (gi == best) {
+                        for (int j = cand_off[gi]; j < cand_off[gi + 1] && ok; ++j)
+                            for (int c4 = 0; c4 < 4 && ok; ++c4) ok = facet(j, cs[c4]) < -fmarg(j);
+                        continue;
+                    }
+                    if (d->grain_incident[gi])
+                        for (int c4 = 0; c4 < 4 && ok; ++c4) ok = d2(gi, cs[c4]) - d2(best, cs[c4]) > 1.0e-4;
+                    bool out = false;
+                    for (int j = cand_off[gi]; j < cand_off[gi + 1] && !out; ++j) {
+                        bool all = true;
+                        for (int c4 = 0; c4 < 4 && all; ++c4) all = facet(j, cs[c4]) > fmarg(j);
+                        out = all;
+                    }
+                    ok = ok && out;
+                }
+                start_map[(size_t)iy * kStartGrid + iz] = ok ? best : 0;
+            }
+    }
     std::vector<int32_t> ginc(d->grain_incident, d->grain_incident + ng);
     std::vector<double> gen(d->generatrix, d->generatrix + 3 * (size_t)ng);
     std::vector<double> vtx(d->vertices, d->vertices + 3 * (size_t)d->nvertex);
@@ -472,21 +536,22 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     int32_t *d_co = nullptr, *d_ci = nullptr, *d_io = nullptr, *d_ii = nullptr, *d_ginc = nullptr;
     CandPlane *d_cp = nullptr;
     float4 *d_rg = nullptr;
-    int32_t *d_ro = nullptr;
+    int32_t *d_ro = nullptr, *d_sm = nullptr;
     if ((rc = upload(surf, &d_surf, c->stream)) || (rc = upload(tang, &d_tang, c->stream)) ||
         (rc = upload(vtx, &d_vtx, c->stream)) || (rc = upload(gen, &d_gen, c->stream)) ||
         (rc = upload(cand_off, &d_co, c->stream)) || (rc = upload(cand_idx, &d_ci, c->stream)) ||
         (rc = upload(inc_off, &d_io, c->stream)) || (rc = upload(inc_idx, &d_ii, c->stream)) ||
         (rc = upload(ginc, &d_ginc, c->stream)) || (rc = upload(cplane, &d_cp, c->stream)) ||
-        (rc = upload(region, &d_rg, c->stream)) || (rc = upload(region_off, &d_ro, c->stream))) {
+        (rc = upload(region, &d_rg, c->stream)) || (rc = upload(region_off, &d_ro, c->stream)) ||
+        (rc = upload(start_map, &d_sm, c->stream))) {
         for (void *p : {(void *)d_surf, (void *)d_tang, (void *)d_vtx, (void *)d_gen, (void *)d_co,
                         (void *)d_ci, (void *)d_io, (void *)d_ii, (void *)d_ginc, (void *)d_cp,
-                        (void *)d_rg, (void *)d_ro})
+                        (void *)d_rg, (void *)d_ro, (void *)d_sm})
             if (p) (void)hipFree(p);
         delete G;
         return rc;
     }
-    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc, d_cp, d_rg, d_ro};
+    G->owned = {d_surf, d_tang, d_vtx, d_gen, d_co, d_ci, d_io, d_ii, d_ginc, d_cp, d_rg, d_ro, d_sm};
     G->surf = surf;
     DevGeom &g = G->dg;
     g.surf = d_surf;
@@ -506,6 +571,10 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     // FP32 region tests: coordinates below the box's largest side, a few roundings of
     // 2^-24 of it each; the margin is far above them and the reference's FP64 rounding
     g.region_margin = (float)(1.0e-2 + 2.0e-5 * std::max(d->box[0], std::max(d->box[1], d->box[2])));
+    g.start_map = d_sm;
+    g.start_grid = d_sm ? kStartGrid : 0;
+    g.start_h[0] = start_h[0];
+    g.start_h[1] = start_h[1];
     for (int k = 0; k < 3; ++k) g.box[k] = d->box[k];
     g.scope[0] = d->scope[0];
     g.scope[1] = d->scope[1];
@@ -737,6 +806,8 @@ TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
     P.precision = PT_PREC_FP64;
     P.queue = c->counters;
     P.stats = c->counters + 1;
+    // the op-counting kernels run both start-grain searches, as the reference does
+    if (c->count_ops) P.g.start_map = nullptr;
     return P;
 }
 
@@ -770,6 +841,7 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
     P.tail_score = ngroup > 0 ? c->tail_score : nullptr;
     P.tail_cap = cap;
     P.tail_cap_p = cap / 4;
+    P.tail_prio_mfp = kTailPrioMfpFrac * std::min(P.g.box[0], std::min(P.g.box[1], P.g.box[2]));
     return PT_OK;
 }
 
@@ -1257,6 +1329,7 @@ int pt_trace_alias_noalloc(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64
         P.tail = (TailRec *)(w + kNoAllocTailWord);
         P.tail_cap = std::min<int64_t>(c->tail_max, kNoAllocTailRecs);
         P.tail_cap_p = P.tail_cap / 4;
+        P.tail_prio_mfp = kTailPrioMfpFrac * std::min(P.g.box[0], std::min(P.g.box[1], P.g.box[2]));
     }
     unsigned long long host_ctr[kCounterWords] = {0};
     float ms = 0.f;

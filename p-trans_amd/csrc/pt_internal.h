@@ -60,6 +60,12 @@ struct DevGeom {
     const float4 *region;
     const int32_t *region_off;
     float region_margin;       // distance to every plane that a region test requires (nm)
+    // Voronoi geometries of polygon grains: the start grain of every point of a cell of a
+    // start_grid x start_grid grid over the start face (y, z), or 0 where the cell needs
+    // the full searches (findNearestIncidentGrain and the start-grain check)
+    const int32_t *start_map;
+    int32_t start_grid;
+    double start_h[2];         // cell sides in y and z
     double box[3];
     double scope[2];
     int32_t ngrain;
@@ -257,6 +263,7 @@ struct TraceParams {
     double *tail_score;        // [tail_cap * ngroup] (adjoint and batch modes)
     int64_t tail_cap;          // records in all; the first tail_cap_p are the priority part
     int64_t tail_cap_p;
+    double tail_prio_mfp;      // paths below this mean free path take priority records
     int64_t tail_events;
 };
 

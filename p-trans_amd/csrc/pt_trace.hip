@@ -1514,16 +1514,29 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     ph.events = 0;
     ph.br = -1.0f;
     int which = 0;
-    if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
-    else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos, oc);
-    else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_MESH)
-        which = bt ? batch_start_grain(g, *bt, ph.sid, ph.pos, oc) : mesh_start_grain(g, ph.pos, oc);
-    ph.which = which;
-    ph.start_grain = which;
-    if (g.kind_grain == PT_GRAIN_MESH ? which < 0
-                                      : find_inside_which_grain(g, ph.pos, oc) != which) {  // RT:1673-1682
-        ph.flags = PT_PATH_START_MISMATCH;
-        return false;
+    if (g.start_map) {
+        // a cell where both start-grain searches certainly return the same grain
+        const int G = g.start_grid;
+        int iy = (int)(ph.pos[1] / g.start_h[0]), iz = (int)(ph.pos[2] / g.start_h[1]);
+        iy = iy < 0 ? 0 : (iy >= G ? G - 1 : iy);
+        iz = iz < 0 ? 0 : (iz >= G ? G - 1 : iz);
+        which = g.start_map[iy * G + iz];
+    }
+    if (which > 0) {
+        ph.which = which;
+        ph.start_grain = which;
+    } else {
+        if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_VORONOI) which = nearest_incident_grain(g, ph.pos, oc);
+        else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_BRICKS) which = incident_collision_grain(g, ph.pos, oc);
+        else if (g.ngrain > 0 && g.kind_grain == PT_GRAIN_MESH)
+            which = bt ? batch_start_grain(g, *bt, ph.sid, ph.pos, oc) : mesh_start_grain(g, ph.pos, oc);
+        ph.which = which;
+        ph.start_grain = which;
+        if (g.kind_grain == PT_GRAIN_MESH ? which < 0
+                                          : find_inside_which_grain(g, ph.pos, oc) != which) {  // RT:1673-1682
+            ph.flags = PT_PATH_START_MISMATCH;
+            return false;
+        }
     }
     if (ph.mfp == kBallistic) ph.fp = kBallistic;                           // RT:1686-1690
     else if (kPrec == kPrecFast) ph.fp = free_path32(ph.mfp, rng.next());
@@ -2180,38 +2193,38 @@ __device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, Ad
 {
     constexpr bool kBatch = kMode == TRACE_BATCH;
     constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
-    const bool prio = ph.mfp < kTailPrioMfpFrac * fmin(P.g.box[0], fmin(P.g.box[1], P.g.box[2]));
+    const bool prio = ph.mfp < P.tail_prio_mfp;
     unsigned long long slot = atomicAdd(P.queue + (prio ? kCtrTailP : kCtrTailN), 1ull);
     if (slot >= (unsigned long long)(prio ? P.tail_cap_p : P.tail_cap - P.tail_cap_p)) return false;
     if (!prio) slot += (unsigned long long)P.tail_cap_p;
-    TailRec r;
+    // field by field from the path's registers (a whole record assembled first would need
+    // 48 more live registers at this point of the loop)
+    TailRec *r = P.tail + slot;
     for (int k = 0; k < 3; ++k) {
-        r.pos[k] = ph.pos[k];
-        r.dir[k] = ph.dir[k];
-        r.bc[k] = ph.bc[k];
+        r->pos[k] = ph.pos[k];
+        r->dir[k] = ph.dir[k];
+        r->bc[k] = ph.bc[k];
     }
-    r.fp = ph.fp;
-    r.mfp = ph.mfp;
-    r.omega = ph.omega;
-    r.spare = rng.spare;
-    r.len = al.len;
-    r.fdx = al.fdx;
-    r.fl = al.fl;
-    r.fw = al.fw;
-    r.pid = pid;
-    r.events = ph.events;
-    r.br = ph.br;
-    r.which = ph.which;
-    r.cur = ph.cur;
-    r.flags = ph.flags & ~kPathTop;
-    r.start_grain = ph.start_grain;
-    r.mode = mode;
-    r.sid = ph.sid;
-    r.nscat = al.nscat;
-    r.fgrain = (kAdj & 1) ? al.fgrain : -1;
-    r.draw = rng.draw;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0;
-    P.tail[slot] = r;
+    r->fp = ph.fp;
+    r->mfp = ph.mfp;
+    r->omega = ph.omega;
+    r->spare = rng.spare;
+    r->len = al.len;
+    r->fdx = al.fdx;
+    r->fl = al.fl;
+    r->fw = al.fw;
+    r->pid = pid;
+    r->events = ph.events;
+    r->br = ph.br;
+    r->which = ph.which;
+    r->cur = ph.cur;
+    r->flags = ph.flags & ~kPathTop;
+    r->start_grain = ph.start_grain;
+    r->mode = mode;
+    r->sid = ph.sid;
+    r->nscat = al.nscat;
+    r->fgrain = (kAdj & 1) ? al.fgrain : -1;
+    r->draw = rng.draw;
     if (kAdj & 2)
         for (int gi = 0; gi < P.ngroup; ++gi) {
             P.tail_score[slot * P.ngroup + gi] = al.score[gi * al.stride];

@@ -4,12 +4,13 @@
     python -m ptrans.profiling counters <run_results.db> [kernel-substring]
     python -m ptrans.profiling traffic  <kt.db> <fetch.db> <write.db> <copy_fetch.db>
                                         <copy_write.db> <sq.db> <copy_mib> <paths> <workload>
-                                        [key=kernel-substring ...] [tcc=<tcc.db>]
+                                        [key=kernel-substring[+kernel-substring] ...] [tcc=<tcc.db>]
 
 'kernels' prints per-kernel call count and average/total duration; 'counters' prints,
 per kernel and counter, the mean value per dispatch.  'traffic' prints the JSON summary
 the bench's roofline reads: per trace-kernel precision its average duration (kernel
-trace), HBM bytes per launch (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the
+trace; a key naming "a+b" sums the kernels of one launch), HBM bytes per launch (FETCH_SIZE
+and WRITE_SIZE in KiB, each scaled by what the
 same counter reads for a device copy of copy_mib MiB) and VALU wave-instructions per
 launch and per path with the issue fraction over 256 CUs x 2.4 GHz; with a TCC_HIT_sum /
 TCC_MISS_sum run (tcc=...) also the L2 hit rate.
@@ -65,21 +66,27 @@ def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy
     kt = {k["name"]: k for k in kernel_stats(kt_db)}
     out = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale,
            "peak_valu_wave_instr_per_s": PEAK_VALU_WAVE_INSTR_PER_S, "kernels": {}}
-    for key, sub in (kernels or TRACE_KERNELS).items():
-        name = next(n for n in kt if sub in n)
-        avg_us = kt[name]["average_ns"] / 1e3
-        _, f = one(fetch_db, "FETCH_SIZE", sub)
-        _, w = one(write_db, "WRITE_SIZE", sub)
-        _, v = one(sq_db, "SQ_INSTS_VALU", sub)
+    for key, subs in (kernels or TRACE_KERNELS).items():
+        # a launch of several kernels (the trace kernel, then the tail kernel): "a+b" sums
+        # their durations, bytes and instructions
+        names, avg_us, f, w, v, hit, miss = [], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
+        for sub in subs.split("+"):
+            name = next(n for n in kt if sub in n)
+            names.append(name)
+            avg_us += kt[name]["average_ns"] / 1e3
+            f += one(fetch_db, "FETCH_SIZE", sub)[1]
+            w += one(write_db, "WRITE_SIZE", sub)[1]
+            v += one(sq_db, "SQ_INSTS_VALU", sub)[1]
+            if tcc_db:
+                hit += one(tcc_db, "TCC_HIT_sum", sub)[1]
+                miss += one(tcc_db, "TCC_MISS_sum", sub)[1]
         fb, wb = f * 1024.0 * fscale, w * 1024.0 * wscale
         out["kernels"][key] = {
-            "kernel": name, "average_us": avg_us,
+            "kernel": " + ".join(names), "average_us": avg_us,
             "hbm_fetch_bytes": fb, "hbm_write_bytes": wb, "hbm_bytes_per_launch": fb + wb,
             "valu_wave_instr_per_launch": v, "valu_wave_instr_per_path": v / paths,
             "valu_issue_fraction": v / (avg_us * 1e-6) / PEAK_VALU_WAVE_INSTR_PER_S}
         if tcc_db:
-            _, hit = one(tcc_db, "TCC_HIT_sum", sub)
-            _, miss = one(tcc_db, "TCC_MISS_sum", sub)
             out["kernels"][key]["l2_hit_rate"] = hit / max(1.0, hit + miss)
     return out
 
