@@ -459,66 +459,96 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
         }
         region_off[ng] = (int32_t)region.size();
     }
-    // start-grain map (Voronoi grains of polygons): a cell of the start face gets grain g
-    // when, on all four corners of the cell widened by a rounding margin, g is the nearest
-    // incident generatrix by 1e-4 nm^2, every facet of g has (p - c).n < -m, and every other
-    // grain has a facet with (p - c).n > m (m = 1e-6 |n|): all of these are affine in p, so
-    // they hold on the whole cell, far beyond the FP64 rounding of the device's tests, and
-    // both start-grain searches return g there
+    // start-grain map: a cell of the start face (y, z at x = 0) gets grain g when, on the
+    // whole cell widened by a rounding margin, both start-grain searches certainly return
+    // g.  Voronoi: on all four corners g is the nearest incident generatrix by 1e-4 nm^2
+    // (the differences of squared distances are affine in p).  Bricks whose grains are
+    // none incident: the fill grain.  Then g's candidates are all certainly inside and
+    // every other grain has a candidate certainly outside (polygons: (p - c).n, affine,
+    // beyond m = 1e-6 |n| on all corners; spheres and z-cylinders: the squared distance to
+    // the centre (axis) over the cell, at least the box distance and at most the farthest
+    // corner's, beyond r^2 -+ 1e-6 (r^2 + 1)).  The margins are far beyond the FP64
+    // rounding of the device's tests.
     std::vector<int32_t> start_map;
     const int kStartGrid = 64;
     double start_h[2] = {d->box[1] * d->scope[0] / kStartGrid, d->box[2] * d->scope[1] / kStartGrid};
-    bool map_ok = d->kind_grain == PT_GRAIN_VORONOI && d->ngrain > 0 && start_h[0] > 0.0 &&
-                  start_h[1] > 0.0;
+    bool any_incident = false;
+    for (int gi = 1; gi <= d->ngrain; ++gi) any_incident = any_incident || d->grain_incident[gi];
+    const bool map_voronoi = d->kind_grain == PT_GRAIN_VORONOI;
+    bool map_ok = (map_voronoi || (d->kind_grain == PT_GRAIN_BRICKS && !any_incident &&
+                                   d->igrain_fill == d->ngrain + 1)) &&
+                  d->ngrain > 0 && start_h[0] > 0.0 && start_h[1] > 0.0;
     for (int gi = 1; gi <= d->ngrain && map_ok; ++gi)
-        for (int j = cand_off[gi]; j < cand_off[gi + 1] && map_ok; ++j)
-            map_ok = d->surfaces[cand_idx[j]].kind == PT_SHAPE_POLYGON;
+        for (int j = cand_off[gi]; j < cand_off[gi + 1] && map_ok; ++j) {
+            const int kd = d->surfaces[cand_idx[j]].kind;
+            map_ok = kd == PT_SHAPE_POLYGON || (!map_voronoi && (kd == PT_SHAPE_SPHERE || kd == PT_SHAPE_CYLINDER));
+        }
     if (map_ok) {
         start_map.assign((size_t)kStartGrid * kStartGrid, 0);
-        auto facet = [&](int j, const double *p) {
-            const pt_surface &sf = d->surfaces[cand_idx[j]];
-            double v = 0.0;
-            for (int k = 0; k < 3; ++k) v += (p[k] - sf.center[k]) * sf.normal[k];
-            return v;
-        };
-        auto fmarg = [&](int j) {
-            const pt_surface &sf = d->surfaces[cand_idx[j]];
-            return 1.0e-6 * std::sqrt(sf.normal[0] * sf.normal[0] + sf.normal[1] * sf.normal[1] +
-                                      sf.normal[2] * sf.normal[2]);
-        };
         auto d2 = [&](int gi, const double *p) {
             const double *q = d->generatrix + 3 * (size_t)gi;
             return (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
         };
+        // +1: candidate j certainly inside (inside_surface true) on the cell, -1: certainly
+        // outside, 0: unknown
+        auto side = [&](int j, double cs[4][3], const double *lo, const double *hi) {
+            const pt_surface &sf = d->surfaces[cand_idx[j]];
+            if (sf.kind == PT_SHAPE_POLYGON) {
+                const double m = 1.0e-6 * std::sqrt(sf.normal[0] * sf.normal[0] + sf.normal[1] * sf.normal[1] +
+                                                    sf.normal[2] * sf.normal[2]);
+                double vmin = 1e300, vmax = -1e300;
+                for (int c4 = 0; c4 < 4; ++c4) {
+                    double v = 0.0;
+                    for (int k = 0; k < 3; ++k) v += (cs[c4][k] - sf.center[k]) * sf.normal[k];
+                    vmin = std::min(vmin, v);
+                    vmax = std::max(vmax, v);
+                }
+                return vmax < -m ? 1 : (vmin > m ? -1 : 0);
+            }
+            const int nk = sf.kind == PT_SHAPE_SPHERE ? 3 : 2;
+            double qmin = 0.0, qmax = 0.0;
+            for (int k = 0; k < nk; ++k) {
+                const double a = lo[k] - sf.center[k], b = hi[k] - sf.center[k];
+                const double near = (a <= 0.0 && b >= 0.0) ? 0.0 : std::min(std::fabs(a), std::fabs(b));
+                const double far = std::max(std::fabs(a), std::fabs(b));
+                qmin += near * near;
+                qmax += far * far;
+            }
+            const double r2 = sf.radius * sf.radius, m = 1.0e-6 * (r2 + 1.0);
+            return qmax < r2 - m ? 1 : (qmin > r2 + m ? -1 : 0);
+        };
         for (int iy = 0; iy < kStartGrid; ++iy)
             for (int iz = 0; iz < kStartGrid; ++iz) {
+                const double ey = 1.0e-6 * start_h[0] + 1.0e-9, ez = 1.0e-6 * start_h[1] + 1.0e-9;
+                const double lo[3] = {0.0, iy * start_h[0] - ey, iz * start_h[1] - ez};
+                const double hi[3] = {0.0, (iy + 1) * start_h[0] + ey, (iz + 1) * start_h[1] + ez};
                 double cs[4][3];
                 for (int c4 = 0; c4 < 4; ++c4) {
-                    const double ey = 1.0e-6 * start_h[0] + 1.0e-9, ez = 1.0e-6 * start_h[1] + 1.0e-9;
                     cs[c4][0] = 0.0;
-                    cs[c4][1] = (c4 & 1) ? (iy + 1) * start_h[0] + ey : iy * start_h[0] - ey;
-                    cs[c4][2] = (c4 & 2) ? (iz + 1) * start_h[1] + ez : iz * start_h[1] - ez;
+                    cs[c4][1] = (c4 & 1) ? hi[1] : lo[1];
+                    cs[c4][2] = (c4 & 2) ? hi[2] : lo[2];
                 }
                 int best = 0;
-                double bd = 1.0e10;
-                for (int gi = 1; gi <= d->ngrain; ++gi)
-                    if (d->grain_incident[gi] && bd > d2(gi, cs[0])) { bd = d2(gi, cs[0]); best = gi; }
-                bool ok = best > 0;
+                bool ok = true;
+                if (map_voronoi) {
+                    double bd = 1.0e10;
+                    for (int gi = 1; gi <= d->ngrain; ++gi)
+                        if (d->grain_incident[gi] && bd > d2(gi, cs[0])) { bd = d2(gi, cs[0]); best = gi; }
+                    ok = best > 0;
+                    for (int gi = 1; gi <= d->ngrain && ok; ++gi)
+                        if (gi != best && d->grain_incident[gi])
+                            for (int c4 = 0; c4 < 4 && ok; ++c4) ok = d2(gi, cs[c4]) - d2(best, cs[c4]) > 1.0e-4;
+                } else {
+                    best = d->igrain_fill;
+                }
                 for (int gi = 1; gi <= d->ngrain && ok; ++gi) {
                     if (gi == best) {
-                        for (int j = cand_off[gi]; j < cand_off[gi + 1] && ok; ++j)
-                            for (int c4 = 0; c4 < 4 && ok; ++c4) ok = facet(j, cs[c4]) < -fmarg(j);
-                        continue;
+                        for (int j = cand_off[gi]; j < cand_off[gi + 1] && ok; ++j) ok = side(j, cs, lo, hi) == 1;
+                    } else {
+                        bool out = false;
+                        for (int j = cand_off[gi]; j < cand_off[gi + 1] && !out; ++j) out = side(j, cs, lo, hi) == -1;
+                        ok = out;
                     }
-                    if (d->grain_incident[gi])
-                        for (int c4 = 0; c4 < 4 && ok; ++c4) ok = d2(gi, cs[c4]) - d2(best, cs[c4]) > 1.0e-4;
-                    bool out = false;
-                    for (int j = cand_off[gi]; j < cand_off[gi + 1] && !out; ++j) {
-                        bool all = true;
-                        for (int c4 = 0; c4 < 4 && all; ++c4) all = facet(j, cs[c4]) > fmarg(j);
-                        out = all;
-                    }
-                    ok = ok && out;
                 }
                 start_map[(size_t)iy * kStartGrid + iz] = ok ? best : 0;
             }
