@@ -203,8 +203,8 @@ struct alignas(16) TailRec {
     double fdx, fl, fw;         // grain tallies: the open grain's sums and 1 / mfp_m
     int64_t pid, events;
     float bc[3], br;
-    int32_t which, cur, flags, start_grain;
-    int32_t mode, sid, nscat, fgrain;
+    int32_t which, cur, flags, pad0;
+    int32_t mode, sid, nscat, fgrain;     // sid: batch traces only
     uint32_t draw;
     int32_t pad[3];
 };

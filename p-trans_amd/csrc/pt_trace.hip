@@ -2124,9 +2124,9 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
 // lane's statistics.  adj_acc: the per-batch sums (the workgroup's LDS accumulators).
 template <int kMode>
 __device__ inline void end_path(const TraceParams &P, const Path &ph, const Rng &rng, AdjLane &al,
-                                double *adj_acc, int adj_k, int mode, int64_t pid,
-                                unsigned long long *st)
+                                double *adj_acc, int adj_k, int mode, unsigned long long *st)
 {
+    const int64_t pid = (int64_t)rng.path;          // the Rng is keyed by the path's id
     constexpr bool kBatch = kMode == TRACE_BATCH;
     constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
     // StartSide /= EndSide (RT:1833): lost paths count, as in the reference; a path stopped
@@ -2189,7 +2189,7 @@ constexpr int kPathStay = 1 << 29;
 // False when the buffer is full (the path then stays).
 template <int kMode>
 __device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, AdjLane &al,
-                        int mode, int64_t pid)
+                        int mode)
 {
     constexpr bool kBatch = kMode == TRACE_BATCH;
     constexpr int kAdj = kMode == TRACE_ADJOINT || kBatch ? 2 : (kMode == TRACE_GRAINS ? 1 : 0);
@@ -2213,15 +2213,14 @@ __device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, Ad
     r->fdx = al.fdx;
     r->fl = al.fl;
     r->fw = al.fw;
-    r->pid = pid;
+    r->pid = (int64_t)rng.path;
     r->events = ph.events;
     r->br = ph.br;
     r->which = ph.which;
     r->cur = ph.cur;
     r->flags = ph.flags & ~kPathTop;
-    r->start_grain = ph.start_grain;
     r->mode = mode;
-    r->sid = ph.sid;
+    if (kBatch) r->sid = ph.sid;
     r->nscat = al.nscat;
     r->fgrain = (kAdj & 1) ? al.fgrain : -1;
     r->draw = rng.draw;
@@ -2302,7 +2301,6 @@ trace_kernel(TraceParams P)
     rng.seed = P.seed;
     Path ph;
     bool active = false;
-    int64_t pid = 0;
     int mode = 0;
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
@@ -2350,7 +2348,7 @@ trace_kernel(TraceParams P)
                 const int nneed = __popcll(need);
                 bool fresh = false;
                 if (!active && rank < avail) {
-                    pid = chunk_next + rank;
+                    const int64_t pid = chunk_next + rank;
                     al.nscat = 0;
                     al.len = 0.0;
                     if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
@@ -2380,7 +2378,7 @@ trace_kernel(TraceParams P)
             const bool few = exhausted && __popcll(__ballot(active)) <= kFewPaths;
             if (active && ph.end_side == 0 && !(ph.flags & kPathStay) &&
                 (few || ph.events >= P.tail_events)) {
-                if (handoff<kMode>(P, ph, rng, al, mode, pid)) {
+                if (handoff<kMode>(P, ph, rng, al, mode)) {
                     if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
                     active = false;
                 } else {
@@ -2392,7 +2390,7 @@ trace_kernel(TraceParams P)
             if (ph.end_side != 0 ||
                 step_path<kPrec, kAdj, kBatch>(P.g, P.m.max_omega, P.event_limit, rng, ph, al,
                                                &P.bt, stk)) {
-                end_path<kMode>(P, ph, rng, al, adj_acc, adj_k, mode, pid, st);
+                end_path<kMode>(P, ph, rng, al, adj_acc, adj_k, mode, st);
                 if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
                 active = false;
             }
@@ -2484,7 +2482,7 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
         ph.cur = r.cur;
         ph.end_side = 0;
         ph.flags = r.flags;
-        ph.start_grain = r.start_grain;
+        ph.start_grain = -1;
         ph.events = r.events;
         ph.br = r.br;
         ph.sid = r.sid;
@@ -2537,7 +2535,7 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
                                                &P.bt, stk, pre_best, pre_dmin))
                 break;
         }
-        if (lane == 0) end_path<kMode>(P, ph, rng, al, P.adj, adj_k, r.mode, r.pid, st);
+        if (lane == 0) end_path<kMode>(P, ph, rng, al, P.adj, adj_k, r.mode, st);
         wave_lds_sync();
     }
     if (lane == 0)
