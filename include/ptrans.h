@@ -235,9 +235,15 @@ int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t bl
  * set-up and scans spread over the lanes, its in-ball events 64 at a time).  At most
  * max_records such paths per launch (the buffer holds total / 256 records, at least 65536,
  * at most max_records; 0: none, every path stays); the rest stay in the persistent kernel.
- * Defaults: 500 events, 2^20 records.  Results do not depend on these.  No counterpart in
- * the reference, whose OpenMP threads each run one cell's particles to the end (RT:1573). */
-int pt_ctx_set_tail(pt_ctx *ctx, int64_t events, int64_t max_records);
+ * live_blocks workgroups (0..64) of that kernel run beside the persistent one, on a second
+ * stream, and take handed-over paths (the slowest first) while the persistent kernel still
+ * runs; the persistent kernel gets 2 x live_blocks workgroups fewer.  Not for launches
+ * built into HIP graphs (pt_batch_trace_graph) or pt_trace_alias_noalloc, whose tail kernel
+ * always runs after the persistent one.
+ * Defaults: 500 events, 2^20 records, 16 live workgroups.  Results do not depend on these.
+ * No counterpart in the reference, whose OpenMP threads each run one cell's particles to
+ * the end (RT:1573). */
+int pt_ctx_set_tail(pt_ctx *ctx, int64_t events, int64_t max_records, int32_t live_blocks);
 
 /* ---- geometry ------------------------------------------------------------------ */
 /* Uploads vgrain (RT:122) as flattened by the host.  Validates indices; at most

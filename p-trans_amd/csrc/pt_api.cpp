@@ -125,6 +125,15 @@ struct pt_ctx {
     int64_t tail_score_alloc = 0;
     int64_t tail_events = 500;                 // a path past this many events is handed over
     int64_t tail_max = 1 << 20;                // at most this many records per launch
+    uint32_t tail_epoch = 0;                   // launches that wrote tail records so far
+    // the tail kernel beside the persistent one: its workgroups (0: none), stream, events
+    int live_blocks = 16;
+    // grow-only list of the paths the queue runs first (alias-mode launches with a tail)
+    int64_t *first_ids = nullptr;
+    int64_t first_alloc = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    int64_t live_ticks = 0;                    // 20 s of the device's wall clock
 };
 
 struct pt_geom {
@@ -184,7 +193,13 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     c->device = device;
     c->seed = seed;
     c->ncu = prop.multiProcessorCount;
+    int wall_khz = 0;
+    PT_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
+    c->live_ticks = (int64_t)(wall_khz > 0 ? wall_khz : 100000) * 20000;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
@@ -203,6 +218,7 @@ int pt_ctx_destroy(pt_ctx *c)
     if (!c) return PT_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->outcome) (void)hipFree(c->outcome);
     if (c->grid_buf) (void)hipFree(c->grid_buf);
     if (c->host_stage) (void)hipHostFree(c->host_stage);
@@ -211,9 +227,13 @@ int pt_ctx_destroy(pt_ctx *c)
     if (c->work) (void)hipFree(c->work);
     if (c->tail) (void)hipFree(c->tail);
     if (c->tail_score) (void)hipFree(c->tail_score);
+    if (c->first_ids) (void)hipFree(c->first_ids);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PT_OK;
@@ -249,13 +269,16 @@ int pt_ctx_set_tuning(pt_ctx *c, int32_t chunk, int32_t refill_min, int32_t bloc
     return PT_OK;
 }
 
-int pt_ctx_set_tail(pt_ctx *c, int64_t events, int64_t max_records)
+int pt_ctx_set_tail(pt_ctx *c, int64_t events, int64_t max_records, int32_t live_blocks)
 {
     if (!c) return fail(PT_E_INVALID, "pt_ctx_set_tail: NULL context");
-    if (events < 1 || max_records < 0 || max_records > ((int64_t)1 << 26))
-        return fail(PT_E_INVALID, "pt_ctx_set_tail: events >= 1, 0 <= max_records <= 2^26");
+    if (events < 1 || max_records < 0 || max_records > ((int64_t)1 << 26) || live_blocks < 0 ||
+        live_blocks > 64)
+        return fail(PT_E_INVALID, "pt_ctx_set_tail: events >= 1, 0 <= max_records <= 2^26, "
+                                  "0 <= live_blocks <= 64");
     c->tail_events = events;
     c->tail_max = max_records;
+    c->live_blocks = live_blocks;
     return PT_OK;
 }
 
@@ -841,6 +864,18 @@ TraceParams base_params(pt_ctx *c, const pt_geom *G, const pt_modes *M)
     return P;
 }
 
+// The context's live tail kernel for launch_trace (nullptr: none)
+const LiveTail *live_tail(pt_ctx *c, LiveTail *lt)
+{
+    if (c->live_blocks <= 0 || !c->side) return nullptr;
+    lt->side = c->side;
+    lt->fork = c->fork;
+    lt->join = c->join;
+    lt->blocks = c->live_blocks;
+    lt->ticks = c->live_ticks;
+    return lt;
+}
+
 // The tail buffer of a launch of `total` paths of (mode, precision) with ngroup adjoint
 // scores per path: room for total / 256 records (at least 65536, at most the context's
 // tail_max), grown as needed and kept.  No records (paths stay in the persistent kernel)
@@ -851,7 +886,23 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
     P.tail_score = nullptr;
     P.tail_cap = 0;
     P.tail_events = c->tail_events;
+    P.first_ids = nullptr;
+    P.first_cap = 0;
     if (!tail_mode(mode, precision) || c->count_ops || c->tail_max <= 0) return PT_OK;
+    if (mode == TRACE_ALIAS || mode == TRACE_ADJOINT || mode == TRACE_GRAINS) {
+        // room for total / 256 first paths (at least 65536, at most 2^24)
+        const int64_t fcap = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(65536, total / 256));
+        if (c->first_alloc < fcap) {
+            if (c->first_ids) PT_HIP(hipFree(c->first_ids));
+            c->first_ids = nullptr;
+            c->first_alloc = 0;
+            PT_HIP(hipMalloc((void **)&c->first_ids, (size_t)fcap * sizeof(int64_t)));
+            c->first_alloc = fcap;
+        }
+        P.first_ids = c->first_ids;
+        P.first_cap = fcap;
+        P.first_mfp = kFirstMfpFrac * std::min(P.g.box[0], std::min(P.g.box[1], P.g.box[2]));
+    }
     const int64_t cap = std::min<int64_t>(c->tail_max, std::max<int64_t>(65536, total / 256));
     if (c->tail_alloc < cap) {
         if (c->tail) PT_HIP(hipFree(c->tail));
@@ -859,6 +910,9 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
         c->tail_alloc = 0;
         PT_HIP(hipMalloc((void **)&c->tail, (size_t)cap * sizeof(TailRec)));
         c->tail_alloc = cap;
+        // epoch 0 marks a record no launch has written
+        PT_HIP(hipMemsetAsync(c->tail, 0, (size_t)cap * sizeof(TailRec), c->stream));
+        c->tail_epoch = 0;
     }
     if (ngroup > 0 && c->tail_score_alloc < cap * ngroup) {
         if (c->tail_score) PT_HIP(hipFree(c->tail_score));
@@ -872,6 +926,12 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
     P.tail_cap = cap;
     P.tail_cap_p = cap / 4;
     P.tail_prio_mfp = kTailPrioMfpFrac * std::min(P.g.box[0], std::min(P.g.box[1], P.g.box[2]));
+    if (++c->tail_epoch == 0) {
+        // 2^32 launches on one buffer: clear it before the numbers repeat
+        PT_HIP(hipMemsetAsync(c->tail, 0, (size_t)c->tail_alloc * sizeof(TailRec), c->stream));
+        c->tail_epoch = 1;
+    }
+    P.tail_epoch = c->tail_epoch;
     return PT_OK;
 }
 
@@ -1079,7 +1139,8 @@ static int adjoint_common(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_
         if ((e = hipMemsetAsync(d_sum, 0, nsum * sizeof(double), c->stream))) break;
         if (ntal && (e = hipMemsetAsync(d_gt, 0, ntal * sizeof(double), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, amode, blocks, c->threads, c->count_ops, lds, c->stream))) break;
+        LiveTail lt;
+        if ((e = launch_trace(P, amode, blocks, c->threads, c->count_ops, lds, c->stream, live_tail(c, &lt)))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipMemcpyAsync(h_sum.data(), d_sum, nsum * sizeof(double), hipMemcpyDeviceToHost, c->stream))) break;
@@ -1181,7 +1242,8 @@ int pt_trace_grid(pt_ctx *c, const pt_geom *G, const pt_modes *M, double mfp, in
         if ((e = hipMemsetAsync(c->counters, 0, kCounterWords * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_counts, 0, ncount * sizeof(long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, gmode, blocks, c->threads, c->count_ops, 0, c->stream))) break;
+        LiveTail lt;
+        if ((e = launch_trace(P, gmode, blocks, c->threads, c->count_ops, 0, c->stream, live_tail(c, &lt)))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = launch_reduce_grid(c->outcome, ncell, np, nb, nk, nangle, d_counts, c->counters + 7, c->stream))) break;
         if ((e = hipEventRecord(c->ev2, c->stream))) break;
@@ -1298,7 +1360,8 @@ int pt_trace_alias(pt_ctx *c, const pt_geom *G, const pt_modes *M, int64_t npath
         if ((e = hipMemsetAsync(c->counters, 0, kCounterWords * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipMemsetAsync(d_tal, 0, 2 * nm * sizeof(unsigned long long), c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, 0, c->stream))) break;
+        LiveTail lt;
+        if ((e = launch_trace(P, TRACE_ALIAS, blocks, c->threads, c->count_ops, 0, c->stream, live_tail(c, &lt)))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipMemcpyAsync(h_tal.data(), d_tal, 2 * nm * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
         if ((e = hipMemcpyAsync(host_ctr, c->counters, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream))) break;
@@ -1998,7 +2061,8 @@ int pt_batch_trace(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t precision,
                                 hipMemcpyHostToDevice, c->stream))) break;
         if ((e = launch_fill(B->scratch, B->scratch_words, c->stream))) break;
         if ((e = hipEventRecord(c->ev0, c->stream))) break;
-        if ((e = launch_trace(P, TRACE_BATCH, blocks, c->threads, c->count_ops, lds, c->stream))) break;
+        LiveTail lt;
+        if ((e = launch_trace(P, TRACE_BATCH, blocks, c->threads, c->count_ops, lds, c->stream, live_tail(c, &lt)))) break;
         if ((e = hipEventRecord(c->ev1, c->stream))) break;
         if ((e = hipEventSynchronize(c->ev1))) break;
         e = hipEventElapsedTime(&ms, c->ev0, c->ev1);

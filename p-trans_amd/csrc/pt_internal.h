@@ -203,7 +203,8 @@ struct alignas(16) TailRec {
     double fdx, fl, fw;         // grain tallies: the open grain's sums and 1 / mfp_m
     int64_t pid, events;
     float bc[3], br;
-    int32_t which, cur, flags, pad0;
+    int32_t which, cur, flags;
+    uint32_t epoch;             // the launch that wrote the record, stored last
     int32_t mode, sid, nscat, fgrain;     // sid: batch traces only
     uint32_t draw;
     int32_t pad[3];
@@ -211,16 +212,29 @@ struct alignas(16) TailRec {
 static_assert(sizeof(TailRec) == 192, "TailRec layout");
 // The counter block of a launch: queue at 0, statistics at 1-14 (pt_stats order, see
 // fill_stats), then the tail records claimed in the general and the priority part of the
-// buffer (either may count past its capacity: those paths stayed) and the tail kernel's
-// own queue.
+// buffer (either may count past its capacity: those paths stayed), the tail kernels' next
+// record of each part and the waves of the persistent kernel that have ended.
 constexpr int kCounterWords = 32;
 constexpr int kCtrTailN = 16;
 constexpr int kCtrTailP = 17;
 constexpr int kCtrTailNext = 18;
+constexpr int kCtrTailNextN = 19;
+constexpr int kCtrMainDone = 20;
+constexpr int kCtrFirstN = 21;       // paths the pre-pass found for the first pass
 // Paths whose mean free path is below this fraction of the smallest box side take records
-// in the priority part of the tail buffer, which the tail kernel serves first: their walks
-// are the longest (~(L / mfp)^2 events), so they start at once.
-constexpr double kTailPrioMfpFrac = 2.0e-3;
+// in the priority part of the tail buffer, which the tail kernels serve first: their walks
+// are the longest (~(L / mfp)^2 events), so they start at once.  The other paths are handed
+// over only after kTailLongFactor times as many events: most of their walks end within a
+// few thousand events, which the persistent kernel runs at a fraction of the tail kernel's
+// cost per event.
+constexpr double kTailPrioMfpFrac = 5.0e-3;
+constexpr int64_t kTailLongFactor = 4;
+// Alias-mode launches with a tail kernel start the paths whose mean free path is below this
+// fraction of the smallest box side first (a pre-pass lists them, the persistent kernel's
+// queue runs the list before the paths in id order and skips them there): the longest walks
+// of a launch then reach the live tail kernel at its start, not whenever the queue gets to
+// their ids.
+constexpr double kFirstMfpFrac = 2.0e-3;
 
 struct TraceParams {
     DevGeom g;
@@ -265,6 +279,18 @@ struct TraceParams {
     int64_t tail_cap_p;
     double tail_prio_mfp;      // paths below this mean free path take priority records
     int64_t tail_events;
+    uint32_t tail_epoch;       // this launch's number, written last into each record
+    // the tail kernel beside the persistent one (on another stream): it takes records while
+    // they are handed over, until all main_waves waves have ended or live_ticks of the wall
+    // clock (s_memrealtime) have passed
+    int32_t tail_live;
+    int64_t main_waves;
+    int64_t live_ticks;
+    // the first pass of the queue: ids of the paths below first_mfp (at most first_cap; a
+    // launch that finds more runs every path in id order)
+    int64_t *first_ids;
+    int64_t first_cap;
+    double first_mfp;
 };
 
 // Modes and precisions whose long walks go to the tail kernel (the op-counting kernels
@@ -306,9 +332,19 @@ inline size_t adj_lds_bytes(const TraceParams &p, int threads)
            (p.bt.top_k ? sizeof(TopHdr) + (size_t)p.bt.top_k * p.bt.top_cap * sizeof(TopSlot) : 0);
 }
 
+// The tail kernel beside a persistent launch: `blocks` workgroups on stream `side`, forked
+// from and joined back into the launch's stream by the two events.  The persistent kernel
+// then gets 2 x blocks workgroups fewer, room for them.
+struct LiveTail {
+    hipStream_t side;
+    hipEvent_t fork, join;
+    int blocks;
+    int64_t ticks;             // the live tail kernel's wall-clock limit (s_memrealtime)
+};
+
 // kernels' launch wrappers (pt_trace.hip)
 hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
-                        size_t lds, hipStream_t s);
+                        size_t lds, hipStream_t s, const LiveTail *live = nullptr);
 hipError_t launch_reduce_grid(const uint8_t *outcome, int64_t ncell, int64_t np,
                               int64_t nb, int64_t nk, int32_t nangle,
                               long long *counts, unsigned long long *skipped, hipStream_t s);

@@ -2187,6 +2187,27 @@ constexpr int kPathStay = 1 << 29;
 // Hands the lane's path to the tail kernel: its state, Rng position and adjoint or grain
 // sums go to the next tail record, and the lane's own sums start over for its next path.
 // False when the buffer is full (the path then stays).
+// Tail records are written and read with device-scope accesses (sc1: through to the level
+// every XCD sees), so that a tail kernel running beside the persistent one reads them
+// complete: the writer waits for its stores before it stores the record's epoch.
+template <class T>
+__device__ inline void put_dev(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline T get_dev(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// every store and load of this wave issued so far has completed
+__device__ inline void wait_mem()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <int kMode>
 __device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, AdjLane &al,
                         int mode)
@@ -2201,34 +2222,36 @@ __device__ bool handoff(const TraceParams &P, const Path &ph, const Rng &rng, Ad
     // 48 more live registers at this point of the loop)
     TailRec *r = P.tail + slot;
     for (int k = 0; k < 3; ++k) {
-        r->pos[k] = ph.pos[k];
-        r->dir[k] = ph.dir[k];
-        r->bc[k] = ph.bc[k];
+        put_dev(&r->pos[k], ph.pos[k]);
+        put_dev(&r->dir[k], ph.dir[k]);
+        put_dev(&r->bc[k], ph.bc[k]);
     }
-    r->fp = ph.fp;
-    r->mfp = ph.mfp;
-    r->omega = ph.omega;
-    r->spare = rng.spare;
-    r->len = al.len;
-    r->fdx = al.fdx;
-    r->fl = al.fl;
-    r->fw = al.fw;
-    r->pid = (int64_t)rng.path;
-    r->events = ph.events;
-    r->br = ph.br;
-    r->which = ph.which;
-    r->cur = ph.cur;
-    r->flags = ph.flags & ~kPathTop;
-    r->mode = mode;
-    if (kBatch) r->sid = ph.sid;
-    r->nscat = al.nscat;
-    r->fgrain = (kAdj & 1) ? al.fgrain : -1;
-    r->draw = rng.draw;
+    put_dev(&r->fp, ph.fp);
+    put_dev(&r->mfp, ph.mfp);
+    put_dev(&r->omega, ph.omega);
+    put_dev(&r->spare, rng.spare);
+    put_dev(&r->len, al.len);
+    put_dev(&r->fdx, al.fdx);
+    put_dev(&r->fl, al.fl);
+    put_dev(&r->fw, al.fw);
+    put_dev(&r->pid, (int64_t)rng.path);
+    put_dev(&r->events, ph.events);
+    put_dev(&r->br, ph.br);
+    put_dev(&r->which, ph.which);
+    put_dev(&r->cur, ph.cur);
+    put_dev(&r->flags, (int32_t)(ph.flags & ~kPathTop));
+    put_dev(&r->mode, (int32_t)mode);
+    put_dev(&r->sid, kBatch ? ph.sid : 0);
+    put_dev(&r->nscat, al.nscat);
+    put_dev(&r->fgrain, (kAdj & 1) ? al.fgrain : -1);
+    put_dev(&r->draw, rng.draw);
     if (kAdj & 2)
         for (int gi = 0; gi < P.ngroup; ++gi) {
-            P.tail_score[slot * P.ngroup + gi] = al.score[gi * al.stride];
+            put_dev(P.tail_score + slot * P.ngroup + gi, al.score[gi * al.stride]);
             al.score[gi * al.stride] = 0.0;
         }
+    wait_mem();
+    put_dev(&r->epoch, P.tail_epoch);
     if (kAdj & 1) {
         al.fgrain = -1;
         al.fdx = 0.0;
@@ -2304,6 +2327,14 @@ trace_kernel(TraceParams P)
     int mode = 0;
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
+    // the queue's first pass: nfirst listed paths, then every path in id order but those
+    int64_t nfirst = -1;
+    if (!kBatch && P.first_ids) {
+        const int64_t n = (int64_t)P.queue[kCtrFirstN];
+        nfirst = n <= P.first_cap ? n : -1;
+    }
+    nfirst = uniform64(nfirst);
+    const int64_t qtotal = P.total + (nfirst > 0 ? nfirst : 0);
 
     // long random walks go to the tail kernel (accelerated precisions; the op-counting
     // kernels keep them)
@@ -2339,8 +2370,8 @@ trace_kernel(TraceParams P)
                 if (lane == 0) base = atomicAdd(P.queue, (unsigned long long)P.chunk);
                 base = __shfl(base, 0, 64);
                 chunk_next = uniform64((int64_t)base);
-                chunk_end = chunk_next + P.chunk < P.total ? chunk_next + P.chunk : P.total;
-                if (chunk_next >= P.total) exhausted = true;
+                chunk_end = chunk_next + P.chunk < qtotal ? chunk_next + P.chunk : qtotal;
+                if (chunk_next >= qtotal) exhausted = true;
             }
             if (!exhausted) {
                 const int64_t avail = chunk_end - chunk_next;
@@ -2348,10 +2379,14 @@ trace_kernel(TraceParams P)
                 const int nneed = __popcll(need);
                 bool fresh = false;
                 if (!active && rank < avail) {
-                    const int64_t pid = chunk_next + rank;
+                    const int64_t v = chunk_next + rank;
+                    const int64_t pid = kBatch || nfirst <= 0 ? v : (v < nfirst ? P.first_ids[v] : v - nfirst);
                     al.nscat = 0;
                     al.len = 0.0;
-                    if (begin_path<kMode, kPrec>(P, pid, rng, ph, &mode)) {
+                    const bool started = begin_path<kMode, kPrec>(P, pid, rng, ph, &mode);
+                    if (!kBatch && nfirst >= 0 && v >= nfirst && ph.mfp < P.first_mfp) {
+                        // ran in the first pass
+                    } else if (started) {
                         active = true;
                         fresh = true;
                         if (kAdj & 1) al.fw = 1.0 / P.m.mfp[mode];
@@ -2373,11 +2408,13 @@ trace_kernel(TraceParams P)
         if (top_on && active && (ph.flags & kPathTop) && ph.events >= kTopUnbindEvents)
             top_unbind(stk, ph);
         if (kHand) {
-            // a path past tail_events events, or one of the last few of the wave once the queue
-            // is empty, goes to the tail kernel
+            // a path past tail_events events (while the queue lasts, kTailLongFactor times as
+            // many above the priority mean free path), or one of the last few of the wave once
+            // the queue is empty, goes to the tail kernel
             const bool few = exhausted && __popcll(__ballot(active)) <= kFewPaths;
-            if (active && ph.end_side == 0 && !(ph.flags & kPathStay) &&
-                (few || ph.events >= P.tail_events)) {
+            const int64_t hand_at = ph.mfp < P.tail_prio_mfp || exhausted ? P.tail_events
+                                                                          : kTailLongFactor * P.tail_events;
+            if (active && ph.end_side == 0 && !(ph.flags & kPathStay) && (few || ph.events >= hand_at)) {
                 if (handoff<kMode>(P, ph, rng, al, mode)) {
                     if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
                     active = false;
@@ -2396,6 +2433,8 @@ trace_kernel(TraceParams P)
             }
         }
     }
+    // this wave hands nothing over any more
+    if (kHand && lane == 0) atomicAdd(P.queue + kCtrMainDone, 1ull);
     if (kAdj) {
         __syncthreads();
         for (int i = threadIdx.x; i < P.nbatch * adj_k; i += blockDim.x)
@@ -2419,6 +2458,67 @@ trace_kernel(TraceParams P)
             atomicAdd(P.stats + 12, bn);
             atomicAdd(P.stats + 13, bs);
         }
+    }
+}
+
+// The pre-pass of the queue's first pass: the ids of the paths whose mean free path is
+// below first_mfp (the mode each samples, as begin_path does), in no particular order; at
+// most first_cap of them, counted in full.
+__global__ void __launch_bounds__(256) first_scan_kernel(TraceParams P)
+{
+    const int lane = threadIdx.x & 63;
+    Rng rng;
+    OpCount oc = {0ull, 0ull, 0ull, 0ull, nullptr};
+    rng.oc = &oc;
+    rng.seed = P.seed;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < P.total; b += stride) {
+        const int64_t pid = b + threadIdx.x;
+        bool hit = false;
+        if (pid < P.total) {
+            rng.reset((uint64_t)pid);
+            const int m = sample_mode(P.m, rng);
+            const int ib = m % P.m.nb;
+            const double mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[ib] : P.m.mfp[m];
+            hit = mfp < P.first_mfp;
+        }
+        const unsigned long long hm = __ballot(hit);
+        if (hm == 0ull) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(P.queue + kCtrFirstN, (unsigned long long)__popcll(hm));
+        base = __shfl(base, 0, 64);
+        const int64_t slot = (int64_t)base + __popcll(hm & ((1ull << lane) - 1ull));
+        if (hit && slot < P.first_cap) P.first_ids[slot] = pid;
+    }
+}
+
+// The live tail kernel's next record (lane 0): the oldest complete record of the priority
+// part, else of the general part, taken by a compare-and-swap on that part's next index, so
+// that it and the tail kernel after the persistent one (atomicAdd on the same index) never
+// take the same record.  Polls, sleeping, until the persistent kernel's waves have all
+// ended or the wall clock passes t_end; -1: nothing more for this wave (the tail kernel after
+// the persistent one takes what is left).
+__device__ int64_t live_claim(const TraceParams &P, uint64_t t_end)
+{
+    unsigned long long *q = P.queue;
+    while (true) {
+        const bool done = get_dev(q + kCtrMainDone) >= (unsigned long long)P.main_waves;
+        for (int part = 0; part < 2; ++part) {
+            const int64_t cap = part ? P.tail_cap - P.tail_cap_p : P.tail_cap_p;
+            const int64_t n = min((int64_t)get_dev(q + (part ? kCtrTailN : kCtrTailP)), cap);
+            unsigned long long *next = q + (part ? kCtrTailNextN : kCtrTailNext);
+            int64_t t = (int64_t)get_dev(next);
+            while (t < n) {
+                const int64_t slot = part ? P.tail_cap_p + t : t;
+                if (get_dev(&P.tail[slot].epoch) != P.tail_epoch) break;   // being written
+                const unsigned long long seen =
+                    atomicCAS(next, (unsigned long long)t, (unsigned long long)t + 1ull);
+                if ((int64_t)seen == t) return slot;
+                t = (int64_t)seen;
+            }
+        }
+        if (done || wall_clock64() > t_end) return -1;
+        __builtin_amdgcn_s_sleep(32);
     }
 }
 
@@ -2459,44 +2559,61 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
     float4 *rl = (float4 *)(cb + kChainXn + 3 * kChainPb);   // the staged region's planes
     int reg_grain = -1;
     const double lim = kBallMfpFrac * fmin(P.g.box[0], fmin(P.g.box[1], P.g.box[2]));
-    // the priority part of the buffer first, then the general part
+    // the priority part of the buffer first, then the general part; the live kernel (beside
+    // the persistent one) takes complete records as they come
     const int64_t np = (int64_t)min(P.queue[kCtrTailP], (unsigned long long)P.tail_cap_p);
-    const int64_t n = np + (int64_t)min(P.queue[kCtrTailN], (unsigned long long)(P.tail_cap - P.tail_cap_p));
+    const int64_t ng = (int64_t)min(P.queue[kCtrTailN], (unsigned long long)(P.tail_cap - P.tail_cap_p));
+    const uint64_t t_end = P.tail_live ? wall_clock64() + (uint64_t)P.live_ticks : 0ull;
     while (true) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(P.queue + kCtrTailNext, 1ull);
-        t = (unsigned long long)uniform64((int64_t)__shfl(t, 0, 64));
-        if ((int64_t)t >= n) break;
-        const int64_t slot = (int64_t)t < np ? (int64_t)t : P.tail_cap_p + ((int64_t)t - np);
-        const TailRec r = P.tail[slot];
+        int64_t slot = -1;
+        if (lane == 0) {
+            if (P.tail_live) {
+                __builtin_amdgcn_s_setprio(0);
+                slot = live_claim(P, t_end);
+                __builtin_amdgcn_s_setprio(3);
+            } else {
+                const int64_t t = (int64_t)atomicAdd(P.queue + kCtrTailNext, 1ull);
+                if (t < np) {
+                    slot = t;
+                } else {
+                    const int64_t u = (int64_t)atomicAdd(P.queue + kCtrTailNextN, 1ull);
+                    if (u < ng) slot = P.tail_cap_p + u;
+                }
+            }
+        }
+        slot = uniform64(__shfl(slot, 0, 64));
+        if (slot < 0) break;
+        const TailRec *r = P.tail + slot;
         Path ph;
         for (int k = 0; k < 3; ++k) {
-            ph.pos[k] = r.pos[k];
-            ph.dir[k] = r.dir[k];
-            ph.bc[k] = r.bc[k];
+            ph.pos[k] = get_dev(&r->pos[k]);
+            ph.dir[k] = get_dev(&r->dir[k]);
+            ph.bc[k] = get_dev(&r->bc[k]);
         }
-        ph.fp = r.fp;
-        ph.mfp = r.mfp;
-        ph.omega = r.omega;
-        ph.which = r.which;
-        ph.cur = r.cur;
+        ph.fp = get_dev(&r->fp);
+        ph.mfp = get_dev(&r->mfp);
+        ph.omega = get_dev(&r->omega);
+        ph.which = get_dev(&r->which);
+        ph.cur = get_dev(&r->cur);
         ph.end_side = 0;
-        ph.flags = r.flags;
+        ph.flags = get_dev(&r->flags);
         ph.start_grain = -1;
-        ph.events = r.events;
-        ph.br = r.br;
-        ph.sid = r.sid;
-        rng.path = (uint64_t)r.pid;
-        rng.draw = r.draw;
-        rng.spare = r.spare;
-        al.nscat = r.nscat;
-        al.len = r.len;
-        al.fgrain = r.fgrain;
-        al.fdx = r.fdx;
-        al.fl = r.fl;
-        al.fw = r.fw;
+        ph.events = get_dev(&r->events);
+        ph.br = get_dev(&r->br);
+        ph.sid = get_dev(&r->sid);
+        const int rmode = get_dev(&r->mode);
+        rng.path = (uint64_t)get_dev(&r->pid);
+        rng.draw = get_dev(&r->draw);
+        rng.spare = get_dev(&r->spare);
+        al.nscat = get_dev(&r->nscat);
+        al.len = get_dev(&r->len);
+        al.fgrain = get_dev(&r->fgrain);
+        al.fdx = get_dev(&r->fdx);
+        al.fl = get_dev(&r->fl);
+        al.fw = get_dev(&r->fw);
         if (kAdj & 2)
-            for (int gi = 0; gi < P.ngroup; ++gi) al.score[gi * al.stride] = P.tail_score[slot * P.ngroup + gi];
+            for (int gi = 0; gi < P.ngroup; ++gi)
+                al.score[gi * al.stride] = get_dev(P.tail_score + slot * P.ngroup + gi);
         while (true) {
             int pre_best = -2;
             double pre_dmin = 0.0;
@@ -2535,7 +2652,7 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
                                                &P.bt, stk, pre_best, pre_dmin))
                 break;
         }
-        if (lane == 0) end_path<kMode>(P, ph, rng, al, P.adj, adj_k, r.mode, st);
+        if (lane == 0) end_path<kMode>(P, ph, rng, al, P.adj, adj_k, rmode, st);
         wave_lds_sync();
     }
     if (lane == 0)
@@ -2652,9 +2769,40 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
 }
 
 hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
-                        size_t lds, hipStream_t s)
+                        size_t lds, hipStream_t s, const LiveTail *live)
 {
-    const dim3 gr(blocks), bl(threads);
+    const bool hand = tail_mode(mode, p.precision) && !count && p.tail_cap > 0;
+    if (hand && threads != 256) return hipErrorInvalidValue;   // four waves per workgroup
+    using TailFn = void (*)(TraceParams);
+    TailFn tail_fn = hand ? reinterpret_cast<TailFn>(const_cast<void *>(tail_kernel_fn(mode, p.precision)))
+                          : nullptr;
+    // the live tail kernel first, on its own stream once the launch's stream has reached
+    // this point (its counters are cleared); the persistent kernel leaves it the room of
+    // two of its own workgroups per workgroup (VGPRs: a tail wave needs up to 192, a trace
+    // wave 128).  Launched before the persistent kernel is resident, it gets that room;
+    // launched after, it waits for the persistent kernel's workgroups to end (either way
+    // it never holds up the persistent kernel, which waits for nothing).
+    const bool side = hand && live && live->blocks > 0 && blocks > 4 * live->blocks;
+    int main_blocks = blocks;
+    hipError_t e = hipSuccess;
+    if (side) {
+        main_blocks = blocks - 2 * live->blocks;
+        TraceParams q = p;
+        q.tail_live = 1;
+        q.main_waves = (int64_t)main_blocks * (threads / 64);
+        q.live_ticks = live->ticks;
+        if ((e = hipEventRecord(live->fork, s))) return e;
+        if ((e = hipStreamWaitEvent(live->side, live->fork, 0))) return e;
+        hipLaunchKernelGGL(tail_fn, dim3(live->blocks), dim3(threads), 0, live->side, q);
+        if ((e = hipGetLastError())) return e;
+    }
+    // the pre-pass of the queue's first pass (the live tail kernel takes its CUs meanwhile:
+    // once the persistent kernel is resident, no CU has room for a tail workgroup)
+    if (p.first_ids) {
+        hipLaunchKernelGGL(first_scan_kernel, dim3(blocks), dim3(256), 0, s, p);
+        if ((e = hipGetLastError())) return e;
+    }
+    const dim3 gr(main_blocks), bl(threads);
 #define PT_LAUNCH(M, C, R) hipLaunchKernelGGL((trace_kernel<M, C, R>), gr, bl, lds, s, p)
 #define PT_LAUNCH_C(M, R) do { if (count) PT_LAUNCH(M, true, R); else PT_LAUNCH(M, false, R); } while (0)
 #define PT_LAUNCH_R(M)                                                    \
@@ -2674,12 +2822,18 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
 #undef PT_LAUNCH_R
 #undef PT_LAUNCH_C
 #undef PT_LAUNCH
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess && tail_mode(mode, p.precision) && !count && p.tail_cap > 0) {
-        if (threads != 256) return hipErrorInvalidValue;       // four waves per workgroup
-        hipLaunchKernelGGL(reinterpret_cast<void (*)(TraceParams)>(const_cast<void *>(tail_kernel_fn(mode, p.precision))),
-                           gr, bl, 0, s, p);
+    e = hipGetLastError();
+    if (e == hipSuccess && hand) {
+        TraceParams q = p;
+        q.tail_live = 0;
+        hipLaunchKernelGGL(tail_fn, dim3(blocks), bl, 0, s, q);
         e = hipGetLastError();
+    }
+    if (side) {
+        // the launch's stream waits for the live tail kernel too
+        const hipError_t f = hipEventRecord(live->join, live->side);
+        const hipError_t w = f ? f : hipStreamWaitEvent(s, live->join, 0);
+        if (e == hipSuccess) e = w;
     }
     return e;
 }

@@ -76,7 +76,7 @@ def load():
     L.pt_ctx_set_event_limit.argtypes = [vp, i64]
     L.pt_ctx_set_op_counting.argtypes = [vp, i32]
     L.pt_ctx_set_tuning.argtypes = [vp, i32, i32, i32]
-    L.pt_ctx_set_tail.argtypes = [vp, i64, i64]
+    L.pt_ctx_set_tail.argtypes = [vp, i64, i64, i32]
     L.pt_geom_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
     L.pt_geom_destroy.argtypes = [vp]
     L.pt_surface_setup_polygon.argtypes = [i32, vp, vp, vp, vp]

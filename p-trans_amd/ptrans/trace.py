@@ -109,10 +109,11 @@ class Context:
     def set_tuning(self, chunk: int = 256, refill_min: int = 16, blocks_per_cu: int = 0):
         _lib.check(_lib.load().pt_ctx_set_tuning(self.handle, chunk, refill_min, blocks_per_cu))
 
-    def set_tail(self, events: int = 500, max_records: int = 1 << 20):
+    def set_tail(self, events: int = 500, max_records: int = 1 << 20, live_blocks: int = 16):
         """Paths past `events` events (and the last few of each wave) go to the tail kernel;
-        at most max_records per launch (0: none)."""
-        _lib.check(_lib.load().pt_ctx_set_tail(self.handle, events, max_records))
+        at most max_records per launch (0: none); live_blocks workgroups of it run beside
+        the persistent kernel (0: it runs after it only)."""
+        _lib.check(_lib.load().pt_ctx_set_tail(self.handle, events, max_records, live_blocks))
 
     def set_op_counting(self, on: bool):
         _lib.check(_lib.load().pt_ctx_set_op_counting(self.handle, int(bool(on))))

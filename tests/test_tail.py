@@ -9,7 +9,9 @@ everything such a path updates against the oracle: adjoint scores (scattering co
 path lengths), per-grain flux tallies (flights inside one grain) and the per-mode tallies.
 They run with the default hand-over (500 events), with every path handed over after its
 first event, with a tail buffer smaller than the paths that ask for it (the rest stay in the
-persistent kernel) and with no hand-over at all.
+persistent kernel) and with no hand-over at all; with the tail kernel's live workgroups
+beside the persistent kernel (16 by default, 64 at most: records taken while they are
+written, the two tail kernels sharing each part's next index) and without them.
 """
 import numpy as np
 import pytest
@@ -21,10 +23,12 @@ from test_adjoint import OracleSetup, _weights
 
 M_TWOPI = 3.14159265358979323846 * 2.0
 FILTER = 2
-# (events, max_records) of pt_ctx_set_tail: the default, every path after one event, a
-# buffer too small for the paths that ask (4096 records), no tail kernel
-TAILS = [(500, 1 << 20), (1, 1 << 20), (1, 4096), (500, 0)]
-TAIL_IDS = ["default", "all", "overflow", "none"]
+# (events, max_records, live_blocks) of pt_ctx_set_tail: the default, every path after one
+# event, a buffer too small for the paths that ask (4096 records), no tail kernel; the first
+# two with no live tail kernel, and every path with the most live workgroups
+TAILS = [(500, 1 << 20, 16), (1, 1 << 20, 16), (1, 4096, 16), (500, 0, 16), (500, 1 << 20, 0),
+         (1, 1 << 20, 0), (1, 1 << 20, 64)]
+TAIL_IDS = ["default", "all", "overflow", "none", "after", "all_after", "all_live64"]
 
 
 @pytest.fixture
@@ -88,7 +92,8 @@ def test_gpu_grain_tallies_diffusive_tail_equal_oracle(tail_ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tail_ctx", TAILS[:2], ids=TAIL_IDS[:2], indirect=True)
+@pytest.mark.parametrize("tail_ctx", TAILS[:2] + TAILS[4:], ids=TAIL_IDS[:2] + TAIL_IDS[4:],
+                         indirect=True)
 def test_gpu_many_candidates_tail_scan_equals_oracle(tail_ctx):
     """80 spheres in a 1000 nm cube: a path in the matrix has 86 candidates, so the wave's
     scan for a path left alone takes two passes of 64 lanes (the first strict minimum kept
