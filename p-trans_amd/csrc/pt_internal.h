@@ -221,6 +221,7 @@ constexpr int kCtrTailNext = 18;
 constexpr int kCtrTailNextN = 19;
 constexpr int kCtrMainDone = 20;
 constexpr int kCtrFirstN = 21;       // paths the pre-pass found for the first pass
+constexpr int kCtrMainStarted = 22;  // waves of the persistent kernel that have begun
 // Paths whose mean free path is below this fraction of the smallest box side take records
 // in the priority part of the tail buffer, which the tail kernels serve first: their walks
 // are the longest (~(L / mfp)^2 events), so they start at once.  The other paths are handed
@@ -282,7 +283,8 @@ struct TraceParams {
     uint32_t tail_epoch;       // this launch's number, written last into each record
     // the tail kernel beside the persistent one (on another stream): it takes records while
     // they are handed over, until all main_waves waves have ended or live_ticks of the wall
-    // clock (s_memrealtime) have passed
+    // clock (s_memrealtime) have passed, or after live_ticks / 20 if no wave of the
+    // persistent kernel has begun (launches serialised, as under a counter profiler)
     int32_t tail_live;
     int64_t main_waves;
     int64_t live_ticks;

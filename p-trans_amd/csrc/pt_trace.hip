@@ -2327,6 +2327,7 @@ trace_kernel(TraceParams P)
     int mode = 0;
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
+    if (tail_mode(kMode, kPrec) && !kCount && lane == 0) atomicAdd(P.queue + kCtrMainStarted, 1ull);
     // the queue's first pass: nfirst listed paths, then every path in id order but those
     int64_t nfirst = -1;
     if (!kBatch && P.first_ids) {
@@ -2496,9 +2497,9 @@ __global__ void __launch_bounds__(256) first_scan_kernel(TraceParams P)
 // part, else of the general part, taken by a compare-and-swap on that part's next index, so
 // that it and the tail kernel after the persistent one (atomicAdd on the same index) never
 // take the same record.  Polls, sleeping, until the persistent kernel's waves have all
-// ended or the wall clock passes t_end; -1: nothing more for this wave (the tail kernel after
-// the persistent one takes what is left).
-__device__ int64_t live_claim(const TraceParams &P, uint64_t t_end)
+// ended or the wall clock passes t_end (t_wait if none of them has begun); -1: nothing more
+// for this wave (the tail kernel after the persistent one takes what is left).
+__device__ int64_t live_claim(const TraceParams &P, uint64_t t_wait, uint64_t t_end)
 {
     unsigned long long *q = P.queue;
     while (true) {
@@ -2517,7 +2518,8 @@ __device__ int64_t live_claim(const TraceParams &P, uint64_t t_end)
                 t = (int64_t)seen;
             }
         }
-        if (done || wall_clock64() > t_end) return -1;
+        const uint64_t now = wall_clock64();
+        if (done || now > t_end || (now > t_wait && get_dev(q + kCtrMainStarted) == 0ull)) return -1;
         __builtin_amdgcn_s_sleep(32);
     }
 }
@@ -2563,13 +2565,14 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
     // the persistent one) takes complete records as they come
     const int64_t np = (int64_t)min(P.queue[kCtrTailP], (unsigned long long)P.tail_cap_p);
     const int64_t ng = (int64_t)min(P.queue[kCtrTailN], (unsigned long long)(P.tail_cap - P.tail_cap_p));
-    const uint64_t t_end = P.tail_live ? wall_clock64() + (uint64_t)P.live_ticks : 0ull;
+    const uint64_t t0 = P.tail_live ? wall_clock64() : 0ull;
+    const uint64_t t_wait = t0 + (uint64_t)P.live_ticks / 20u, t_end = t0 + (uint64_t)P.live_ticks;
     while (true) {
         int64_t slot = -1;
         if (lane == 0) {
             if (P.tail_live) {
                 __builtin_amdgcn_s_setprio(0);
-                slot = live_claim(P, t_end);
+                slot = live_claim(P, t_wait, t_end);
                 __builtin_amdgcn_s_setprio(3);
             } else {
                 const int64_t t = (int64_t)atomicAdd(P.queue + kCtrTailNext, 1ull);

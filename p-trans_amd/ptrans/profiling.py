@@ -9,7 +9,10 @@
 'kernels' prints per-kernel call count and average/total duration; 'counters' prints,
 per kernel and counter, the mean value per dispatch.  'traffic' prints the JSON summary
 the bench's roofline reads: per trace-kernel precision its average duration (kernel
-trace; a key naming "a+b" sums the kernels of one launch), HBM bytes per launch (FETCH_SIZE
+trace; a key naming "a+b" is a launch of several kernels, some of them concurrent: its
+duration is the span from the first start to the last end, launches delimited by the
+dispatches of kernel a, and its counters are summed over every dispatch of the launch),
+HBM bytes per launch (FETCH_SIZE
 and WRITE_SIZE in KiB, each scaled by what the
 same counter reads for a device copy of copy_mib MiB) and VALU wave-instructions per
 launch and per path with the issue fraction over 256 CUs x 2.4 GHz; with a TCC_HIT_sum /
@@ -30,6 +33,23 @@ def kernel_stats(db: str):
     tot = sum(r[2] for r in rows) or 1
     return [dict(name=r[0], calls=r[1], total_ns=r[2], average_ns=r[3], percent=100.0 * r[2] / tot)
             for r in rows]
+
+
+def launch_spans(db: str, subs, lead_ns: float = 2.0e6):
+    """Average span (ns) of the launches made of the kernels whose names contain one of
+    subs, and the number of launches.  Each dispatch of the first kernel marks a launch,
+    which holds the dispatches that start from lead_ns before it (a kernel enqueued just
+    ahead of it, on another stream) to lead_ns before the next launch's mark."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, start, end from kernels order by start").fetchall()
+    rows = [r for r in rows if any(sub in r[0] for sub in subs)]
+    marks = [r[1] for r in rows if subs[0] in r[0]]
+    spans = []
+    for i, m0 in enumerate(marks):
+        m1 = marks[i + 1] - lead_ns if i + 1 < len(marks) else float("inf")
+        sel = [r for r in rows if m0 - lead_ns <= r[1] < m1]
+        spans.append(max(r[2] for r in sel) - min(r[1] for r in sel))
+    return (sum(spans) / len(spans) if spans else 0.0), len(spans)
 
 
 def counter_means(db: str, kernel_filter: str = ""):
@@ -60,6 +80,19 @@ def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy
                 return name, mean
         raise KeyError("%s of %s not in %s" % (counter, sub, db))
 
+    def per_launch(db, counter, subs):
+        # the counter summed over every dispatch of the launch's kernels, per launch
+        tot, nl = 0.0, 0
+        for sub in subs:
+            for (name, cname), (mean, n) in counter_means(db, sub).items():
+                if cname == counter:
+                    tot += mean * n
+                    if sub == subs[0]:
+                        nl = n
+        if nl == 0:
+            raise KeyError("%s of %s not in %s" % (counter, subs[0], db))
+        return tot / nl
+
     _, cf = one(cfetch_db, "FETCH_SIZE", "copyBuffer")
     _, cw = one(cwrite_db, "WRITE_SIZE", "copyBuffer")
     fscale, wscale = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
@@ -67,19 +100,25 @@ def traffic_summary(kt_db, fetch_db, write_db, cfetch_db, cwrite_db, sq_db, copy
     out = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale,
            "peak_valu_wave_instr_per_s": PEAK_VALU_WAVE_INSTR_PER_S, "kernels": {}}
     for key, subs in (kernels or TRACE_KERNELS).items():
-        # a launch of several kernels (the trace kernel, then the tail kernel): "a+b" sums
-        # their durations, bytes and instructions
-        names, avg_us, f, w, v, hit, miss = [], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
-        for sub in subs.split("+"):
-            name = next(n for n in kt if sub in n)
-            names.append(name)
-            avg_us += kt[name]["average_ns"] / 1e3
-            f += one(fetch_db, "FETCH_SIZE", sub)[1]
-            w += one(write_db, "WRITE_SIZE", sub)[1]
-            v += one(sq_db, "SQ_INSTS_VALU", sub)[1]
+        sl = subs.split("+")
+        names = [next(n for n in kt if sub in n) for sub in sl]
+        hit = miss = 0.0
+        if len(sl) == 1:
+            avg_us = kt[names[0]]["average_ns"] / 1e3
+            f = one(fetch_db, "FETCH_SIZE", sl[0])[1]
+            w = one(write_db, "WRITE_SIZE", sl[0])[1]
+            v = one(sq_db, "SQ_INSTS_VALU", sl[0])[1]
             if tcc_db:
-                hit += one(tcc_db, "TCC_HIT_sum", sub)[1]
-                miss += one(tcc_db, "TCC_MISS_sum", sub)[1]
+                hit = one(tcc_db, "TCC_HIT_sum", sl[0])[1]
+                miss = one(tcc_db, "TCC_MISS_sum", sl[0])[1]
+        else:
+            avg_us = launch_spans(kt_db, sl)[0] / 1e3
+            f = per_launch(fetch_db, "FETCH_SIZE", sl)
+            w = per_launch(write_db, "WRITE_SIZE", sl)
+            v = per_launch(sq_db, "SQ_INSTS_VALU", sl)
+            if tcc_db:
+                hit = per_launch(tcc_db, "TCC_HIT_sum", sl)
+                miss = per_launch(tcc_db, "TCC_MISS_sum", sl)
         fb, wb = f * 1024.0 * fscale, w * 1024.0 * wscale
         out["kernels"][key] = {
             "kernel": " + ".join(names), "average_us": avg_us,
