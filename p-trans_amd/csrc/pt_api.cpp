@@ -890,8 +890,9 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
     P.first_cap = 0;
     if (!tail_mode(mode, precision) || c->count_ops || c->tail_max <= 0) return PT_OK;
     if (mode == TRACE_ALIAS || mode == TRACE_ADJOINT || mode == TRACE_GRAINS) {
-        // room for total / 256 first paths (at least 65536, at most 2^24)
-        const int64_t fcap = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(65536, total / 256));
+        // the list covers the last quarter of the ids: room for total / 1024 paths (at least
+        // 65536, at most 2^24)
+        const int64_t fcap = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(65536, total / 1024));
         if (c->first_alloc < fcap) {
             if (c->first_ids) PT_HIP(hipFree(c->first_ids));
             c->first_ids = nullptr;
@@ -901,6 +902,7 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
         }
         P.first_ids = c->first_ids;
         P.first_cap = fcap;
+        P.first_from = total - total / 4;
         P.first_mfp = kFirstMfpFrac * std::min(P.g.box[0], std::min(P.g.box[1], P.g.box[2]));
     }
     const int64_t cap = std::min<int64_t>(c->tail_max, std::max<int64_t>(65536, total / 256));

@@ -230,11 +230,12 @@ constexpr int kCtrMainStarted = 22;  // waves of the persistent kernel that have
 // cost per event.
 constexpr double kTailPrioMfpFrac = 5.0e-3;
 constexpr int64_t kTailLongFactor = 4;
-// Alias-mode launches with a tail kernel start the paths whose mean free path is below this
-// fraction of the smallest box side first (a pre-pass lists them, the persistent kernel's
-// queue runs the list before the paths in id order and skips them there): the longest walks
-// of a launch then reach the live tail kernel at its start, not whenever the queue gets to
-// their ids.
+// Alias-mode launches with a tail kernel start the paths of the last quarter of the ids
+// whose mean free path is below this fraction of the smallest box side before the rest of
+// that quarter (a pre-pass lists them; the persistent kernel's queue runs the ids of the
+// first three quarters, the list, then the last quarter without the listed paths): the
+// longest walks then reach the live tail kernel while the persistent kernel has at least a
+// quarter of its work left, not in its last moments.
 constexpr double kFirstMfpFrac = 2.0e-3;
 
 struct TraceParams {
@@ -288,10 +289,11 @@ struct TraceParams {
     int32_t tail_live;
     int64_t main_waves;
     int64_t live_ticks;
-    // the first pass of the queue: ids of the paths below first_mfp (at most first_cap; a
-    // launch that finds more runs every path in id order)
+    // the queue's list of the paths from first_from on below first_mfp (at most first_cap;
+    // a launch that finds more runs every path in id order)
     int64_t *first_ids;
     int64_t first_cap;
+    int64_t first_from;
     double first_mfp;
 };
 

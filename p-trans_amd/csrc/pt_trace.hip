@@ -2328,7 +2328,8 @@ trace_kernel(TraceParams P)
     int64_t chunk_next = 0, chunk_end = 0;       // wave-uniform slice of the queue
     bool exhausted = false;
     if (tail_mode(kMode, kPrec) && !kCount && lane == 0) atomicAdd(P.queue + kCtrMainStarted, 1ull);
-    // the queue's first pass: nfirst listed paths, then every path in id order but those
+    // the queue: ids below first_from, the nfirst listed paths, then the ids from first_from
+    // on but those
     int64_t nfirst = -1;
     if (!kBatch && P.first_ids) {
         const int64_t n = (int64_t)P.queue[kCtrFirstN];
@@ -2381,12 +2382,15 @@ trace_kernel(TraceParams P)
                 bool fresh = false;
                 if (!active && rank < avail) {
                     const int64_t v = chunk_next + rank;
-                    const int64_t pid = kBatch || nfirst <= 0 ? v : (v < nfirst ? P.first_ids[v] : v - nfirst);
+                    const int64_t a = P.first_from;
+                    const int64_t pid = kBatch || nfirst <= 0 || v < a
+                                            ? v
+                                            : (v < a + nfirst ? P.first_ids[v - a] : v - nfirst);
                     al.nscat = 0;
                     al.len = 0.0;
                     const bool started = begin_path<kMode, kPrec>(P, pid, rng, ph, &mode);
-                    if (!kBatch && nfirst >= 0 && v >= nfirst && ph.mfp < P.first_mfp) {
-                        // ran in the first pass
+                    if (!kBatch && nfirst >= 0 && v >= a + nfirst && ph.mfp < P.first_mfp) {
+                        // a listed path: it ran from the list
                     } else if (started) {
                         active = true;
                         fresh = true;
@@ -2462,34 +2466,46 @@ trace_kernel(TraceParams P)
     }
 }
 
-// The pre-pass of the queue's first pass: the ids of the paths whose mean free path is
-// below first_mfp (the mode each samples, as begin_path does), in no particular order; at
-// most first_cap of them, counted in full.
+// The pre-pass of the queue's list: the ids from first_from on of the paths whose mean free
+// path is below first_mfp (the mode each samples, as begin_path does), in no particular
+// order; at most first_cap of them, counted in full.
 __global__ void __launch_bounds__(256) first_scan_kernel(TraceParams P)
 {
+    // four paths per lane at once (their draws, then their table loads), so the wave waits
+    // for one round of loads per four paths
+    constexpr int kUnroll = 4;
     const int lane = threadIdx.x & 63;
     Rng rng;
     OpCount oc = {0ull, 0ull, 0ull, 0ull, nullptr};
     rng.oc = &oc;
     rng.seed = P.seed;
+    const int n = P.m.nb * P.m.nk;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < P.total; b += stride) {
-        const int64_t pid = b + threadIdx.x;
-        bool hit = false;
-        if (pid < P.total) {
-            rng.reset((uint64_t)pid);
-            const int m = sample_mode(P.m, rng);
-            const int ib = m % P.m.nb;
-            const double mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[ib] : P.m.mfp[m];
-            hit = mfp < P.first_mfp;
+    for (int64_t b = P.first_from + (int64_t)blockIdx.x * blockDim.x; b < P.total; b += kUnroll * stride) {
+        int col[kUnroll];
+        double u1[kUnroll];
+        for (int k = 0; k < kUnroll; ++k) {
+            const int64_t pid = b + k * stride + threadIdx.x;
+            // sample_mode's two draws and column
+            rng.reset((uint64_t)(pid < P.total ? pid : 0));
+            const double u0 = rng.next();
+            u1[k] = rng.next();
+            const int i = (int)(u0 * (double)n);
+            col[k] = i < n ? i : n - 1;
         }
-        const unsigned long long hm = __ballot(hit);
-        if (hm == 0ull) continue;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(P.queue + kCtrFirstN, (unsigned long long)__popcll(hm));
-        base = __shfl(base, 0, 64);
-        const int64_t slot = (int64_t)base + __popcll(hm & ((1ull << lane) - 1ull));
-        if (hit && slot < P.first_cap) P.first_ids[slot] = pid;
+        for (int k = 0; k < kUnroll; ++k) {
+            const int64_t pid = b + k * stride + threadIdx.x;
+            const int m = u1[k] < P.m.prob[col[k]] ? col[k] : P.m.alias[col[k]];
+            const double mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[m % P.m.nb] : P.m.mfp[m];
+            const bool hit = pid < P.total && mfp < P.first_mfp;
+            const unsigned long long hm = __ballot(hit);
+            if (hm == 0ull) continue;
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(P.queue + kCtrFirstN, (unsigned long long)__popcll(hm));
+            base = __shfl(base, 0, 64);
+            const int64_t slot = (int64_t)base + __popcll(hm & ((1ull << lane) - 1ull));
+            if (hit && slot < P.first_cap) P.first_ids[slot] = pid;
+        }
     }
 }
 
