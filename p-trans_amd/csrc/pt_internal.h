@@ -125,6 +125,10 @@ struct BvhNode {
 };
 static_assert(sizeof(BvhNode) == 128, "BvhNode layout");
 constexpr int kBvhStack = 16;          // traversal stack per lane (LDS); tree depth < 16
+// Radius of a tail walk's local candidate list (batch): this many mean free paths, at least
+// kLocalMin nm (a quarter, then a sixteenth of it when more than 64 surfaces lie within)
+constexpr float kLocalMfps = 12.0f;
+constexpr float kLocalMin = 6.0f;
 
 // Top levels of the grain BVHs of one structure staged in a workgroup's LDS.  A node
 // reference r >= kLdsRef names LDS slot r - kLdsRef (grain i of the structure, heap

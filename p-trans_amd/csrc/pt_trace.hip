@@ -1023,9 +1023,13 @@ __device__ inline void top_unbind(int *stk, Path &ph)
 // against host-padded bounds (they only prune); surfaces are tested by the reference's
 // FP64 routines, and the winner is the smallest (distance, surface index): the first
 // strictly nearest surface in the reference's scan order (box grain first, then the grain
-// in order).  With no hit the previous surface is kept.
+// in order).  With no hit the previous surface is kept.  Nodes entered beyond `reach` are
+// pruned like nodes beyond the nearest hit so far: the accelerated precisions pass the free
+// path, since the event is an internal scattering whenever the nearest hit lies beyond it,
+// whichever surface that is (the caller then discards the scan's surface); the FP64 kernel
+// passes 1e300 and scans the whole ray, as the reference does.
 __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, double *dmin,
-                         int *best, OpCount &oc, int *stk)
+                         int *best, OpCount &oc, int *stk, double reach)
 {
     double bl = 1.0e10;
     int bi = ph.cur;
@@ -1053,8 +1057,9 @@ __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, d
             bool g0 = slab_enter(rf, nd.lo[0], nd.hi[0], &t0) && may_face(rf, nd.nlo[0], nd.nhi[0]);
             bool g1 = slab_enter(rf, nd.lo[1], nd.hi[1], &t1) && may_face(rf, nd.nlo[1], nd.nhi[1]);
             // margin 1e-5 relative on the entry parameter (the padded bounds cover the rest)
-            g0 = g0 && !(hit && (double)t0 * (1.0 - 1.0e-5) > bl);
-            g1 = g1 && !(hit && (double)t1 * (1.0 - 1.0e-5) > bl);
+            const double lim = hit && bl < reach ? bl : reach;
+            g0 = g0 && !((double)t0 * (1.0 - 1.0e-5) > lim);
+            g1 = g1 && !((double)t1 * (1.0 - 1.0e-5) > lim);
             // nearer child first: its hits can prune the farther one
             const bool sw = g1 && (!g0 || t1 < t0);
             const int nc = sw ? nd.child[1] : nd.child[0], fc = sw ? nd.child[0] : nd.child[1];
@@ -1063,7 +1068,7 @@ __device__ void scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, d
             bool fg = sw ? g0 : g1;
             const float ft = sw ? t0 : t1;
             if (ng && nc < 0) bvh_leaf_hits(g, bt, ph, -nc - 1, nn, &bl, &bi, &hit, oc);
-            fg = fg && !(hit && (double)ft * (1.0 - 1.0e-5) > bl);
+            fg = fg && !((double)ft * (1.0 - 1.0e-5) > (hit && bl < reach ? bl : reach));
             if (fg && fc < 0) bvh_leaf_hits(g, bt, ph, -fc - 1, fn, &bl, &bi, &hit, oc);
             if (fg && fc >= 0 && sp < kBvhStack) stk[sp++ * 256] = fc;
             if (ng && nc >= 0 && sp < kBvhStack) stk[sp++ * 256] = nc;
@@ -1660,8 +1665,8 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         if (pre_best >= 0) best = pre_best;
         decided = true;
     }
-    if (kBatch && !in_safe) {
-        scan_bvh(g, *bt, ph, &dmin, &best, oc, stk);
+    if (kBatch && !in_safe && pre_best == -2) {
+        scan_bvh(g, *bt, ph, &dmin, &best, oc, stk, kPrec != kPrecFP64 ? ph.fp : 1.0e300);
         decided = true;
     }
     if (!kBatch && kPrec != kPrecFP64 && !decided) {
@@ -1822,8 +1827,9 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
     if (kPrec != kPrecFP64) {
         const double lim = kBallMfpFrac * fmin(g.box[0], fmin(g.box[1], g.box[2]));
         if (scattered && !in_safe && ph.mfp < lim) {
-            if (kBatch) ball_setup_bvh(g, *bt, ph, stk, oc);
-            else if (pre_best != -2) ph.br = __builtin_nanf("");
+            // a caller that gave the scan (the tail kernel) sets the ball up itself
+            if (pre_best != -2) ph.br = __builtin_nanf("");
+            else if (kBatch) ball_setup_bvh(g, *bt, ph, stk, oc);
             else ball_setup(g, ph);
         }
     }
@@ -1865,6 +1871,286 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The wave's traversal stack in the tail kernel: its lanes' BVH stacks (entries 256 apart,
+// lane l of the wave at wb + l), as one array of 64 x kBvhStack entries.
+constexpr int kWaveStack = 64 * kBvhStack;
+__device__ inline int &wave_stack_at(int *wb, int j)
+{
+    return wb[(j & 63) + (j >> 6) * 256];
+}
+
+// Pushes the references of the lanes whose flag is set on the wave's stack (sp: its depth,
+// wave-uniform).  False, and nothing pushed, when they would not fit.
+__device__ inline bool wave_push2(int *wb, int &sp, bool p0, int r0, bool p1, int r1)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+    const int n0 = __popcll(m0), n = n0 + __popcll(m1);
+    if (sp + n > kWaveStack) return false;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (p0) wave_stack_at(wb, sp + __popcll(m0 & below)) = r0;
+    if (p1) wave_stack_at(wb, sp + n0 + __popcll(m1 & below)) = r1;
+    sp += n;
+    return true;
+}
+
+// scan_bvh for a path the whole wave holds (the tail kernel): each round the lanes take up
+// to 64 nodes off the wave's stack, test both children of their node (the leaves' surfaces
+// by the reference's FP64 routines) and push the children to visit; the wave then keeps the
+// smallest (distance, surface index) of all hits, which is what scan_bvh's nearer-child-
+// first traversal ends with (its pruning and the lanes' only drop nodes that cannot hold a
+// nearer hit, or an equally near one of a smaller index; nodes entered beyond `reach` as in
+// scan_bvh).  The rounds follow the tree's depth, not the number of nodes visited.  False
+// when the stack would overflow (the caller then runs scan_bvh).  wb: the wave's stack.
+__device__ bool wave_scan_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, double *dmin,
+                              int *best, OpCount &oc, int *stk, int *wb, double reach)
+{
+    const int lane = threadIdx.x & 63;
+    double bl = 1.0e10;
+    int bi = ph.cur;
+    bool hit = false;
+    for (int a = 0; a < 3; ++a) {
+        const double d = a == 0 ? ph.dir[0] : (a == 1 ? ph.dir[1] : ph.dir[2]);
+        if (d == 0.0) continue;
+        const int w = 2 * a + (d < 0.0 ? 0 : 1);
+        double l;
+        if (surface_distance(g, g.surf[w], ph.pos, ph.dir, &l, oc) && (l < bl || (hit && l == bl && w < bi))) {
+            bl = l;
+            bi = w;
+            hit = true;
+        }
+    }
+    const int root = bvh_root_ref(bt, ph, stk);
+    if (root >= 0) {
+        const RayF32 rf = ray_f32(ph.pos, ph.dir);
+        if (lane == 0) wave_stack_at(wb, 0) = root;
+        int sp = 1;
+        wave_lds_sync();
+        while (sp > 0) {
+            const int take = sp < 64 ? sp : 64;
+            sp -= take;
+            const int r = lane < take ? wave_stack_at(wb, sp + lane) : 0;
+            wave_lds_sync();
+            double lb = bl;
+            int li = bi;
+            bool lh = hit;
+            bool p0 = false, p1 = false;
+            int c0 = 0, c1 = 0;
+            if (lane < take) {
+                const BvhNode nd = bvh_node(bt, r, stk);
+                float t0, t1;
+                bool g0 = slab_enter(rf, nd.lo[0], nd.hi[0], &t0) && may_face(rf, nd.nlo[0], nd.nhi[0]);
+                bool g1 = slab_enter(rf, nd.lo[1], nd.hi[1], &t1) && may_face(rf, nd.nlo[1], nd.nhi[1]);
+                const double lim = hit && bl < reach ? bl : reach;
+                g0 = g0 && !((double)t0 * (1.0 - 1.0e-5) > lim);
+                g1 = g1 && !((double)t1 * (1.0 - 1.0e-5) > lim);
+                c0 = nd.child[0];
+                c1 = nd.child[1];
+                if (g0 && c0 < 0) bvh_leaf_hits(g, bt, ph, -c0 - 1, nd.count[0], &lb, &li, &lh, oc);
+                if (g1 && c1 < 0) bvh_leaf_hits(g, bt, ph, -c1 - 1, nd.count[1], &lb, &li, &lh, oc);
+                p0 = g0 && c0 >= 0;
+                p1 = g1 && c1 >= 0;
+            }
+            // the smallest (distance, index) over the lanes' hits: every lane ends with it
+            for (int m = 1; m < 64; m <<= 1) {
+                const double ol = __shfl_xor(lb, m, 64);
+                const int oi = __shfl_xor(li, m, 64);
+                const bool oh = __shfl_xor((int)lh, m, 64) != 0;
+                if (oh && (!lh || ol < lb || (ol == lb && oi < li))) {
+                    lb = ol;
+                    li = oi;
+                    lh = true;
+                }
+            }
+            bl = lb;
+            bi = li;
+            hit = lh;
+            if (!wave_push2(wb, sp, p0, c0, p1, c1)) return false;
+            wave_lds_sync();
+        }
+    }
+    *dmin = bl;
+    *best = bi;
+    return true;
+}
+
+// ball_setup_bvh for a path the whole wave holds, in rounds of up to 64 nodes as
+// wave_scan_bvh: the radius is the minimum over the walls and the leaves of the leaf's
+// bound (the larger of its box distance and its surfaces' bound), which does not depend on
+// the order the leaves are met in (a leaf or subtree left out by either traversal has a box
+// distance at least the radius).  False when the stack would overflow (the caller then runs
+// ball_setup_bvh).
+__device__ bool wave_ball_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, int *stk, int *wb)
+{
+    const int lane = threadIdx.x & 63;
+    const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
+    float r = fminf(fminf(fminf(px, (float)g.box[0] - px), fminf(py, (float)g.box[1] - py)),
+                    fminf(pz, (float)g.box[2] - pz));
+    r = fmaxf(r, 0.0f);
+    const int root = bvh_root_ref(bt, ph, stk);
+    if (root >= 0) {
+        if (lane == 0) wave_stack_at(wb, 0) = root;
+        int sp = 1;
+        wave_lds_sync();
+        while (sp > 0) {
+            const int take = sp < 64 ? sp : 64;
+            sp -= take;
+            const int ref = lane < take ? wave_stack_at(wb, sp + lane) : 0;
+            wave_lds_sync();
+            float lr = r;
+            bool p0 = false, p1 = false;
+            int c0 = 0, c1 = 0;
+            if (lane < take) {
+                const BvhNode nd = bvh_node(bt, ref, stk);
+                const float d0 = box_dist2(nd.lo[0], nd.hi[0], px, py, pz);
+                const float d1 = box_dist2(nd.lo[1], nd.hi[1], px, py, pz);
+                c0 = nd.child[0];
+                c1 = nd.child[1];
+                if (d0 < r * r && c0 < 0)
+                    lr = fminf(lr, fmaxf(sqrtf(d0), bvh_leaf_bound(g, bt, -c0 - 1, nd.count[0], px, py, pz)));
+                if (d1 < r * r && c1 < 0)
+                    lr = fminf(lr, fmaxf(sqrtf(d1), bvh_leaf_bound(g, bt, -c1 - 1, nd.count[1], px, py, pz)));
+                p0 = d0 < r * r && c0 >= 0;
+                p1 = d1 < r * r && c1 >= 0;
+            }
+            for (int m = 1; m < 64; m <<= 1) lr = fminf(lr, __shfl_xor(lr, m, 64));
+            r = lr;
+            if (!wave_push2(wb, sp, p0, c0, p1, c1)) return false;
+            wave_lds_sync();
+        }
+    }
+    ph.bc[0] = px;
+    ph.bc[1] = py;
+    ph.bc[2] = pz;
+    ph.br = ball_radius_of(r);
+    return true;
+}
+
+// A local candidate list of a path the whole wave holds (batch, tail kernel): the surfaces
+// of the path's grain whose bound (surf_bound) from a centre c is at most R, gathered by one
+// traversal over the leaves whose padded box lies within R of c.  Every surface of the grain
+// left out lies at least R from c.  Scans and safety balls near c then test the list in one
+// round instead of walking the tree again.  Returns the count, -1 when more than 64.
+__device__ int wave_local_bvh(const DevGeom &g, const DevBatch &bt, const Path &ph, int *stk,
+                              int *wb, int *lst, float cx, float cy, float cz, float R)
+{
+    const int lane = threadIdx.x & 63;
+    const int root = bvh_root_ref(bt, ph, stk);
+    if (root < 0) return 0;
+    int n = 0;
+    if (lane == 0) wave_stack_at(wb, 0) = root;
+    int sp = 1;
+    wave_lds_sync();
+    while (sp > 0) {
+        const int take = sp < 64 ? sp : 64;
+        sp -= take;
+        const int ref = lane < take ? wave_stack_at(wb, sp + lane) : 0;
+        wave_lds_sync();
+        bool p0 = false, p1 = false;
+        int c0 = 0, c1 = 0, f0 = 0, n0 = 0, f1 = 0, n1 = 0;
+        if (lane < take) {
+            const BvhNode nd = bvh_node(bt, ref, stk);
+            const bool w0 = box_dist2(nd.lo[0], nd.hi[0], cx, cy, cz) < R * R;
+            const bool w1 = box_dist2(nd.lo[1], nd.hi[1], cx, cy, cz) < R * R;
+            c0 = nd.child[0];
+            c1 = nd.child[1];
+            if (w0 && c0 < 0) { f0 = -c0 - 1; n0 = nd.count[0]; }
+            if (w1 && c1 < 0) { f1 = -c1 - 1; n1 = nd.count[1]; }
+            p0 = w0 && c0 >= 0;
+            p1 = w1 && c1 >= 0;
+        }
+        // the lanes' leaf surfaces within R, one per lane and step
+        int mx = n0 + n1;
+        for (int m = 1; m < 64; m <<= 1) mx = max(mx, __shfl_xor(mx, m, 64));
+        for (int j = 0; j < mx; ++j) {
+            int idx = -1;
+            if (j < n0 + n1) {
+                idx = bt.prims[j < n0 ? f0 + j : f1 + (j - n0)];
+                if (!(surf_bound(g.surf[idx], g.vtx, cx, cy, cz) <= R)) idx = -1;
+            }
+            const unsigned long long am = __ballot(idx >= 0);
+            if (n + __popcll(am) > 64) return -1;
+            if (idx >= 0) lst[n + __popcll(am & ((1ull << lane) - 1ull))] = idx;
+            n += __popcll(am);
+        }
+        if (!wave_push2(wb, sp, p0, c0, p1, c1)) return -1;
+        wave_lds_sync();
+    }
+    return n;
+}
+
+// Lower bound on the distance from p to the box walls and the surfaces of the grain, from a
+// local list of centre c and radius R (d: |p - c|, below R): the listed surfaces' bounds,
+// and R - d for the others.
+__device__ inline float list_ball_bound(const DevGeom &g, const int *lst, int n, float px, float py,
+                                        float pz, float R, float d)
+{
+    const int lane = threadIdx.x & 63;
+    float r = fminf(fminf(fminf(px, (float)g.box[0] - px), fminf(py, (float)g.box[1] - py)),
+                    fminf(pz, (float)g.box[2] - pz));
+    r = fmaxf(fminf(r, R - d), 0.0f);
+    float b = lane < n ? surf_bound(g.surf[lst[lane]], g.vtx, px, py, pz) : 3.0e38f;
+    for (int m = 1; m < 64; m <<= 1) b = fminf(b, __shfl_xor(b, m, 64));
+    return fminf(r, b);
+}
+
+// scan_bvh over a local list whose region holds every point the flight can reach: the box
+// walls, then the listed surfaces (one per lane, the reference's FP64 tests), keeping the
+// smallest (distance, surface index).  Every surface that could be hit within the free
+// path is listed, so the event is the one scan_bvh would give.
+__device__ void list_scan(const DevGeom &g, const int *lst, int n, const Path &ph, double *dmin,
+                          int *best, OpCount &oc)
+{
+    const int lane = threadIdx.x & 63;
+    double bl = 1.0e10;
+    int bi = ph.cur;
+    bool hit = false;
+    for (int a = 0; a < 3; ++a) {
+        const double d = a == 0 ? ph.dir[0] : (a == 1 ? ph.dir[1] : ph.dir[2]);
+        if (d == 0.0) continue;
+        const int w = 2 * a + (d < 0.0 ? 0 : 1);
+        double l;
+        if (surface_distance(g, g.surf[w], ph.pos, ph.dir, &l, oc) && (l < bl || (hit && l == bl && w < bi))) {
+            bl = l;
+            bi = w;
+            hit = true;
+        }
+    }
+    double lb = bl;
+    int li = bi;
+    bool lh = hit;
+    if (lane < n) {
+        const int idx = lst[lane];
+        const DSurf &s = g.surf[idx];
+        double l;
+        bool h;
+        if (s.kind == PT_SHAPE_POLYGON) {
+            h = polygon_plane(g.vtx, s, ph.pos, ph.dir, &l, oc) &&
+                (l < lb || (lh && l == lb && idx < li)) &&
+                polygon_contains(g.vtx, s, ph.pos, ph.dir, l, oc);
+        } else {
+            h = surface_distance(g, s, ph.pos, ph.dir, &l, oc) && (l < lb || (lh && l == lb && idx < li));
+        }
+        if (h) {
+            lb = l;
+            li = idx;
+            lh = true;
+        }
+    }
+    for (int m = 1; m < 64; m <<= 1) {
+        const double ol = __shfl_xor(lb, m, 64);
+        const int oi = __shfl_xor(li, m, 64);
+        const bool oh = __shfl_xor((int)lh, m, 64) != 0;
+        if (oh && (!lh || ol < lb || (ol == lb && oi < li))) {
+            lb = ol;
+            li = oi;
+            lh = true;
+        }
+    }
+    *dmin = lb;
+    *best = li;
 }
 
 __device__ inline double pick3(const double *v, int k)
@@ -2573,8 +2859,10 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
     al.gacc = ((kAdj & 1) && P.grain_tally && lane == 0) ? P.grain_tally : nullptr;
     const int adj_k = kAdj ? adj_stride(P) : 0;
     int *stk = tstk + (kBatch ? threadIdx.x : 0);
+    int *wb = tstk + (kBatch ? (threadIdx.x & ~63) : 0);    // the wave's stack (batch)
     double *cb = chain[wave];
     float4 *rl = (float4 *)(cb + kChainXn + 3 * kChainPb);   // the staged region's planes
+    int *lst = (int *)rl;                 // batch (no regions): the wave's local list, 64 entries
     int reg_grain = -1;
     const double lim = kBallMfpFrac * fmin(P.g.box[0], fmin(P.g.box[1], P.g.box[2]));
     // the priority part of the buffer first, then the general part; the live kernel (beside
@@ -2633,6 +2921,9 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
         if (kAdj & 2)
             for (int gi = 0; gi < P.ngroup; ++gi)
                 al.score[gi * al.stride] = get_dev(P.tail_score + slot * P.ngroup + gi);
+        // batch: the wave's local candidate list (none yet)
+        int lc_grain = -1, lc_n = 0;
+        float lc_x = 0.0f, lc_y = 0.0f, lc_z = 0.0f, lc_r = 0.0f;
         while (true) {
             int pre_best = -2;
             double pre_dmin = 0.0;
@@ -2657,12 +2948,68 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
                                                          P.g.region_margin);
                 }
             } else {
-                if (!kBatch && __builtin_isnan(ph.br)) {
+                if (__builtin_isnan(ph.br)) {
                     // the ball step_path asked for at its last scattering
-                    for (int k = 0; k < 3; ++k) ph.bc[k] = (float)ph.pos[k];
-                    ph.br = lone_ball_radius(P.g, ph.which, ph.bc[0], ph.bc[1], ph.bc[2]);
+                    if (kBatch) {
+                        const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
+                        float d = 3.0e38f;
+                        if (lc_grain == ph.which) {
+                            const float dx = px - lc_x, dy = py - lc_y, dz = pz - lc_z;
+                            d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+                        }
+                        if (!(d <= 0.5f * lc_r)) {
+                            // a new list around this point (a smaller one if it would hold more
+                            // than 64 surfaces)
+                            lc_grain = -1;
+                            float R = fmaxf(kLocalMfps * (float)ph.mfp, kLocalMin);
+                            for (int t = 0; t < 3 && lc_grain < 0; ++t, R *= 0.25f) {
+                                const int n = wave_local_bvh(P.g, P.bt, ph, stk, wb, lst, px, py, pz, R);
+                                if (n >= 0) {
+                                    lc_grain = ph.which;
+                                    lc_n = n;
+                                    lc_x = px;
+                                    lc_y = py;
+                                    lc_z = pz;
+                                    lc_r = R;
+                                    d = 0.0f;
+                                }
+                            }
+                            wave_lds_sync();
+                        }
+                        if (lc_grain == ph.which) {
+                            ph.bc[0] = px;
+                            ph.bc[1] = py;
+                            ph.bc[2] = pz;
+                            ph.br = ball_radius_of(list_ball_bound(P.g, lst, lc_n, px, py, pz, lc_r, d));
+                        } else if (!wave_ball_bvh(P.g, P.bt, ph, stk, wb)) {
+                            ball_setup_bvh(P.g, P.bt, ph, stk, oc);
+                        }
+                    } else {
+                        for (int k = 0; k < 3; ++k) ph.bc[k] = (float)ph.pos[k];
+                        ph.br = lone_ball_radius(P.g, ph.which, ph.bc[0], ph.bc[1], ph.bc[2]);
+                    }
                 }
                 if (in_ball(ph)) tail_ball_run<kPrec, kAdj, false>(P.g, P.event_limit, rng, ph, al, cb);
+            }
+            if (kBatch && !in_ball(ph)) {
+                double d;
+                int b;
+                bool near = false;
+                if (lc_grain == ph.which) {
+                    // the flight stays inside the list's region (the margins of a safety ball)
+                    const float dx = (float)ph.pos[0] - lc_x, dy = (float)ph.pos[1] - lc_y,
+                                dz = (float)ph.pos[2] - lc_z;
+                    near = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) + (float)ph.fp * (1.0f + 1.0e-6f) <
+                           ball_radius_of(lc_r);
+                }
+                if (near) {
+                    list_scan(P.g, lst, lc_n, ph, &d, &b, oc);
+                    pre_best = b;
+                    pre_dmin = d;
+                } else if (wave_scan_bvh(P.g, P.bt, ph, &d, &b, oc, stk, wb, ph.fp)) {
+                    pre_best = b;
+                    pre_dmin = d;
+                }
             }
             if (!kBatch && !in_ball(ph))
                 pre_best = P.g.lane_scan ? lone_scan_planes(P.g, ph.which, ph.pos, ph.dir, &pre_dmin, oc)
