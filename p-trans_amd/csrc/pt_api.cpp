@@ -889,9 +889,10 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
     P.first_ids = nullptr;
     P.first_cap = 0;
     if (!tail_mode(mode, precision) || c->count_ops || c->tail_max <= 0) return PT_OK;
-    if (mode == TRACE_ALIAS || mode == TRACE_ADJOINT || mode == TRACE_GRAINS) {
-        // the list covers the last quarter of the ids: room for total / 1024 paths (at least
-        // 65536, at most 2^24)
+    if (mode == TRACE_ALIAS || mode == TRACE_ADJOINT || mode == TRACE_GRAINS || mode == TRACE_BATCH) {
+        // the list covers the last quarter of the ids (batch: all of them, its queue runs the
+        // structures in no order the ids tell): room for total / 1024 paths (at least 65536,
+        // at most 2^24)
         const int64_t fcap = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(65536, total / 1024));
         if (c->first_alloc < fcap) {
             if (c->first_ids) PT_HIP(hipFree(c->first_ids));
@@ -902,7 +903,7 @@ int tail_prepare(pt_ctx *c, int mode, int precision, int64_t total, int ngroup, 
         }
         P.first_ids = c->first_ids;
         P.first_cap = fcap;
-        P.first_from = total - total / 4;
+        P.first_from = mode == TRACE_BATCH ? 0 : total - total / 4;
         P.first_mfp = kFirstMfpFrac * std::min(P.g.box[0], std::min(P.g.box[1], P.g.box[2]));
     }
     const int64_t cap = std::min<int64_t>(c->tail_max, std::max<int64_t>(65536, total / 256));
@@ -2085,6 +2086,8 @@ int pt_batch_trace_graph(pt_ctx *c, pt_batch *B, const pt_modes *M, int32_t prec
     int blocks = 0;
     int rc = batch_prepare(B, M, precision, P, &lds, &blocks);
     if (rc) return rc;
+    // the graph has no pre-pass node: its queue runs the paths in order
+    P.first_ids = nullptr;
     if (stats) memset(stats, 0, sizeof *stats);
     if (!B->exec || B->graph_modes != M || B->graph_prec != precision ||
         B->graph_blocks != blocks || B->graph_lds != lds || B->graph_tail != P.tail) {

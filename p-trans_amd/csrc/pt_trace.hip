@@ -2081,6 +2081,23 @@ __device__ int wave_local_bvh(const DevGeom &g, const DevBatch &bt, const Path &
     return n;
 }
 
+// Lower bound on the distance from a point to a polygon: the larger of the distance to its
+// plane (surf_bound) and to the box of its vertices (FP32); other shapes: surf_bound.
+__device__ inline float surf_bound_box(const DevGeom &g, const DSurf &s, float px, float py, float pz)
+{
+    const float b = surf_bound(s, g.vtx, px, py, pz);
+    if (s.kind != PT_SHAPE_POLYGON) return b;
+    const double *v = g.vtx + 3 * (int64_t)s.voff;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int i = 0; i < s.nv; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float x = (float)v[3 * i + k];
+            lo[k] = fminf(lo[k], x);
+            hi[k] = fmaxf(hi[k], x);
+        }
+    return fmaxf(b, __builtin_amdgcn_sqrtf(box_dist2(lo, hi, px, py, pz)));
+}
+
 // Lower bound on the distance from p to the box walls and the surfaces of the grain, from a
 // local list of centre c and radius R (d: |p - c|, below R): the listed surfaces' bounds,
 // and R - d for the others.
@@ -2091,7 +2108,7 @@ __device__ inline float list_ball_bound(const DevGeom &g, const int *lst, int n,
     float r = fminf(fminf(fminf(px, (float)g.box[0] - px), fminf(py, (float)g.box[1] - py)),
                     fminf(pz, (float)g.box[2] - pz));
     r = fmaxf(fminf(r, R - d), 0.0f);
-    float b = lane < n ? surf_bound(g.surf[lst[lane]], g.vtx, px, py, pz) : 3.0e38f;
+    float b = lane < n ? surf_bound_box(g, g.surf[lst[lane]], px, py, pz) : 3.0e38f;
     for (int m = 1; m < 64; m <<= 1) b = fminf(b, __shfl_xor(b, m, 64));
     return fminf(r, b);
 }
@@ -2617,12 +2634,13 @@ trace_kernel(TraceParams P)
     // the queue: ids below first_from, the nfirst listed paths, then the ids from first_from
     // on but those
     int64_t nfirst = -1;
-    if (!kBatch && P.first_ids) {
+    bool first_mode = false, first_done = false;   // batch: the wave's chunk is of the list
+    if (P.first_ids) {
         const int64_t n = (int64_t)P.queue[kCtrFirstN];
         nfirst = n <= P.first_cap ? n : -1;
     }
     nfirst = uniform64(nfirst);
-    const int64_t qtotal = P.total + (nfirst > 0 ? nfirst : 0);
+    const int64_t qtotal = P.total + (!kBatch && nfirst > 0 ? nfirst : 0);
 
     // long random walks go to the tail kernel (accelerated precisions; the op-counting
     // kernels keep them)
@@ -2632,9 +2650,19 @@ trace_kernel(TraceParams P)
         // refill in batches: start_path is long, so issue it for many lanes at once
         if (need != 0ull && !exhausted && (__popcll(need) >= P.refill_min || need == ~0ull)) {
             if (chunk_next >= chunk_end && kBatch) {
-                // the queue of this wave's XCD first (hardware register XCC_ID), then the others
+                // the listed paths first (their own queue, word 0), then the queue of this
+                // wave's XCD (hardware register XCC_ID), then the others
                 long long b0 = -1, e0 = -1;
-                if (lane == 0) {
+                int fl = 0;
+                if (lane == 0 && nfirst > 0 && !first_done) {
+                    const unsigned long long b = atomicAdd(P.queue, (unsigned long long)P.chunk);
+                    if ((int64_t)b < nfirst) {
+                        b0 = (long long)b;
+                        e0 = b0 + P.chunk < nfirst ? b0 + P.chunk : nfirst;
+                        fl = 1;
+                    }
+                }
+                if (lane == 0 && !fl) {
                     const int home = __builtin_amdgcn_s_getreg(20 | (3 << 11)) & (kXcdQueues - 1);
                     for (int t = 0; t < kXcdQueues; ++t) {
                         const int q = (home + t) & (kXcdQueues - 1);
@@ -2650,6 +2678,8 @@ trace_kernel(TraceParams P)
                 }
                 b0 = __shfl(b0, 0, 64);
                 e0 = __shfl(e0, 0, 64);
+                first_mode = __builtin_amdgcn_readfirstlane(fl) != 0;
+                if (!first_mode) first_done = true;
                 chunk_next = uniform64(b0 < 0 ? P.total : b0);
                 chunk_end = uniform64(e0 < 0 ? P.total : e0);
                 if (b0 < 0) exhausted = true;
@@ -2669,13 +2699,13 @@ trace_kernel(TraceParams P)
                 if (!active && rank < avail) {
                     const int64_t v = chunk_next + rank;
                     const int64_t a = P.first_from;
-                    const int64_t pid = kBatch || nfirst <= 0 || v < a
-                                            ? v
-                                            : (v < a + nfirst ? P.first_ids[v - a] : v - nfirst);
+                    const int64_t pid = kBatch ? (first_mode ? P.first_ids[v] : v)
+                                               : (nfirst <= 0 || v < a ? v
+                                                                       : (v < a + nfirst ? P.first_ids[v - a] : v - nfirst));
                     al.nscat = 0;
                     al.len = 0.0;
                     const bool started = begin_path<kMode, kPrec>(P, pid, rng, ph, &mode);
-                    if (!kBatch && nfirst >= 0 && v >= a + nfirst && ph.mfp < P.first_mfp) {
+                    if (nfirst >= 0 && (kBatch ? !first_mode : v >= a + nfirst) && ph.mfp < P.first_mfp) {
                         // a listed path: it ran from the list
                     } else if (started) {
                         active = true;
