@@ -27,9 +27,11 @@ Other legs of the same line (reported, not `value`):
            forward and likelihood-ratio adjoint (12 specularity + 6 branch-scale derivatives
            of kappa), kappa-gradient evaluations/s;
   batch    the screening configuration: 256 jittered 4 x 4 x 4 simple-cubic grain structures
-           of 2e3 - 1e5 triangles, one graph-replayed persistent launch with BVH scans, tile
-           and leaf size from the autotuner, p50/p99 per-structure completion, per-structure
-           kappa against the bit-identical precision.
+           of 2e3 - 1e5 triangles, one persistent launch per replay (fresh keys) with BVH
+           scans and 32 live tail workgroups beside it (--batch-graph: HIP-graph replays,
+           whose tail kernel runs after the persistent one), tile and leaf size from the
+           autotuner, p50/p99 per-structure completion, per-structure kappa against the
+           bit-identical precision.
 """
 from __future__ import annotations
 
@@ -468,10 +470,11 @@ def adjoint_leg(ctx, npaths: int, cpu, table_seed: int = 20):
 # ------------------------------------------------------------------------------ batch leg
 
 def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table_seed: int = 20,
-              leaf: int = 0):
-    """C5 screening batch: `replays` graph replays of paths_per_struct paths per structure
-    (each replay a fresh Philox key), FP32 trace, and the same replays in the bit-identical
-    precision (the FP64 results) for the per-structure kappa error."""
+              leaf: int = 0, graph: bool = False):
+    """C5 screening batch: `replays` launches of paths_per_struct paths per structure (each
+    replay a fresh Philox key; graph: HIP-graph replays), FP32 trace, and the same replays
+    in the bit-identical precision (the FP64 results) for the per-structure kappa error.
+    The tail kernel runs 32 live workgroups beside the persistent one (plain launches)."""
     from ptrans import autotune
     from ptrans.trace import Batch
     t0 = time.perf_counter()
@@ -497,10 +500,11 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     info = b.info()
     say("batch: built", json.dumps(info), "in %.1f s" % t_build)
     b.set_specularity(np.full(nstruct, 0.3))
+    ctx.set_tail(500, 1 << 20, 32)
     ctx.set_seed(4242)
-    b.trace(m, precision=PREC_FP32_RETEST, graph=True)                     # graph build
+    b.trace(m, precision=PREC_FP32_RETEST, graph=graph)                    # warm-up / graph build
     ctx.set_seed(4243)
-    pilot, _, _ = b.trace(m, precision=PREC_FP32_RETEST, graph=True)
+    pilot, _, _ = b.trace(m, precision=PREC_FP32_RETEST, graph=graph)
     base = pilot[:, 1] / np.maximum(pilot[:, 0], 1.0)
     acc = {PREC_FP32_RETEST: np.zeros((nstruct, 5)), PREC_FP32_FILTER: np.zeros((nstruct, 5))}
     kern = {PREC_FP32_RETEST: [], PREC_FP32_FILTER: []}
@@ -509,13 +513,14 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
         for r in range(replays):
             ctx.set_seed(5000 + r)
             t0 = time.perf_counter()
-            sums, fin, st = b.trace(m, precision=prec, baseline=base, graph=True)
+            sums, fin, st = b.trace(m, precision=prec, baseline=base, graph=graph)
             if prec == PREC_FP32_RETEST:
                 walls.append(time.perf_counter() - t0)
                 fins.append(fin)
             acc[prec] += sums
             kern[prec].append(st["kernel_ms"])
         say("batch: precision %d, %d replays, %.1f ms per replay" % (prec, replays, np.mean(kern[prec])))
+    ctx.set_tail()
     # the FP64 kernel itself on every 16th structure (1e4 paths each): its sums equal the
     # filter's on the same paths, and its rate
     sub = geos[::16]
@@ -565,8 +570,9 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     out = {
         "workload": "%d jittered 4x4x4 simple-cubic grain structures (1000 nm cube, +-5 nm "
                     "jitter, Gamma law, p = 0.3), %d - %d triangles each, 3x1000 modes, "
-                    "%d paths per structure per launch x %d graph replays, FP32 trace"
-                    % (nstruct, min(tris), max(tris), paths_per_struct, replays),
+                    "%d paths per structure per launch x %d %s, FP32 trace"
+                    % (nstruct, min(tris), max(tris), paths_per_struct, replays,
+                       "graph replays" if graph else "launches (32 live tail workgroups)"),
         "structures": nstruct, "triangles": int(sum(tris)), "surfaces": info["surfaces"],
         "bvh_nodes": info["nodes"], "bvh_depth": info["depth"],
         "paths_per_structure": int(paths_per_struct * replays),
@@ -651,7 +657,9 @@ def main():
     ap.add_argument("--batch-structures", type=int, default=256, help="0: skip the batch leg")
     ap.add_argument("--batch-paths", type=float, default=1e5, help="paths per structure per launch")
     ap.add_argument("--batch-replays", type=int, default=1,
-                    help="graph replays of the batch (fresh keys): 100 x 1e5 = the configured 1e7")
+                    help="launches of the batch (fresh keys): 100 x 1e5 = the configured 1e7")
+    ap.add_argument("--batch-graph", action="store_true",
+                    help="replay the batch as a HIP graph (its tail kernel after the persistent one)")
     ap.add_argument("--headline-only", action="store_true", help="skip every leg")
     ap.add_argument("--only", choices=("film", "adjoint", "batch"), default=None,
                     help="run one leg alone, without the headline (profiling runs)")
@@ -690,7 +698,8 @@ def main():
                 res = adjoint_leg(ctx, int(args.adjoint_paths), cpu_leg)
             else:
                 res = batch_leg(ctx, args.batch_structures, int(args.batch_paths),
-                                args.batch_replays, cpu_leg, leaf=args.batch_leaf)
+                                args.batch_replays, cpu_leg, leaf=args.batch_leaf,
+                                graph=args.batch_graph)
             print(json.dumps({"leg": args.only, "result": res}))
             return
         head = headline(ctx, npaths, args.steps, args.warmup, int(args.count_paths),
@@ -703,7 +712,8 @@ def main():
                 say("adjoint: %.1f ms" % adj["adjoint_kernel_ms"])
             if args.batch_structures > 0:
                 bat = batch_leg(ctx, args.batch_structures, int(args.batch_paths),
-                                args.batch_replays, cpu_leg, leaf=args.batch_leaf)
+                                args.batch_replays, cpu_leg, leaf=args.batch_leaf,
+                                graph=args.batch_graph)
     dt = head.pop("dt")
     value = npaths * args.steps / dt
     cpu = head.pop("cpu_baseline")

@@ -35,11 +35,11 @@ def kernel_stats(db: str):
             for r in rows]
 
 
-def launch_spans(db: str, subs, lead_ns: float = 2.0e6):
+def launch_spans(db: str, subs, lead_ns: float = 20.0e6):
     """Average span (ns) of the launches made of the kernels whose names contain one of
     subs, and the number of launches.  Each dispatch of the first kernel marks a launch,
-    which holds the dispatches that start from lead_ns before it (a kernel enqueued just
-    ahead of it, on another stream) to lead_ns before the next launch's mark."""
+    which holds the dispatches that start from lead_ns before it (kernels enqueued ahead of
+    it: a pre-pass, a kernel on another stream) to lead_ns before the next launch's mark."""
     con = sqlite3.connect(db)
     rows = con.execute("select name, start, end from kernels order by start").fetchall()
     rows = [r for r in rows if any(sub in r[0] for sub in subs)]

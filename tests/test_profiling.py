@@ -42,7 +42,7 @@ def test_launch_span_of_concurrent_kernels(tmp_path):
             ("other", 61 * ms, 90 * ms),
             ("side", 99 * ms, 120 * ms), ("mark", 100 * ms, 140 * ms), ("after", 141 * ms, 150 * ms)]
     db = _kt(tmp_path / "kt.db", rows)
-    avg, n = profiling.launch_spans(db, ["mark", "side", "after"])
+    avg, n = profiling.launch_spans(db, ["mark", "side", "after"], lead_ns=2 * ms)
     assert n == 2
     assert avg == pytest.approx((60 * ms + 51 * ms) / 2)
     # a sum of the kernels' durations would count the overlap twice
