@@ -115,3 +115,46 @@ def test_gpu_many_candidates_tail_scan_equals_oracle(tail_ctx):
     print("events per path", st["events"] / st["paths"], "fallbacks", st["fallbacks"], flush=True)
     np.testing.assert_array_equal(gp, mp)
     np.testing.assert_array_equal(gt, mt)
+
+
+_BATCH_REF = {}      # oracle sums of the batch case, shared by its parameters
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tail_ctx", [TAILS[0], TAILS[1], TAILS[4]], ids=["default", "all", "after"],
+                         indirect=True)
+def test_gpu_batch_long_walks_equal_oracle(tail_ctx, request):
+    """Mean free paths of 1.2 - 4 nm in two structures of the screening generator (1,920 and
+    12,000 triangles; walks of up to 7e5 events): the paths below 2 nm start first (their
+    own queue), and the long walks run in the tail kernel, beside the persistent kernel and
+    after it, with local candidate lists, scans limited to the free path and wave-cooperative
+    traversals.  Per-structure path counts and sums of T / mfp equal the oracle's
+    brute-force trace of each structure alone, in the bit-identical FP32 filter; with the
+    default hand-over also as a graph replay (no first pass, the tail kernel after the
+    persistent one)."""
+    from test_batch import _oracle_struct_sums
+    from ptrans.trace import Batch
+    geos = [gen.sc_lattice_flat(seed=s, k=4, m=m_, jitter=5.0, trans_kind=2, trans_param=2.0, p=0.3)
+            for s, m_ in ((402, 2), (415, 5))]
+    t = synthetic_table(2, 4, 9, mfp_min_nm=1.2, mfp_max_nm=4.0)
+    omega = t.omega_cm() * 0.03 * M_TWOPI * 1.0e12
+    prob, alias = O.build_alias(_weights(omega, t.vel, t.mfp_nm, t.weight))
+    mods = (omega, t.vel, t.mfp_nm, t.weight, prob, alias)
+    npaths = [4000, 1000]
+    tail_ctx.set_seed(51)
+    m = tail_ctx.modes(omega=omega, vel=t.vel, mfp=t.mfp_nm, weight=t.weight)
+    b = Batch(tail_ctx, geos, npaths, leaf_size=2, grid=32)
+    runs = [b.trace(m, precision=FILTER)]
+    if request.node.callspec.id == "default":
+        runs.append(b.trace(m, precision=FILTER, graph=True))
+    st = runs[0][2]
+    print("events per path", st["events"] / st["paths"], flush=True)
+    assert st["events"] / st["paths"] > 100
+    for s in range(2):
+        if s not in _BATCH_REF:
+            _BATCH_REF[s] = _oracle_struct_sums(geos[s], mods, 51, b.path_off[s], npaths[s])
+        n, sx, _nt = _BATCH_REF[s]
+        for sums, _fin, _st in runs:
+            assert sums[s, 0] == n, (s, sums[s, 0], n)
+            assert abs(sums[s, 1] - sx) <= 1e-12 * max(sx, 1e-300), (s, sums[s, 1], sx)
+    b.close()
