@@ -444,7 +444,10 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
             const pt_surface &s = d->surfaces[cand_idx[j]];
             CandPlane &q = cplane[j];
             memset(&q, 0, sizeof q);
-            for (int k = 0; k < 3; ++k) q.n[k] = s.normal[k];
+            for (int k = 0; k < 3; ++k) {
+                q.n[k] = s.normal[k];
+                q.nf[k] = (float)s.normal[k];
+            }
             if (s.kind == PT_SHAPE_POLYGON)
                 for (int k = 0; k < 3; ++k) q.v1[k] = d->vertices[3 * s.vertex_offset + k];
             q.idx = cand_idx[j];

@@ -32,8 +32,8 @@ static_assert(sizeof(DSurf) == 112, "DSurf layout");
 struct alignas(16) CandPlane {
     double n[3];        // the surface's normal (as in DSurf)
     double v1[3];       // its first vertex
+    float nf[3];        // the normal rounded to FP32 (the filter's)
     int32_t idx;
-    int32_t pad[3];
 };
 static_assert(sizeof(CandPlane) == 64, "CandPlane layout");
 

@@ -644,15 +644,26 @@ __device__ inline int filt_polygon(VT *vtx, const DSurf &s, const double *rs, co
 // is still due (surface indices stay below 2^30).
 constexpr int kPlaneBit = 1 << 30;
 
+// The intervals are FP32; an exact FP64 distance enters as the FP32 interval two units in
+// the last place around it (it lies inside), and is kept for the winner.
 struct Pick {
-    double hi_min, lo1, lo2;
+    float hi_min, lo1, lo2;
     int idx_hi, idx_lo1;
     double exact_l;       // FP64 distance of idx_hi when it was evaluated exactly
     bool exact;
-    __device__ void init() { hi_min = lo1 = lo2 = 1.0e300; idx_hi = idx_lo1 = -1; exact = false; }
-    __device__ void add(int idx, double lo, double hi, bool is_exact)
+    __device__ void init() { hi_min = lo1 = lo2 = 3.0e38f; idx_hi = idx_lo1 = -1; exact = false; }
+    __device__ void add(int idx, float lo, float hi)
     {
-        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = is_exact; exact_l = lo; }
+        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = false; }
+        if (lo < lo1) { lo2 = lo1; lo1 = lo; idx_lo1 = idx; }
+        else if (lo < lo2) lo2 = lo;
+    }
+    __device__ void add_exact(int idx, double l)
+    {
+        const float f = (float)l;
+        const float e = fabsf(f) * 2.4e-7f + 1.0e-37f;
+        const float lo = f - e, hi = f + e;
+        if (hi < hi_min) { hi_min = hi; idx_hi = idx; exact = true; exact_l = l; }
         if (lo < lo1) { lo2 = lo1; lo1 = lo; idx_lo1 = idx; }
         else if (lo < lo2) lo2 = lo;
     }
@@ -696,11 +707,11 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
             const int r = filt_box_axis(g, a, rs, nd, &w, &lo, &hi);
             oc.flops += 10;
             if (r == F_HIT) {
-                pk.add(w, lo, hi, false);
+                pk.add(w, lo, hi);
             } else if (r == F_UNSURE) {
                 double l;
                 ++oc.st[256 * 6];
-                if (surface_distance(g, g.surf[w], rs, nd, &l, oc)) pk.add(w, l, l, true);
+                if (surface_distance(g, g.surf[w], rs, nd, &l, oc)) pk.add_exact(w, l);
             }
         }
     }
@@ -714,12 +725,12 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
             if (j < j1) {
                 const CandPlane c = g.cand_plane[j];
                 float lo = 0.f, hi = 0.f;
-                const int r = filt_plane(c.n, c.v1, rs, nd, &lo, &hi);
+                const int r = filt_plane(c.nf, c.v1, rs, nd, &lo, &hi);
                 oc.flops += 12;
                 double l;
-                if (r == F_HIT) pk.add(c.idx | kPlaneBit, lo, hi, false);
+                if (r == F_HIT) pk.add(c.idx | kPlaneBit, lo, hi);
                 else if (r == F_UNSURE && plane_hit(c.n, c.v1, rs, nd, &l, oc))
-                    pk.add(c.idx | kPlaneBit, l, l, true);
+                    pk.add_exact(c.idx | kPlaneBit, l);
             }
         }
     }
@@ -741,7 +752,7 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                     if (r == F_HIT) {
                         // a polygon's plane interval bounds its distance from below (the hit
                         // point may still lie outside the polygon: the winner's test below)
-                        pk.add(s.kind == PT_SHAPE_POLYGON ? idx | kPlaneBit : idx, lo, hi, false);
+                        pk.add(s.kind == PT_SHAPE_POLYGON ? idx | kPlaneBit : idx, lo, hi);
                     } else if (r == F_UNSURE) {
                         if (s.kind != PT_SHAPE_POLYGON && q0 < 0) {
                             q0 = idx;
@@ -753,11 +764,11 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
                             // into a hit (the in-polygon test below)
                             double l;
                             if (polygon_plane((cdouble *)g.vtx, s, rs, nd, &l, oc))
-                                pk.add(idx | kPlaneBit, l, l, true);
+                                pk.add_exact(idx | kPlaneBit, l);
                         } else {
                             double l;
                             ++oc.st[256 * 6];
-                            if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add(idx, l, l, true);
+                            if (surface_distance_c(g, s, rs, nd, &l, oc)) pk.add_exact(idx, l);
                         }
                     }
                 }
@@ -770,7 +781,7 @@ __device__ bool scan_fast(const DevGeom &g, Path &ph, double *dmin, int *best, O
         if (q0 >= 0) {
             double l;
             ++oc.st[256 * 6];
-            if (surface_distance(g, g.surf[q0], rs, nd, &l, oc)) pk.add(q0, l, l, true);
+            if (surface_distance(g, g.surf[q0], rs, nd, &l, oc)) pk.add_exact(q0, l);
             q0 = q1;
             q1 = -1;
         }
