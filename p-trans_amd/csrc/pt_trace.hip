@@ -524,7 +524,7 @@ __device__ int filt_cylinder(const DSurf &s, const double *rs, const double *nd,
     if (D <= mD) return F_UNSURE;
     const float sq = fsqrt(D);
     const float mrd = kMarg * fsqrt(rr);
-    const float ml = mrd + mD / sq + kMarg * (fabsf(rd) + sq) + 1e-30f;
+    const float ml = mrd + mD * frcp(sq) * 1.0001f + kMarg * (fabsf(rd) + sq) + 1e-30f;
     const float l1 = rd - sq, l2 = rd + sq;
     float l;
     if (l1 > ml) {
@@ -587,7 +587,7 @@ __device__ int filt_sphere(const DSurf &s, const double *rs, const double *nd, f
     if (D < -mD) return F_MISS;
     if (D <= mD) return F_UNSURE;
     const float sq = fsqrt(D);
-    const float ml = mrd + mD / sq + kMarg * (fabsf(rd) + sq) + 1e-30f;
+    const float ml = mrd + mD * frcp(sq) * 1.0001f + kMarg * (fabsf(rd) + sq) + 1e-30f;
     const float l1 = rd - sq, l2 = rd + sq;
     float l;
     if (l1 > ml) l = l1;
@@ -624,9 +624,12 @@ __device__ inline int filt_plane(NT *n, VT *v, const double *rs, const double *n
     const float ea = kMarg * (fabsf(a0) + fabsf(a1) + fabsf(a2)) + 1e-30f;
     if (a < -ea) return F_MISS;
     if (a <= ea) return F_UNSURE;
-    const float r = ea / a + ec / c + kMarg;     // relative error bound of the quotient
+    // relative error bound of the quotient (its own terms from v_rcp_f32, ~1 ulp, with a
+    // 1e-4 relative allowance: a bound needs no correctly rounded division)
+    const float rc = frcp(c);
+    const float r = (ea * frcp(a) + ec * rc) * 1.0001f + kMarg;
     if (!(r < 1.0e-3f)) return F_UNSURE;
-    const float L = a * frcp(c);
+    const float L = a * rc;
     *lo = L * (1.0f - r);
     *hi = L * (1.0f + r) + 1e-30f;
     return F_HIT;
