@@ -629,8 +629,10 @@ int pt_geom_create(pt_ctx *c, const pt_geom_desc *d, pt_geom **out)
     g.region_margin = (float)(1.0e-2 + 2.0e-5 * std::max(d->box[0], std::max(d->box[1], d->box[2])));
     g.start_map = d_sm;
     g.start_grid = d_sm ? kStartGrid : 0;
-    g.start_h[0] = start_h[0];
-    g.start_h[1] = start_h[1];
+    // the cell of a start point from a product (a rounding away from the quotient; the
+    // certified cells are widened by 1e-6 of a side)
+    g.start_ih[0] = start_h[0] > 0.0 ? 1.0 / start_h[0] : 0.0;
+    g.start_ih[1] = start_h[1] > 0.0 ? 1.0 / start_h[1] : 0.0;
     for (int k = 0; k < 3; ++k) g.box[k] = d->box[k];
     g.scope[0] = d->scope[0];
     g.scope[1] = d->scope[1];
@@ -776,16 +778,21 @@ int pt_modes_create(pt_ctx *c, const pt_modes_desc *d, pt_modes **out)
     M->nb = d->nbranch;
     M->nk = d->nkpoint;
     M->has_alias = alias;
-    double *d_om = nullptr, *d_mfp = nullptr, *d_prob = nullptr;
+    double *d_om = nullptr, *d_mfp = nullptr, *d_imfp = nullptr, *d_prob = nullptr;
     int32_t *d_al = nullptr;
+    // 1 / mfp on the host: IEEE division rounds as the device's would
+    std::vector<double> imfp(mfp.size());
+    for (size_t i = 0; i < mfp.size(); ++i) imfp[i] = 1.0 / mfp[i];
     int rc = upload(om, &d_om, c->stream);
     if (!rc && alias) rc = upload(mfp, &d_mfp, c->stream);
+    if (!rc && alias) rc = upload(imfp, &d_imfp, c->stream);
     if (!rc && alias) rc = upload(prob, &d_prob, c->stream);
     if (!rc && alias) rc = upload(al, &d_al, c->stream);
-    M->owned = {d_om, d_mfp, d_prob, d_al};
+    M->owned = {d_om, d_mfp, d_imfp, d_prob, d_al};
     if (rc) { pt_modes_destroy(M); return rc; }
     M->dm.omega = d_om;
     M->dm.mfp = d_mfp;
+    M->dm.inv_mfp = d_imfp;
     M->dm.prob = d_prob;
     M->dm.alias = d_al;
     M->dm.scale = nullptr;

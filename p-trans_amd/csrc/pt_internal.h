@@ -65,7 +65,7 @@ struct DevGeom {
     // the full searches (findNearestIncidentGrain and the start-grain check)
     const int32_t *start_map;
     int32_t start_grid;
-    double start_h[2];         // cell sides in y and z
+    double start_ih[2];        // inverse cell sides in y and z
     double box[3];
     double scope[2];
     int32_t ngrain;
@@ -81,6 +81,7 @@ struct DevGeom {
 struct DevModes {
     const double *omega;       // (nb, nk) column-major
     const double *mfp;         // (nb, nk) mean free paths [nm] (alias mode)
+    const double *inv_mfp;     // 1.0 / mfp, correctly rounded (alias mode)
     const double *prob;        // Vose table (alias mode)
     const int32_t *alias;
     const double *scale;       // per-branch scattering-rate scale or nullptr

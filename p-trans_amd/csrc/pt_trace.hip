@@ -1536,7 +1536,7 @@ __device__ bool start_path(const DevGeom &g, double cos_t, double sin_t, bool la
     if (g.start_map) {
         // a cell where both start-grain searches certainly return the same grain
         const int G = g.start_grid;
-        int iy = (int)(ph.pos[1] / g.start_h[0]), iz = (int)(ph.pos[2] / g.start_h[1]);
+        int iy = (int)(ph.pos[1] * g.start_ih[0]), iz = (int)(ph.pos[2] * g.start_ih[1]);
         iy = iy < 0 ? 0 : (iy >= G ? G - 1 : iy);
         iz = iz < 0 ? 0 : (iz >= G ? G - 1 : iz);
         which = g.start_map[iy * G + iz];
@@ -2427,9 +2427,8 @@ __device__ inline bool begin_path(const TraceParams &P, int64_t pid, Rng &rng, P
             ph.sid = lo;
         }
         const int m = sample_mode(P.m, rng);
-        const int ib = m % P.m.nb;
         ph.omega = P.m.omega[m];
-        ph.mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[ib] : P.m.mfp[m];
+        ph.mfp = P.m.scale ? P.m.mfp[m] / P.m.scale[m % P.m.nb] : P.m.mfp[m];
         *mode = m;
         return start_path<kPrec>(P.g, 0.0, 0.0, true, rng, ph,
                                  kMode == TRACE_BATCH ? &P.bt : nullptr);
@@ -2456,7 +2455,7 @@ __device__ inline void end_path(const TraceParams &P, const Path &ph, const Rng 
         if (transmitted) atomicAdd(P.mode_trans + mode, 1ull);
     }
     if (kBatch) {
-        const double x = transmitted ? 1.0 / P.m.mfp[mode] : 0.0;
+        const double x = transmitted ? P.m.inv_mfp[mode] : 0.0;
         const int sid = ph.sid;
         double *acc = P.bt.sums + kBatchSums * sid;
         atomicAdd(acc + 0, 1.0);
@@ -2724,7 +2723,7 @@ trace_kernel(TraceParams P)
                     } else if (started) {
                         active = true;
                         fresh = true;
-                        if (kAdj & 1) al.fw = 1.0 / P.m.mfp[mode];
+                        if (kAdj & 1) al.fw = P.m.inv_mfp[mode];
                     } else {
                         if (is_grid_mode(kMode)) P.outcome[pid] = OUT_MISMATCH;
                         st[0] += 1ull;
