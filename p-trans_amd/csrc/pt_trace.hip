@@ -2746,8 +2746,11 @@ trace_kernel(TraceParams P)
             // many above the priority mean free path), or one of the last few of the wave once
             // the queue is empty, goes to the tail kernel
             const bool few = exhausted && __popcll(__ballot(active)) <= kFewPaths;
-            const int64_t hand_at = ph.mfp < P.tail_prio_mfp || exhausted ? P.tail_events
-                                                                          : kTailLongFactor * P.tail_events;
+            // (batch: an iteration of the persistent kernel is ~50 times longer than on the
+            // Voronoi cells, so the paths of the queue's list go after an eighth of tail_events)
+            const int64_t hand_at = ph.mfp < P.tail_prio_mfp
+                                        ? (kBatch && ph.mfp < P.first_mfp ? (P.tail_events + 7) / 8 : P.tail_events)
+                                        : (exhausted ? P.tail_events : kTailLongFactor * P.tail_events);
             if (active && ph.end_side == 0 && !(ph.flags & kPathStay) && (few || ph.events >= hand_at)) {
                 if (handoff<kMode>(P, ph, rng, al, mode)) {
                     if (top_on && (ph.flags & kPathTop)) top_unbind(stk, ph);
