@@ -134,6 +134,7 @@ struct pt_ctx {
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     int64_t live_ticks = 0;                    // 20 s of the device's wall clock
+    size_t lds_cu = 0;                         // LDS bytes per CU
 };
 
 struct pt_geom {
@@ -193,6 +194,8 @@ int pt_ctx_create(int device, uint64_t seed, pt_ctx **out)
     c->device = device;
     c->seed = seed;
     c->ncu = prop.multiProcessorCount;
+    c->lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                      : prop.sharedMemPerMultiprocessor;
     int wall_khz = 0;
     PT_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
     c->live_ticks = (int64_t)(wall_khz > 0 ? wall_khz : 100000) * 20000;
@@ -883,6 +886,8 @@ const LiveTail *live_tail(pt_ctx *c, LiveTail *lt)
     lt->join = c->join;
     lt->blocks = c->live_blocks;
     lt->ticks = c->live_ticks;
+    lt->ncu = c->ncu;
+    lt->lds_cu = c->lds_cu;
     return lt;
 }
 

@@ -289,8 +289,9 @@ struct TraceParams {
     uint32_t tail_epoch;       // this launch's number, written last into each record
     // the tail kernel beside the persistent one (on another stream): it takes records while
     // they are handed over, until all main_waves waves have ended or live_ticks of the wall
-    // clock (s_memrealtime) have passed, or after live_ticks / 20 if no wave of the
-    // persistent kernel has begun (launches serialised, as under a counter profiler)
+    // clock (s_memrealtime) have passed, or after live_ticks / 200 if not every wave of the
+    // persistent kernel has begun (launches serialised, as under a counter profiler, or
+    // workgroups of the persistent kernel waiting for the room the live ones hold)
     int32_t tail_live;
     int64_t main_waves;
     int64_t live_ticks;
@@ -343,12 +344,14 @@ inline size_t adj_lds_bytes(const TraceParams &p, int threads)
 
 // The tail kernel beside a persistent launch: `blocks` workgroups on stream `side`, forked
 // from and joined back into the launch's stream by the two events.  The persistent kernel
-// then gets 2 x blocks workgroups fewer, room for them.
+// gets as many workgroups fewer as make room for them (from both kernels' VGPRs and LDS).
 struct LiveTail {
     hipStream_t side;
     hipEvent_t fork, join;
     int blocks;
     int64_t ticks;             // the live tail kernel's wall-clock limit (s_memrealtime)
+    int ncu;                   // CUs of the device and LDS bytes per CU
+    size_t lds_cu;
 };
 
 // kernels' launch wrappers (pt_trace.hip)

@@ -134,17 +134,16 @@ __device__ inline bool polygon_plane(VT *vtx, const DSurf &s, const double *rs, 
 }
 
 template <class VT>
-__device__ inline bool polygon_contains(VT *vtx, const DSurf &s, const double *rs,
-                                        const double *nd, double l, OpCount &oc)
+__device__ inline bool polygon_contains_at(VT *v, int nv, const double *rs, const double *nd,
+                                           double l, OpCount &oc)
 {
-    VT *v = vtx + 3 * (int64_t)s.voff;
     double h[3];
     for (int k = 0; k < 3; ++k) h[k] = rs[k] + nd[k] * l;
     oc.flops += 6;
     double c1[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < s.nv; ++i) {
+    for (int i = 0; i < nv; ++i) {
         VT *vi = v + 3 * i;
-        VT *vn = v + 3 * (i + 1 == s.nv ? 0 : i + 1);
+        VT *vn = v + 3 * (i + 1 == nv ? 0 : i + 1);
         double ra[3], rb[3], ci[3];
         for (int k = 0; k < 3; ++k) { ra[k] = vn[k] - vi[k]; rb[k] = h[k] - vi[k]; }
         cross3(ra, rb, ci);
@@ -156,6 +155,13 @@ __device__ inline bool polygon_contains(VT *vtx, const DSurf &s, const double *r
         }
     }
     return true;
+}
+
+template <class VT>
+__device__ inline bool polygon_contains(VT *vtx, const DSurf &s, const double *rs,
+                                        const double *nd, double l, OpCount &oc)
+{
+    return polygon_contains_at(vtx + 3 * (int64_t)s.voff, s.nv, rs, nd, l, oc);
 }
 
 template <class VT>
@@ -2845,8 +2851,8 @@ __global__ void __launch_bounds__(256) first_scan_kernel(TraceParams P)
 // part, else of the general part, taken by a compare-and-swap on that part's next index, so
 // that it and the tail kernel after the persistent one (atomicAdd on the same index) never
 // take the same record.  Polls, sleeping, until the persistent kernel's waves have all
-// ended or the wall clock passes t_end (t_wait if none of them has begun); -1: nothing more
-// for this wave (the tail kernel after the persistent one takes what is left).
+// ended or the wall clock passes t_end (t_wait if not all of them have begun); -1: nothing
+// more for this wave (the tail kernel after the persistent one takes what is left).
 __device__ int64_t live_claim(const TraceParams &P, uint64_t t_wait, uint64_t t_end)
 {
     unsigned long long *q = P.queue;
@@ -2867,7 +2873,9 @@ __device__ int64_t live_claim(const TraceParams &P, uint64_t t_wait, uint64_t t_
             }
         }
         const uint64_t now = wall_clock64();
-        if (done || now > t_end || (now > t_wait && get_dev(q + kCtrMainStarted) == 0ull)) return -1;
+        if (done || now > t_end ||
+            (now > t_wait && get_dev(q + kCtrMainStarted) < (unsigned long long)P.main_waves))
+            return -1;
         __builtin_amdgcn_s_sleep(32);
     }
 }
@@ -2916,7 +2924,7 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
     const int64_t np = (int64_t)min(P.queue[kCtrTailP], (unsigned long long)P.tail_cap_p);
     const int64_t ng = (int64_t)min(P.queue[kCtrTailN], (unsigned long long)(P.tail_cap - P.tail_cap_p));
     const uint64_t t0 = P.tail_live ? wall_clock64() : 0ull;
-    const uint64_t t_wait = t0 + (uint64_t)P.live_ticks / 20u, t_end = t0 + (uint64_t)P.live_ticks;
+    const uint64_t t_wait = t0 + (uint64_t)P.live_ticks / 200u, t_end = t0 + (uint64_t)P.live_ticks;
     while (true) {
         int64_t slot = -1;
         if (lane == 0) {
@@ -3180,6 +3188,24 @@ __global__ void __launch_bounds__(64) replay_kernel(TraceParams P, int64_t first
     rec[i] = r;
 }
 
+// Workgroups of the persistent kernel that one tail workgroup displaces on a CU holding
+// per_cu of them: per SIMD one wave of each workgroup (256 threads: four waves), 512 VGPRs
+// per SIMD lane in granules of 8, lds_cu bytes of LDS per CU.  0 when the attributes are
+// not available.
+static int live_displace(const void *main_fn, size_t main_dyn_lds, const void *tail_fn, int per_cu,
+                         size_t lds_cu)
+{
+    hipFuncAttributes am, at;
+    if (!main_fn || !tail_fn || per_cu <= 0 || hipFuncGetAttributes(&am, main_fn) != hipSuccess ||
+        hipFuncGetAttributes(&at, tail_fn) != hipSuccess)
+        return 0;
+    const int vm = (am.numRegs + 7) / 8 * 8, vt = (at.numRegs + 7) / 8 * 8;
+    const size_t lm = am.sharedSizeBytes + main_dyn_lds, lt = at.sharedSizeBytes;
+    for (int k = per_cu; k >= 0; --k)
+        if (k * vm + vt <= 512 && k * lm + lt <= lds_cu) return per_cu - k;
+    return 0;
+}
+
 hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads, bool count,
                         size_t lds, hipStream_t s, const LiveTail *live)
 {
@@ -3190,15 +3216,20 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
                           : nullptr;
     // the live tail kernel first, on its own stream once the launch's stream has reached
     // this point (its counters are cleared); the persistent kernel leaves it the room of
-    // two of its own workgroups per workgroup (VGPRs: a tail wave needs up to 192, a trace
-    // wave 128).  Launched before the persistent kernel is resident, it gets that room;
-    // launched after, it waits for the persistent kernel's workgroups to end (either way
-    // it never holds up the persistent kernel, which waits for nothing).
-    const bool side = hand && live && live->blocks > 0 && blocks > 4 * live->blocks;
+    // as many of its own workgroups per tail workgroup as share a CU's VGPRs and LDS with
+    // it.  Launched before the persistent kernel is resident, it gets that room; launched
+    // after, it waits for the persistent kernel's workgroups to end; if the persistent
+    // kernel cannot become resident beside it, it steps aside after live_ticks / 200
+    // (the persistent kernel waits for nothing).
+    const int displace = hand && live && live->blocks > 0
+                             ? live_displace(trace_kernel_fn(mode, p.precision), lds, tail_kernel_fn(mode, p.precision),
+                                             blocks / (live->ncu > 0 ? live->ncu : 1), live->lds_cu)
+                             : 0;
+    const bool side = displace > 0 && blocks - displace * live->blocks >= blocks / 2;
     int main_blocks = blocks;
     hipError_t e = hipSuccess;
     if (side) {
-        main_blocks = blocks - 2 * live->blocks;
+        main_blocks = blocks - displace * live->blocks;
         TraceParams q = p;
         q.tail_live = 1;
         q.main_waves = (int64_t)main_blocks * (threads / 64);
