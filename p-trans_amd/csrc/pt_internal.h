@@ -227,6 +227,7 @@ constexpr int kCtrTailNextN = 19;
 constexpr int kCtrMainDone = 20;
 constexpr int kCtrFirstN = 21;       // paths the pre-pass found for the first pass
 constexpr int kCtrMainStarted = 22;  // waves of the persistent kernel that have begun
+constexpr int kCtrLiveStarted = 23;  // live tail workgroups that have begun
 // Paths whose mean free path is below this fraction of the smallest box side take records
 // in the priority part of the tail buffer, which the tail kernels serve first: their walks
 // are the longest (~(L / mfp)^2 events), so they start at once.  The other paths are handed
@@ -293,6 +294,7 @@ struct TraceParams {
     // persistent kernel has begun (launches serialised, as under a counter profiler, or
     // workgroups of the persistent kernel waiting for the room the live ones hold)
     int32_t tail_live;
+    int32_t live_gate;         // the pre-pass waits for this many live workgroups (0: none)
     int64_t main_waves;
     int64_t live_ticks;
     // the queue's list of the paths from first_from on below first_mfp (at most first_cap;

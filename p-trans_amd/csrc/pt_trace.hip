@@ -2753,9 +2753,11 @@ trace_kernel(TraceParams P)
             // the queue is empty, goes to the tail kernel
             const bool few = exhausted && __popcll(__ballot(active)) <= kFewPaths;
             // (batch: an iteration of the persistent kernel is ~50 times longer than on the
-            // Voronoi cells, so the paths of the queue's list go after an eighth of tail_events)
+            // Voronoi cells, so the paths of the queue's list that are already 20 mean free
+            // paths deep, the ones likely to walk long, go after an eighth of tail_events)
             const int64_t hand_at = ph.mfp < P.tail_prio_mfp
-                                        ? (kBatch && ph.mfp < P.first_mfp ? (P.tail_events + 7) / 8 : P.tail_events)
+                                        ? (kBatch && ph.mfp < P.first_mfp && ph.pos[0] > 20.0 * ph.mfp
+                                               ? (P.tail_events + 7) / 8 : P.tail_events)
                                         : (exhausted ? P.tail_events : kTailLongFactor * P.tail_events);
             if (active && ph.end_side == 0 && !(ph.flags & kPathStay) && (few || ph.events >= hand_at)) {
                 if (handoff<kMode>(P, ph, rng, al, mode)) {
@@ -2813,6 +2815,13 @@ __global__ void __launch_bounds__(256) first_scan_kernel(TraceParams P)
     // for one round of loads per four paths
     constexpr int kUnroll = 4;
     const int lane = threadIdx.x & 63;
+    if (P.live_gate > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the persistent kernel, which follows, must not take the room of the live tail
+        // workgroups before they are resident: wait for them, at most live_ticks / 4000
+        const uint64_t t_end = wall_clock64() + (uint64_t)P.live_ticks / 4000u;
+        while (get_dev(P.queue + kCtrLiveStarted) < (unsigned long long)P.live_gate && wall_clock64() < t_end)
+            __builtin_amdgcn_s_sleep(8);
+    }
     Rng rng;
     OpCount oc = {0ull, 0ull, 0ull, 0ull, nullptr};
     rng.oc = &oc;
@@ -2924,6 +2933,7 @@ __global__ void __launch_bounds__(256) tail_kernel(TraceParams P)
     const int64_t np = (int64_t)min(P.queue[kCtrTailP], (unsigned long long)P.tail_cap_p);
     const int64_t ng = (int64_t)min(P.queue[kCtrTailN], (unsigned long long)(P.tail_cap - P.tail_cap_p));
     const uint64_t t0 = P.tail_live ? wall_clock64() : 0ull;
+    if (P.tail_live && threadIdx.x == 0) atomicAdd(P.queue + kCtrLiveStarted, 1ull);
     const uint64_t t_wait = t0 + (uint64_t)P.live_ticks / 200u, t_end = t0 + (uint64_t)P.live_ticks;
     while (true) {
         int64_t slot = -1;
@@ -3242,7 +3252,10 @@ hipError_t launch_trace(const TraceParams &p, int mode, int blocks, int threads,
     // the pre-pass of the queue's first pass (the live tail kernel takes its CUs meanwhile:
     // once the persistent kernel is resident, no CU has room for a tail workgroup)
     if (p.first_ids) {
-        hipLaunchKernelGGL(first_scan_kernel, dim3(blocks), dim3(256), 0, s, p);
+        TraceParams q = p;
+        q.live_gate = side ? live->blocks : 0;
+        q.live_ticks = side ? live->ticks : 0;
+        hipLaunchKernelGGL(first_scan_kernel, dim3(blocks), dim3(256), 0, s, q);
         if ((e = hipGetLastError())) return e;
     }
     const dim3 gr(main_blocks), bl(threads);
