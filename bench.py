@@ -558,7 +558,8 @@ def batch_leg(ctx, nstruct: int, paths_per_struct: int, replays: int, cpu, table
     need = n * (ks / (1e-3 * k32)) ** 2
     fin = np.concatenate(fins)
     nodes, tests = cst["bvh_nodes"] / cst["paths"], cst["bvh_tests"] / cst["paths"]
-    # SURVEY's cost table: 32 B alias + mode record, 32 B per node fetch, 40 B per triangle
+    # algorithmic bytes per path: 32 B alias + mode record, 32 B per node fetch, 40 B per
+    # triangle test
     alg_bytes = 32.0 + 32.0 * nodes + 40.0 * tests
     achieved_gbs = alg_bytes * paths_launch / (kms * 1e-3) / 1e9
     prof = load_profile("trace_kernel_batch_traffic.json").get("kernels", {}).get("fp32_trace", {})
