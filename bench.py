@@ -735,7 +735,11 @@ def main():
         "data": "synthetic (seeded Voronoi geometry and bulk table)",
         "config": {"workload": head.pop("workload"), "paths_per_step": npaths,
                    "precision": "PT_PREC_FP32_FILTER: FP32 intersection bounds certified in FP64; "
-                                "every result computed in FP64, bit-identical to the FP64 trace"},
+                                "every result computed in FP64, bit-identical to the FP64 trace",
+                   "launch": "pt_trace_alias_grains: first-pass pre-pass, persistent trace kernel, "
+                             "tail kernel with 16 live workgroups beside it (long random walks "
+                             "handed over after 500 events) and after it; defaults of "
+                             "pt_ctx_set_tail and pt_ctx_set_tuning"},
         "roofline": head.pop("roofline"),
         "cpu_baseline": cpu,
         "speedup_vs_cpu_baseline": value / cpu["value"] if cpu else None,
