@@ -2894,8 +2894,11 @@ __device__ int64_t live_claim(const TraceParams &P, uint64_t t_wait, uint64_t t_
 // wave.  Every lane of the wave keeps the same copy of the path and computes the same
 // values, so the control flow is uniform and the whole wave is at the path's service:
 // its ball set-up and its scans spread the candidates over the lanes (lone_ball_radius,
-// lone_scan), its in-ball events run 64 at a time (tail_ball_run), and the rest of each
-// event runs in step_path.  Lane 0 ends the path (end_path: the same tallies as the
+// lone_scan; mesh grains: the wave's local candidate list, else BVH traversals of up to 64
+// nodes a round), its in-ball events run 64 at a time (tail_ball_run), and the rest of
+// each event runs in step_path.  Launched beside the persistent kernel (tail_live), it
+// takes records as they are handed over (live_claim); launched after it, it takes what is
+// left.  Lane 0 ends the path (end_path: the same tallies as the
 // persistent kernel; adjoint batch sums and grain sums go to HBM directly) and holds the
 // statistics.  The path takes the same steps as it would have in the persistent kernel.
 template <int kMode, int kPrec>
