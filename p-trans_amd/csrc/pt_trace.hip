@@ -1127,34 +1127,6 @@ __device__ inline float surf_bound(const DSurf &s, V *vtx, float px, float py, f
 
 __device__ inline float ball_radius_of(float r) { return r * (1.0f - 1.0e-5f) - 1.0e-2f; }
 
-__device__ void ball_setup(const DevGeom &g, Path &ph)
-{
-    const float px = (float)ph.pos[0], py = (float)ph.pos[1], pz = (float)ph.pos[2];
-    float r = 3.0e38f;
-    bool todo0 = true, todo1 = ph.which != 0;
-    while (__any(todo0 || todo1)) {
-        if (todo0 || todo1) {
-            const int mine = todo0 ? 0 : ph.which;
-            const int ig = __builtin_amdgcn_readfirstlane(mine);
-            if (mine == ig) {
-                const int j0 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig]);
-                const int j1 = __builtin_amdgcn_readfirstlane(((cint *)g.cand_off)[ig + 1]);
-                for (int j = j0; j < j1; ++j) {
-                    const int idx = __builtin_amdgcn_readfirstlane(((cint *)g.cand_idx)[j]);
-                    const DSurf s = load_surf_c(g.surf, idx);
-                    r = fminf(r, surf_bound(s, (cdouble *)g.vtx, px, py, pz));
-                }
-                if (todo0) todo0 = false;
-                else todo1 = false;
-            }
-        }
-    }
-    ph.bc[0] = px;
-    ph.bc[1] = py;
-    ph.bc[2] = pz;
-    ph.br = ball_radius_of(r);
-}
-
 // The scan order of findCollisionSurface for a path in grain `which`: the box grain's
 // candidate list, then the grain's own (RT:1989-2096).  Position k of that order is the
 // global surface index cand(k); n positions in all.
@@ -1173,9 +1145,9 @@ struct ScanOrder {
     }
 };
 
-// ball_setup for one path of a wave down to a few, with the candidates spread over the
-// lanes: the minimum of the same per-surface bounds (an exact, order-free FP32 minimum),
-// so the ball equals ball_setup's.  Uniform control flow, every lane active, arguments
+// The safety-ball radius of one path of a wave down to a few: the minimum of surf_bound
+// over the box grain's and the path's grain's candidates, spread over the lanes (an
+// exact, order-free FP32 minimum).  Uniform control flow, every lane active, arguments
 // broadcast.
 __device__ float lone_ball_radius(const DevGeom &g, int which, float px, float py, float pz)
 {
@@ -1319,8 +1291,7 @@ __device__ inline float region_dist(const float4 &pl, float cx, float cy, float 
     return pl.x * cx + pl.y * cy + pl.z * cz - pl.w;
 }
 
-// Lower bound on the distance from a point to the surfaces of one BVH leaf, as in
-// ball_setup.
+// Lower bound on the distance from a point to the surfaces of one BVH leaf (surf_bound).
 __device__ inline float bvh_leaf_bound(const DevGeom &g, const DevBatch &bt, int first, int count,
                                        float px, float py, float pz)
 {
@@ -1330,9 +1301,9 @@ __device__ inline float bvh_leaf_bound(const DevGeom &g, const DevBatch &bt, int
     return r;
 }
 
-// ball_setup for batch structures: the box walls, then the current grain's candidates
+// The safety ball for batch structures: the box walls, then the current grain's candidates
 // through its BVH, pruning nodes whose (outward-padded) box is farther than the radius
-// found so far.  Leaf surfaces are bounded as in ball_setup.
+// found so far.  Leaf surfaces are bounded by surf_bound.
 __device__ void ball_setup_bvh(const DevGeom &g, const DevBatch &bt, Path &ph, int *stk,
                                OpCount &oc)
 {
@@ -1843,15 +1814,14 @@ __device__ bool step_path(const DevGeom &g, double max_omega, int64_t event_limi
         }
     }
     // a diffusive path (mean free path under kBallMfpFrac of the smallest box side) that
-    // scattered outside its ball takes a new ball around the scattering point
+    // scattered outside its ball asks the tail kernel (the caller that gave the scan) for a
+    // new ball around the scattering point.  The persistent kernel sets up no balls: there a
+    // lane inside its ball spares no work its wave does not do anyway for the other lanes'
+    // scans, while setting a ball up costs a pass over the candidates (polycrystal 8%, mesh
+    // batch 2.5% faster without them)
     if (kPrec != kPrecFP64) {
         const double lim = kBallMfpFrac * fmin(g.box[0], fmin(g.box[1], g.box[2]));
-        if (scattered && !in_safe && ph.mfp < lim) {
-            // a caller that gave the scan (the tail kernel) sets the ball up itself
-            if (pre_best != -2) ph.br = __builtin_nanf("");
-            else if (kBatch) ball_setup_bvh(g, *bt, ph, stk, oc);
-            else ball_setup(g, ph);
-        }
+        if (scattered && !in_safe && ph.mfp < lim && pre_best != -2) ph.br = __builtin_nanf("");
     }
     // whichV on transmission, RT:1793-1796
     if (jreflect == 0) {
