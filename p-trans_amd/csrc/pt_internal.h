@@ -98,16 +98,17 @@ enum : uint8_t {
 // TRACE_ADJOINT: the alias-mode trace plus likelihood-ratio scores of each path;
 // TRACE_BATCH: the adjoint-mode trace over a batch of mesh structures (BVH scans);
 // TRACE_GRID_TAIL: the grid trace of a diffusive sampling mean free path (one that gets
-// safety balls, below kBallMfpFrac of the smallest box side), with the few-path service at
-// the end of each wave.  The ballistic grid trace keeps the leaner TRACE_GRID kernel.
+// safety balls, below kBallMfpFrac of the smallest box side), with its long walks and the
+// last few paths of each wave handed to the tail kernel.  The ballistic grid trace keeps the
+// leaner TRACE_GRID kernel.
 // TRACE_GRAINS: the alias-mode trace plus per-grain flux tallies (no likelihood ratio).
 enum { TRACE_GRID = 0, TRACE_ALIAS = 1, TRACE_ADJOINT = 2, TRACE_BATCH = 3, TRACE_GRID_TAIL = 4,
        TRACE_GRAINS = 5 };
 
 constexpr bool is_grid_mode(int mode) { return mode == TRACE_GRID || mode == TRACE_GRID_TAIL; }
 
-// Safety balls (accelerated precisions) are set up for paths whose mean free path is below
-// this fraction of the smallest box side.
+// Safety balls (accelerated precisions) are set up, by the tail kernel only, for paths whose
+// mean free path is below this fraction of the smallest box side.
 constexpr double kBallMfpFrac = 0.05;
 
 // Binary BVH node holding both children's bounds (outward padded, FP32), so one load
