@@ -241,6 +241,8 @@ int pt_ctx_set_tuning(pt_ctx *ctx, int32_t chunk, int32_t refill_min, int32_t bl
  * built into HIP graphs (pt_batch_trace_graph) or pt_trace_alias_noalloc, whose tail kernel
  * always runs after the persistent one.
  * Defaults: 500 events, 2^20 records, 16 live workgroups.  Results do not depend on these.
+ * Mesh batches (pt_batch_trace), whose events cost more in the persistent kernel, run
+ * fastest at about 200 events and 32 live workgroups.
  * No counterpart in the reference, whose OpenMP threads each run one cell's particles to
  * the end (RT:1573). */
 int pt_ctx_set_tail(pt_ctx *ctx, int64_t events, int64_t max_records, int32_t live_blocks);
